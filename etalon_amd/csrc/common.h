@@ -1,0 +1,63 @@
+// Shared runtime state and helpers of libetalon_hip.so (gfx950 only).
+//
+// Floating-point rules for every kernel in this library: the build uses
+// -ffp-contract=off (no fused multiply-add), keeps f32 denormals, and never uses
+// fast-math, so each + - * rounds once in IEEE binary32/binary64, exactly as the
+// NumPy reference backend does (etalon/fdtd/backends/numpy_backend.py).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/etalon_hip.h"
+
+namespace etl {
+
+// Last error message of the calling thread.
+void set_error(const char* fmt, ...);
+
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    char name[256] = {0};
+    char arch[64] = {0};
+    size_t mem = 0;
+};
+Context& ctx();
+int require_init();
+// Stream-ordered zero fill with a kernel (safe inside hipGraph capture).
+int zero_async(void* dst, size_t bytes);
+
+}  // namespace etl
+
+#define ETL_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t etl_err_ = (call);                                                   \
+        if (etl_err_ != hipSuccess) {                                                   \
+            etl::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(etl_err_), \
+                           __FILE__, __LINE__);                                         \
+            return ETL_EHIP;                                                            \
+        }                                                                               \
+    } while (0)
+
+#define ETL_REQUIRE(cond, ...)          \
+    do {                                \
+        if (!(cond)) {                  \
+            etl::set_error(__VA_ARGS__); \
+            return ETL_EARG;            \
+        }                               \
+    } while (0)
+
+#define ETL_CHECK_INIT()                     \
+    do {                                     \
+        int etl_rc_ = etl::require_init();   \
+        if (etl_rc_ != ETL_OK) return etl_rc_; \
+    } while (0)
+
+// Launch check: the launch itself (configuration errors) is reported here; faults
+// inside kernels surface at the next synchronising call.
+#define ETL_LAUNCHED() ETL_HIP(hipGetLastError())

@@ -1,0 +1,633 @@
+// Fused time-stepping plans: the C ABI around te_fused.hip.
+//
+// A plan replaces the host loop of FDTDSolver.step()/run() (etalon/fdtd/solver.py:
+// 83-128,155-160).  The per-step host work of the reference — evaluating the source
+// pulse and the DFT phasors in float64 — is done by the caller ahead of time into
+// tables (same NumPy expressions, so the REAL values are identical), and the plan
+// replays the steps on the device: one fused kernel per step, one DFT kernel per
+// trace chunk, and a counter bump, captured once into a hipGraph per buffer parity.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "fused.h"
+
+namespace etl {
+namespace {
+
+__global__ void set_counter_kernel(int* counter, int value) {
+    if (threadIdx.x == 0) counter[0] = value;
+}
+
+// flag[0] |= 1 if psi is non-zero anywhere the fused kernel assumes it stays +0:
+// outside the CPML strips, and on the outer tangential-E rows/columns of psi_E.
+template <typename R>
+__global__ void psi_check_kernel(const R* __restrict__ psi, int rows, int cols, int lo_r, int hi_r,
+                                 int lo_c, int hi_c, int edge_rows, int edge_cols,
+                                 int* __restrict__ flag) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= int64_t(rows) * cols) return;
+    const int i = int(k / cols);
+    const int j = int(k - int64_t(i) * cols);
+    const bool in_strip = (i < lo_r || i >= hi_r || j < lo_c || j >= hi_c);
+    const bool on_edge =
+        (edge_rows && (i == 0 || i == rows - 1)) || (edge_cols && (j == 0 || j == cols - 1));
+    if ((!in_strip || on_edge) && psi[k] != R(0)) flag[0] = 1;
+}
+
+template <typename R>
+void strip_bounds(const std::vector<R>& b, const std::vector<R>& c, const std::vector<R>& ik,
+                  int& lo, int& hi) {
+    const int n = int(b.size());
+    const int half = (n + 1) / 2;
+    lo = 0;
+    hi = n;
+    for (int k = 0; k < n; ++k) {
+        const bool nontrivial = (b[k] != R(1)) || (c[k] != R(0)) || (ik[k] != R(1));
+        if (!nontrivial) continue;
+        if (k < half)
+            lo = k + 1;
+        else if (hi == n)
+            hi = k;
+    }
+}
+
+}  // namespace
+}  // namespace etl
+
+using namespace etl;
+
+struct EtlPlan {
+    int precision = 0;
+    int nx = 0, ny = 0;
+    EtlTune tune{};
+    EtlTeDesc desc{};
+    int parity = 0;
+    int strips[8] = {0};  // xh_lo, xh_hi, xe_lo, xe_hi, yh_lo, yh_hi, ye_lo, ye_hi
+
+    // sources
+    int n_src = 0;
+    int src_comp[kMaxSources] = {0}, src_i[kMaxSources] = {0}, src_j[kMaxSources] = {0};
+    void* delta = nullptr;
+    int64_t delta_n0 = 0;
+    int delta_rows = 0;
+    size_t delta_bytes = 0;
+
+    // monitors
+    struct Mon {
+        int comp, n_nodes, n_f;
+        void* acc;
+        std::vector<int64_t> flat;
+        int slot_base, q_base;
+        int64_t n0 = 0;
+        int rows = -1;
+    };
+    std::vector<Mon> mons;
+    bool mon_dirty = true;
+    int n_slots = 0, nq = 0, n_pairs = 0;
+    int* mon_ptr[3] = {nullptr, nullptr, nullptr};
+    int2* mon_ent[3] = {nullptr, nullptr, nullptr};
+    void* trace = nullptr;
+    int4* pairs = nullptr;
+    void** accs = nullptr;
+    void* ph = nullptr;  // (ph_rows, nq) complex
+    int ph_rows_cap = 0;
+
+    int* counter = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraph_t graph_src[2] = {nullptr, nullptr};
+
+    int elem() const { return precision == ETL_F32 ? 4 : 8; }
+    int n_tj() const { return (ny + 1 + 64 * tune.vec - 1) / (64 * tune.vec); }
+    int n_bands() const { return (nx + tune.band_rows - 1) / tune.band_rows; }
+    int rows_of(int comp) const { return comp == 1 ? nx + 1 : nx; }
+    int pitch_of(int comp) const { return comp == 0 ? ny + 1 : ny; }
+};
+
+namespace {
+
+void free_graphs(EtlPlan* p) {
+    for (int k = 0; k < 2; ++k) {
+        if (p->graph[k]) (void)hipGraphExecDestroy(p->graph[k]);
+        if (p->graph_src[k]) (void)hipGraphDestroy(p->graph_src[k]);
+        p->graph[k] = nullptr;
+        p->graph_src[k] = nullptr;
+    }
+}
+
+void free_monitors(EtlPlan* p) {
+    for (int c = 0; c < 3; ++c) {
+        etl_free(p->mon_ptr[c]);
+        etl_free(p->mon_ent[c]);
+        p->mon_ptr[c] = nullptr;
+        p->mon_ent[c] = nullptr;
+    }
+    etl_free(p->trace);
+    etl_free(p->pairs);
+    etl_free(p->accs);
+    p->trace = nullptr;
+    p->pairs = nullptr;
+    p->accs = nullptr;
+}
+
+int upload(void** dst, const void* src, size_t bytes) {
+    int rc = etl_malloc(dst, bytes);
+    if (rc) return rc;
+    return etl_memcpy_h2d(*dst, src, bytes);
+}
+
+// Device CSR of monitored nodes, DFT pairs and the trace buffer.
+int build_monitors(EtlPlan* p) {
+    free_monitors(p);
+    free_graphs(p);
+    const int ntj = p->n_tj();
+    const int tw = 64 * p->tune.vec;
+    int slot = 0, q = 0;
+    std::vector<std::vector<int>> cnt(3);
+    std::vector<std::vector<std::pair<int64_t, int2>>> ent(3);  // (cell, (col, slot))
+    std::vector<int4> pairs;
+    std::vector<void*> accs;
+    for (size_t m = 0; m < p->mons.size(); ++m) {
+        auto& mon = p->mons[m];
+        mon.slot_base = slot;
+        mon.q_base = q;
+        const int pitch = p->pitch_of(mon.comp);
+        for (int k = 0; k < mon.n_nodes; ++k) {
+            const int64_t fl = mon.flat[k];
+            const int row = int(fl / pitch);
+            const int col = int(fl - int64_t(row) * pitch);
+            ent[mon.comp].push_back({int64_t(row) * ntj + col / tw, make_int2(col, slot + k)});
+        }
+        for (int f = 0; f < mon.n_f; ++f)
+            for (int k = 0; k < mon.n_nodes; ++k)
+                pairs.push_back(make_int4(slot + k, q + f, int(m), f * mon.n_nodes + k));
+        accs.push_back(mon.acc);
+        slot += mon.n_nodes;
+        q += mon.n_f;
+    }
+    p->n_slots = slot;
+    p->nq = q;
+    p->n_pairs = int(pairs.size());
+    int rc;
+    for (int c = 0; c < 3; ++c) {
+        if (ent[c].empty()) continue;
+        const int64_t cells = int64_t(p->rows_of(c)) * ntj;
+        std::stable_sort(ent[c].begin(), ent[c].end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        std::vector<int> ptr(cells + 1, 0);
+        for (auto& e : ent[c]) ptr[e.first + 1]++;
+        for (int64_t k = 0; k < cells; ++k) ptr[k + 1] += ptr[k];
+        std::vector<int2> flat(ent[c].size());
+        for (size_t k = 0; k < ent[c].size(); ++k) flat[k] = ent[c][k].second;
+        if ((rc = upload((void**)&p->mon_ptr[c], ptr.data(), ptr.size() * sizeof(int)))) return rc;
+        if ((rc = upload((void**)&p->mon_ent[c], flat.data(), flat.size() * sizeof(int2))))
+            return rc;
+    }
+    if (p->n_pairs) {
+        if ((rc = upload((void**)&p->pairs, pairs.data(), pairs.size() * sizeof(int4)))) return rc;
+        if ((rc = upload((void**)&p->accs, accs.data(), accs.size() * sizeof(void*)))) return rc;
+        if ((rc = etl_malloc(&p->trace,
+                             size_t(p->tune.trace_chunk) * std::max(1, p->n_slots) * p->elem())))
+            return rc;
+    }
+    p->mon_dirty = false;
+    return ETL_OK;
+}
+
+template <typename R>
+FusedArgs<R> make_args(const EtlPlan* p) {
+    FusedArgs<R> a;
+    std::memset(&a, 0, sizeof a);
+    for (int k = 0; k < 7; ++k)
+        for (int b = 0; b < 2; ++b) a.f[k][b] = static_cast<const R*>(p->desc.field[k][b]);
+    const EtlTeDesc& d = p->desc;
+    a.cb_ex = (const R*)d.cb_ex;
+    a.cb_ey = (const R*)d.cb_ey;
+    a.bx_e = (const R*)d.bx_e;
+    a.cx_e = (const R*)d.cx_e;
+    a.ikx_e = (const R*)d.ikx_e;
+    a.bx_h = (const R*)d.bx_h;
+    a.cx_h = (const R*)d.cx_h;
+    a.ikx_h = (const R*)d.ikx_h;
+    a.by_e = (const R*)d.by_e;
+    a.cy_e = (const R*)d.cy_e;
+    a.iky_e = (const R*)d.iky_e;
+    a.by_h = (const R*)d.by_h;
+    a.cy_h = (const R*)d.cy_h;
+    a.iky_h = (const R*)d.iky_h;
+    a.db = R(d.db);
+    a.inv_dx = R(d.inv_dx);
+    a.inv_dy = R(d.inv_dy);
+    a.nx = p->nx;
+    a.ny = p->ny;
+    a.band = p->tune.band_rows;
+    a.n_bands = p->n_bands();
+    a.n_tj = p->n_tj();
+    a.tiles_per_xcd = (a.n_bands * a.n_tj + 7) / 8;
+    a.xh_lo = p->strips[0];
+    a.xh_hi = p->strips[1];
+    a.xe_lo = p->strips[2];
+    a.xe_hi = p->strips[3];
+    a.yh_lo = p->strips[4];
+    a.yh_hi = p->strips[5];
+    a.ye_lo = p->strips[6];
+    a.ye_hi = p->strips[7];
+    a.n_src = p->n_src;
+    for (int s = 0; s < p->n_src; ++s) {
+        a.src_comp[s] = p->src_comp[s];
+        a.src_i[s] = p->src_i[s];
+        a.src_j[s] = p->src_j[s];
+    }
+    a.delta = (const R*)p->delta;
+    a.has_mon = p->n_slots > 0;
+    for (int c = 0; c < 3; ++c) {
+        a.mon_ptr[c] = p->mon_ptr[c];
+        a.mon_ent[c] = p->mon_ent[c];
+    }
+    a.trace = (R*)p->trace;
+    a.n_slots = p->n_slots;
+    return a;
+}
+
+// Enqueue `count` (<= trace_chunk) steps starting at buffer parity `parity`, then the
+// DFT of their trace rows and the counter bump.  `events` (optional) receives a
+// start/stop pair around every fused launch.
+template <typename R>
+int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events) {
+    const FusedArgs<R> a = make_args<R>(p);
+    hipStream_t s = ctx().stream;
+    for (int k = 0; k < count; ++k) {
+        if (events) ETL_HIP(hipEventRecord(events[2 * k], s));
+        int rc = launch_fused_step<R>(a, p->tune.vec, p->counter, k, parity ^ (k & 1), s);
+        if (rc) return rc;
+        if (events) ETL_HIP(hipEventRecord(events[2 * k + 1], s));
+    }
+    if (p->n_pairs) {
+        int rc = launch_trace_dft<R>((const R*)p->trace, p->n_slots, count, (const R*)p->ph,
+                                     p->nq, p->counter, p->pairs, p->n_pairs, (R* const*)p->accs,
+                                     s);
+        if (rc) return rc;
+    }
+    return launch_bump(p->counter, count, s);
+}
+
+int enqueue(EtlPlan* p, int count, int parity, hipEvent_t* events) {
+    return p->precision == ETL_F32 ? enqueue_block<float>(p, count, parity, events)
+                                   : enqueue_block<double>(p, count, parity, events);
+}
+
+int ensure_graph(EtlPlan* p, int parity) {
+    if (p->graph[parity]) return ETL_OK;
+    hipStream_t s = ctx().stream;
+    ETL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(p, p->tune.trace_chunk, parity, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    ETL_HIP(e);
+    p->graph_src[parity] = g;
+    ETL_HIP(hipGraphInstantiate(&p->graph[parity], g, nullptr, nullptr, 0));
+    return ETL_OK;
+}
+
+int check_tables(EtlPlan* p, int64_t n, int64_t count) {
+    if (p->n_src > 0) {
+        ETL_REQUIRE(p->delta != nullptr, "etl_plan_advance: no source table");
+        if (!(n >= p->delta_n0 && n + count <= p->delta_n0 + p->delta_rows)) {
+            set_error("etl_plan_advance: source table covers steps [%lld, %lld), asked [%lld, %lld)",
+                      (long long)p->delta_n0, (long long)(p->delta_n0 + p->delta_rows),
+                      (long long)n, (long long)(n + count));
+            return ETL_ESTATE;
+        }
+    }
+    for (auto& m : p->mons) {
+        if (m.rows < 0 || !(n >= m.n0 && n + count <= m.n0 + m.rows)) {
+            set_error("etl_plan_advance: a phasor table does not cover steps [%lld, %lld)",
+                      (long long)n, (long long)(n + count));
+            return ETL_ESTATE;
+        }
+        if (p->n_src > 0 && m.n0 != p->delta_n0) {
+            set_error("etl_plan_advance: source and phasor tables start at different steps");
+            return ETL_ESTATE;
+        }
+    }
+    for (size_t k = 1; k < p->mons.size(); ++k)
+        if (p->mons[k].n0 != p->mons[0].n0) {
+            set_error("etl_plan_advance: phasor tables start at different steps");
+            return ETL_ESTATE;
+        }
+    return ETL_OK;
+}
+
+int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events) {
+    int rc = check_tables(p, n, count);
+    if (rc) return rc;
+    if (p->mon_dirty) {
+        if ((rc = build_monitors(p))) return rc;
+    }
+    const int64_t n0 = p->n_src > 0 ? p->delta_n0 : (p->mons.empty() ? n : p->mons[0].n0);
+    const int64_t start = n - n0;
+    ETL_REQUIRE(start >= 0 && start + count < (int64_t(1) << 31), "etl_plan_advance: step range");
+    hipStream_t s = ctx().stream;
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(64), 0, s, p->counter, int(start));
+    ETL_LAUNCHED();
+    const int K = p->tune.trace_chunk;
+    int64_t done = 0;
+    if (events == nullptr && p->tune.use_graph) {
+        while (count - done >= K) {
+            if ((rc = ensure_graph(p, p->parity))) return rc;
+            ETL_HIP(hipGraphLaunch(p->graph[p->parity], s));
+            done += K;  // K is even: parity unchanged
+        }
+    }
+    while (done < count) {
+        const int blk = int(std::min<int64_t>(K, count - done));
+        if ((rc = enqueue(p, blk, p->parity, events ? events + 2 * done : nullptr))) return rc;
+        p->parity ^= (blk & 1);
+        done += blk;
+    }
+    return ETL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(desc && out, "etl_plan_create: null argument");
+    ETL_REQUIRE(desc->precision == ETL_F32 || desc->precision == ETL_F64,
+                "etl_plan_create: unknown precision %d", desc->precision);
+    ETL_REQUIRE(desc->nx >= 1 && desc->ny >= 1 && desc->nx < (1 << 24) && desc->ny < (1 << 24),
+                "etl_plan_create: bad grid %d x %d", desc->nx, desc->ny);
+    for (int k = 0; k < 7; ++k)
+        ETL_REQUIRE(desc->field[k][0] && desc->field[k][1] && desc->field[k][0] != desc->field[k][1],
+                    "etl_plan_create: field %d needs two distinct buffers", k);
+    ETL_REQUIRE(desc->cb_ex && desc->cb_ey && desc->bx_e && desc->cx_e && desc->ikx_e &&
+                    desc->bx_h && desc->cx_h && desc->ikx_h && desc->by_e && desc->cy_e &&
+                    desc->iky_e && desc->by_h && desc->cy_h && desc->iky_h,
+                "etl_plan_create: null coefficient array");
+    EtlTune t{};
+    if (tune) t = *tune;
+    const bool f32 = desc->precision == ETL_F32;
+    if (t.vec == 0) t.vec = f32 ? 4 : 2;
+    if (t.band_rows == 0) t.band_rows = 8;
+    if (t.trace_chunk == 0) t.trace_chunk = 64;
+    if (t.temporal == 0) t.temporal = 1;
+    if (tune == nullptr) t.use_graph = 1;
+    ETL_REQUIRE(t.vec == 1 || t.vec == 2 || t.vec == 4, "etl_plan_create: vec must be 1, 2 or 4");
+    ETL_REQUIRE(t.band_rows >= 1 && t.band_rows <= 4096, "etl_plan_create: band_rows %d",
+                t.band_rows);
+    ETL_REQUIRE(t.trace_chunk >= 2 && t.trace_chunk <= 4096 && t.trace_chunk % 2 == 0,
+                "etl_plan_create: trace_chunk must be even, in [2, 4096]");
+    ETL_REQUIRE(t.temporal == 1, "etl_plan_create: temporal blocking depth %d not available",
+                t.temporal);
+
+    EtlPlan* p = new EtlPlan();
+    p->precision = desc->precision;
+    p->nx = desc->nx;
+    p->ny = desc->ny;
+    p->tune = t;
+    p->desc = *desc;
+    const int nx = p->nx, ny = p->ny;
+
+    // CPML strips from the 1D profiles.
+    auto fetch = [&](const void* src, int n, auto& v) -> int {
+        v.resize(n);
+        return etl_memcpy_d2h(v.data(), src, size_t(n) * p->elem());
+    };
+    auto bounds = [&](auto tag, int* out) -> int {
+        using R = decltype(tag);
+        std::vector<R> b, c, ik;
+        const void* prof[4][3] = {{desc->bx_h, desc->cx_h, desc->ikx_h},
+                                  {desc->bx_e, desc->cx_e, desc->ikx_e},
+                                  {desc->by_h, desc->cy_h, desc->iky_h},
+                                  {desc->by_e, desc->cy_e, desc->iky_e}};
+        const int len[4] = {nx, nx + 1, ny, ny + 1};
+        for (int a = 0; a < 4; ++a) {
+            int rc;
+            if ((rc = fetch(prof[a][0], len[a], b)) || (rc = fetch(prof[a][1], len[a], c)) ||
+                (rc = fetch(prof[a][2], len[a], ik)))
+                return rc;
+            strip_bounds<R>(b, c, ik, out[2 * a], out[2 * a + 1]);
+        }
+        return ETL_OK;
+    };
+    int rc = f32 ? bounds(float(0), p->strips) : bounds(double(0), p->strips);
+    if (rc == ETL_OK) rc = etl_malloc((void**)&p->counter, 256);
+    // psi must be +0 wherever the fused kernel does not touch it.
+    if (rc == ETL_OK) {
+        int* flag = nullptr;
+        rc = etl_malloc((void**)&flag, 256);
+        if (rc == ETL_OK) rc = zero_async(flag, 4);
+        struct {
+            int field, rows, cols, lo_r, hi_r, lo_c, hi_c, er, ec;
+        } chk[4] = {
+            {3, nx, ny + 1, 0, nx, p->strips[6], p->strips[7], 0, 1},        // psi_ex_y
+            {4, nx + 1, ny, p->strips[2], p->strips[3], 0, ny, 1, 0},        // psi_ey_x
+            {5, nx, ny, p->strips[0], p->strips[1], 0, ny, 0, 0},            // psi_hz_x
+            {6, nx, ny, 0, nx, p->strips[4], p->strips[5], 0, 0},            // psi_hz_y
+        };
+        for (int k = 0; k < 4 && rc == ETL_OK; ++k) {
+            const auto& q = chk[k];
+            const int64_t n = int64_t(q.rows) * q.cols;
+            const dim3 grid(unsigned((n + 255) / 256));
+            if (f32)
+                hipLaunchKernelGGL(psi_check_kernel<float>, grid, dim3(256), 0, ctx().stream,
+                                   (const float*)desc->field[q.field][0], q.rows, q.cols, q.lo_r,
+                                   q.hi_r, q.lo_c, q.hi_c, q.er, q.ec, flag);
+            else
+                hipLaunchKernelGGL(psi_check_kernel<double>, grid, dim3(256), 0, ctx().stream,
+                                   (const double*)desc->field[q.field][0], q.rows, q.cols, q.lo_r,
+                                   q.hi_r, q.lo_c, q.hi_c, q.er, q.ec, flag);
+            if (hipGetLastError() != hipSuccess) rc = ETL_EHIP;
+        }
+        int host_flag = 0;
+        if (rc == ETL_OK) rc = etl_memcpy_d2h(&host_flag, flag, 4);
+        etl_free(flag);
+        if (rc == ETL_OK && host_flag) {
+            set_error("etl_plan_create: psi is non-zero outside the CPML strips; the fused "
+                      "path needs psi = +0 there (use the per-half-step primitives)");
+            rc = ETL_ESTATE;
+        }
+    }
+    if (rc != ETL_OK) {
+        etl_free(p->counter);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return ETL_OK;
+}
+
+int etl_plan_destroy(etl_plan_t p) {
+    if (p == nullptr) return ETL_OK;
+    ETL_CHECK_INIT();
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    free_graphs(p);
+    free_monitors(p);
+    etl_free(p->delta);
+    etl_free(p->ph);
+    etl_free(p->counter);
+    delete p;
+    return ETL_OK;
+}
+
+int etl_plan_set_sources(etl_plan_t p, int n_nodes, const int* comp, const int64_t* i,
+                         const int64_t* j) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_plan_set_sources: null plan");
+    ETL_REQUIRE(n_nodes >= 0 && n_nodes <= kMaxSources,
+                "etl_plan_set_sources: %d source nodes (at most %d)", n_nodes, kMaxSources);
+    ETL_REQUIRE(n_nodes == 0 || (comp && i && j), "etl_plan_set_sources: null array");
+    for (int s = 0; s < n_nodes; ++s) {
+        ETL_REQUIRE(comp[s] >= 0 && comp[s] <= 2, "etl_plan_set_sources: component %d", comp[s]);
+        ETL_REQUIRE(i[s] >= 0 && i[s] < p->rows_of(comp[s]) && j[s] >= 0 &&
+                        j[s] < p->pitch_of(comp[s]),
+                    "etl_plan_set_sources: node (%lld, %lld) outside component %d",
+                    (long long)i[s], (long long)j[s], comp[s]);
+        p->src_comp[s] = comp[s];
+        p->src_i[s] = int(i[s]);
+        p->src_j[s] = int(j[s]);
+    }
+    p->n_src = n_nodes;
+    free_graphs(p);
+    return ETL_OK;
+}
+
+int etl_plan_source_table(etl_plan_t p, int64_t n0, int rows, const void* delta) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_plan_source_table: null plan");
+    ETL_REQUIRE(rows >= 0 && n0 >= 0, "etl_plan_source_table: bad rows");
+    if (p->n_src == 0) return ETL_OK;
+    ETL_REQUIRE(delta != nullptr, "etl_plan_source_table: null table");
+    const size_t bytes = size_t(rows) * p->n_src * p->elem();
+    if (bytes > p->delta_bytes) {
+        etl_free(p->delta);
+        p->delta = nullptr;
+        int rc = etl_malloc(&p->delta, bytes);
+        if (rc) return rc;
+        p->delta_bytes = bytes;
+        free_graphs(p);
+    }
+    ETL_HIP(hipStreamSynchronize(ctx().stream));  // the previous table may be in use
+    int rc = etl_memcpy_h2d(p->delta, delta, bytes);
+    if (rc) return rc;
+    p->delta_n0 = n0;
+    p->delta_rows = rows;
+    return ETL_OK;
+}
+
+int etl_plan_add_monitor(etl_plan_t p, int comp, int n_nodes, const int64_t* flat_index, int n_f,
+                         void* acc, int* id) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p && id, "etl_plan_add_monitor: null argument");
+    ETL_REQUIRE(comp >= 0 && comp <= 2, "etl_plan_add_monitor: component %d", comp);
+    ETL_REQUIRE(n_nodes >= 1 && n_f >= 1 && flat_index && acc,
+                "etl_plan_add_monitor: bad monitor (%d nodes, %d frequencies)", n_nodes, n_f);
+    const int64_t size = int64_t(p->rows_of(comp)) * p->pitch_of(comp);
+    EtlPlan::Mon m;
+    m.comp = comp;
+    m.n_nodes = n_nodes;
+    m.n_f = n_f;
+    m.acc = acc;
+    m.flat.assign(flat_index, flat_index + n_nodes);
+    for (int64_t f : m.flat)
+        ETL_REQUIRE(f >= 0 && f < size, "etl_plan_add_monitor: index %lld outside component %d",
+                    (long long)f, comp);
+    m.slot_base = m.q_base = 0;
+    p->mons.push_back(m);
+    p->mon_dirty = true;
+    *id = int(p->mons.size()) - 1;
+    // The combined phasor table changes shape: drop it.
+    etl_free(p->ph);
+    p->ph = nullptr;
+    p->ph_rows_cap = 0;
+    for (auto& mm : p->mons) mm.rows = -1;
+    free_graphs(p);
+    return ETL_OK;
+}
+
+int etl_plan_phasor_table(etl_plan_t p, int id, int64_t n0, int rows, const void* ph) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_plan_phasor_table: null plan");
+    ETL_REQUIRE(id >= 0 && id < int(p->mons.size()), "etl_plan_phasor_table: no monitor %d", id);
+    ETL_REQUIRE(rows >= 0 && n0 >= 0 && (ph || rows == 0), "etl_plan_phasor_table: bad table");
+    int rc;
+    if (p->mon_dirty) {
+        if ((rc = build_monitors(p))) return rc;
+    }
+    const size_t cplx = size_t(2) * p->elem();
+    if (rows > p->ph_rows_cap) {
+        ETL_HIP(hipStreamSynchronize(ctx().stream));
+        etl_free(p->ph);
+        p->ph = nullptr;
+        if ((rc = etl_malloc(&p->ph, size_t(rows) * p->nq * cplx))) return rc;
+        p->ph_rows_cap = rows;
+        for (auto& mm : p->mons) mm.rows = -1;
+        free_graphs(p);
+    }
+    auto& m = p->mons[id];
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (rows > 0)
+        ETL_HIP(hipMemcpy2DAsync((char*)p->ph + size_t(m.q_base) * cplx, size_t(p->nq) * cplx, ph,
+                                 size_t(m.n_f) * cplx, size_t(m.n_f) * cplx, size_t(rows),
+                                 hipMemcpyHostToDevice, ctx().stream));
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    m.n0 = n0;
+    m.rows = rows;
+    return ETL_OK;
+}
+
+int etl_plan_advance(etl_plan_t p, int64_t n, int64_t count, int* parity) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_plan_advance: null plan");
+    ETL_REQUIRE(n >= 0 && count >= 0, "etl_plan_advance: bad step range");
+    if (count > 0) {
+        int rc = run_steps(p, n, count, nullptr);
+        if (rc) return rc;
+    }
+    if (parity) *parity = p->parity;
+    return ETL_OK;
+}
+
+int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, double* kernel_ms, double* total_ms,
+                  int* parity) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_plan_time: null plan");
+    ETL_REQUIRE(count >= 1 && count <= 100000, "etl_plan_time: count %lld outside [1, 100000]",
+                (long long)count);
+    std::vector<hipEvent_t> ev(2 * count);
+    for (auto& e : ev) ETL_HIP(hipEventCreate(&e));
+    hipEvent_t t0, t1;
+    ETL_HIP(hipEventCreate(&t0));
+    ETL_HIP(hipEventCreate(&t1));
+    hipStream_t s = ctx().stream;
+    ETL_HIP(hipEventRecord(t0, s));
+    int rc = run_steps(p, n, count, ev.data());
+    if (rc == ETL_OK) {
+        ETL_HIP(hipEventRecord(t1, s));
+        ETL_HIP(hipEventSynchronize(t1));
+        double sum = 0.0;
+        for (int64_t k = 0; k < count; ++k) {
+            float ms = 0.f;
+            ETL_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+            sum += ms;
+        }
+        float tot = 0.f;
+        ETL_HIP(hipEventElapsedTime(&tot, t0, t1));
+        if (kernel_ms) *kernel_ms = sum / double(count);
+        if (total_ms) *total_ms = tot;
+        if (parity) *parity = p->parity;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc;
+}
+
+}  // extern "C"

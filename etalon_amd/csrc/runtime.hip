@@ -1,0 +1,235 @@
+// Device selection, errors and memory of libetalon_hip.so.
+//
+// Memory: a small caching allocator keyed by exact (rounded) size, so that the
+// per-step host uploads of the backend protocol (one tiny delta array per source
+// per step, etalon/fdtd/solver.py:87) do not call hipMalloc/hipFree each step
+// (hipFree synchronises the device).
+#include <cstdarg>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "common.h"
+
+namespace etl {
+
+static thread_local std::string g_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+Context& ctx() {
+    static Context c;
+    return c;
+}
+
+int require_init() {
+    if (ctx().stream == nullptr) {
+        set_error("libetalon_hip: etl_init() has not been called");
+        return ETL_ESTATE;
+    }
+    return ETL_OK;
+}
+
+namespace {
+std::mutex g_pool_mutex;
+std::multimap<size_t, void*> g_free;           // size -> block, cached for reuse
+std::unordered_map<void*, size_t> g_live;      // block -> size
+size_t g_cached_bytes = 0;
+constexpr size_t kCacheLimit = size_t(1) << 30;  // keep at most 1 GiB of freed blocks
+
+size_t round_size(size_t bytes) {
+    if (bytes <= 256) return 256;
+    if (bytes <= (size_t(1) << 20)) {  // powers of two up to 1 MiB
+        size_t s = 256;
+        while (s < bytes) s <<= 1;
+        return s;
+    }
+    return (bytes + 65535) & ~size_t(65535);  // 64 KiB granules above
+}
+
+__global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n_words) {
+    size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (; k < n_words; k += stride) p[k] = 0u;
+}
+
+__global__ void fill_zero_bytes_kernel(uint8_t* __restrict__ p, size_t n) {
+    size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+}  // namespace
+
+int zero_async(void* dst, size_t bytes) {
+    if (bytes == 0) return ETL_OK;
+    uint8_t* p = static_cast<uint8_t*>(dst);
+    // Word part (blocks are 256-byte aligned; a tail of < 4 bytes is cleared bytewise).
+    size_t words = bytes / 4;
+    if (words) {
+        size_t blocks = (words + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(blocks)), dim3(256), 0, ctx().stream,
+                           reinterpret_cast<uint32_t*>(p), words);
+        ETL_LAUNCHED();
+    }
+    size_t tail = bytes - words * 4;
+    if (tail) {
+        hipLaunchKernelGGL(fill_zero_bytes_kernel, dim3(1), dim3(64), 0, ctx().stream,
+                           p + words * 4, tail);
+        ETL_LAUNCHED();
+    }
+    return ETL_OK;
+}
+
+}  // namespace etl
+
+using namespace etl;
+
+extern "C" {
+
+int etl_last_error(char* buf, size_t n) {
+    if (buf == nullptr || n == 0) return ETL_EARG;
+    snprintf(buf, n, "%s", g_error.c_str());
+    return ETL_OK;
+}
+
+int etl_device_count(int* count) {
+    ETL_REQUIRE(count != nullptr, "etl_device_count: null pointer");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+    return ETL_OK;
+}
+
+int etl_init(int device) {
+    Context& c = ctx();
+    if (c.stream != nullptr) {
+        ETL_REQUIRE(device == c.device, "etl_init: already initialised on device %d, asked for %d",
+                    c.device, device);
+        return ETL_OK;
+    }
+    int n = 0;
+    ETL_HIP(hipGetDeviceCount(&n));
+    ETL_REQUIRE(device >= 0 && device < n, "etl_init: device %d does not exist (%d visible)",
+                device, n);
+    ETL_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    ETL_HIP(hipGetDeviceProperties(&prop, device));
+    snprintf(c.name, sizeof c.name, "%s", prop.name);
+    snprintf(c.arch, sizeof c.arch, "%s", prop.gcnArchName);
+    c.cu_count = prop.multiProcessorCount;
+    c.mem = prop.totalGlobalMem;
+    ETL_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.device = device;
+    return ETL_OK;
+}
+
+int etl_device_info(char* buf, size_t n) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(buf != nullptr && n > 0, "etl_device_info: null buffer");
+    snprintf(buf, n, "%s %s CUs=%d mem=%zu", ctx().name, ctx().arch, ctx().cu_count, ctx().mem);
+    return ETL_OK;
+}
+
+int etl_malloc(void** ptr, size_t bytes) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(ptr != nullptr, "etl_malloc: null pointer");
+    size_t size = round_size(bytes == 0 ? 1 : bytes);
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        auto it = g_free.find(size);
+        if (it != g_free.end()) {
+            *ptr = it->second;
+            g_free.erase(it);
+            g_cached_bytes -= size;
+            g_live[*ptr] = size;
+            return ETL_OK;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, size);
+    if (e != hipSuccess) {
+        // Release the cache and retry once.
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        ETL_HIP(hipStreamSynchronize(ctx().stream));
+        for (auto& kv : g_free) (void)hipFree(kv.second);
+        g_free.clear();
+        g_cached_bytes = 0;
+        e = hipMalloc(&p, size);
+        if (e != hipSuccess) {
+            set_error("etl_malloc: hipMalloc(%zu) failed: %s", size, hipGetErrorString(e));
+            return ETL_ENOMEM;
+        }
+    }
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    g_live[p] = size;
+    *ptr = p;
+    return ETL_OK;
+}
+
+int etl_free(void* ptr) {
+    if (ptr == nullptr) return ETL_OK;
+    ETL_CHECK_INIT();
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    auto it = g_live.find(ptr);
+    ETL_REQUIRE(it != g_live.end(), "etl_free: %p was not allocated by etl_malloc", ptr);
+    size_t size = it->second;
+    g_live.erase(it);
+    if (g_cached_bytes + size <= kCacheLimit) {
+        // Stream-ordered reuse: every later use of this block is issued on the same
+        // stream, after the work that used it before.
+        g_free.emplace(size, ptr);
+        g_cached_bytes += size;
+        return ETL_OK;
+    }
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    ETL_HIP(hipFree(ptr));
+    return ETL_OK;
+}
+
+int etl_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    ETL_CHECK_INIT();
+    if (bytes == 0) return ETL_OK;
+    ETL_REQUIRE(dst && src, "etl_memcpy_h2d: null pointer");
+    ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx().stream));
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    return ETL_OK;
+}
+
+int etl_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    ETL_CHECK_INIT();
+    if (bytes == 0) return ETL_OK;
+    ETL_REQUIRE(dst && src, "etl_memcpy_d2h: null pointer");
+    ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream));
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    return ETL_OK;
+}
+
+int etl_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    ETL_CHECK_INIT();
+    if (bytes == 0) return ETL_OK;
+    ETL_REQUIRE(dst && src, "etl_memcpy_d2d: null pointer");
+    ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx().stream));
+    return ETL_OK;
+}
+
+int etl_zero(void* dst, size_t bytes) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(dst != nullptr || bytes == 0, "etl_zero: null pointer");
+    return zero_async(dst, bytes);
+}
+
+int etl_synchronize(void) {
+    ETL_CHECK_INIT();
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    return ETL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+"""Fused time-stepping plan: Python side of etl_plan_* (include/etalon_hip.h).
+
+A plan owns nothing but device-side bookkeeping; the ping-pong buffer pairs are
+DeviceArrays held here.  After ``advance`` the state lives in buffer
+``parity`` of each pair.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from etalon_amd.hip import _lib
+from etalon_amd.hip.backend import DeviceArray, _precision_of, zeros
+from etalon_amd.state import FIELDS, PROFILES
+
+COMPONENT_CODE = {"Ex": 0, "Ey": 1, "Hz": 2}
+MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h
+
+
+@dataclass
+class Tune:
+    band_rows: int = 0
+    vec: int = 0
+    use_graph: int = 1
+    trace_chunk: int = 0
+    temporal: int = 1
+
+    def as_struct(self) -> _lib.EtlTune:
+        return _lib.EtlTune(self.band_rows, self.vec, self.use_graph, self.trace_chunk,
+                            self.temporal)
+
+
+class FusedPlan:
+    """Fused kernel plan over a TEState whose arrays are DeviceArrays."""
+
+    def __init__(self, state, tune: Optional[Tune] = None):
+        self.precision = _precision_of(state.hz.dtype)
+        self.state = state
+        self.pairs = {name: [getattr(state, name), zeros(getattr(state, name).shape,
+                                                         getattr(state, name).dtype)]
+                      for name in FIELDS}
+        d = _lib.EtlTeDesc()
+        d.precision = self.precision
+        d.nx, d.ny = int(state.nx), int(state.ny)
+        d.db, d.inv_dx, d.inv_dy = float(state.db), float(state.inv_dx), float(state.inv_dy)
+        for k, name in enumerate(FIELDS):
+            d.field[k][0] = self.pairs[name][0].ptr
+            d.field[k][1] = self.pairs[name][1].ptr
+        d.cb_ex = state.cb_ex.ptr
+        d.cb_ey = state.cb_ey.ptr
+        for name in PROFILES:
+            setattr(d, name, getattr(state, name).ptr)
+        self._desc = d
+        handle = C.c_void_p()
+        t = tune.as_struct() if tune is not None else None
+        _lib.call("etl_plan_create", C.byref(d), C.byref(t) if t is not None else None,
+                  C.byref(handle))
+        self.handle = handle
+        self.parity = 0
+        self.n_src = 0
+        self._keep: list = []
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.call("etl_plan_destroy", self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_sources(self, comp: np.ndarray, i: np.ndarray, j: np.ndarray) -> None:
+        comp = np.ascontiguousarray(comp, dtype=np.int32)
+        i = np.ascontiguousarray(i, dtype=np.int64)
+        j = np.ascontiguousarray(j, dtype=np.int64)
+        _lib.call("etl_plan_set_sources", self.handle, int(comp.size),
+                  comp.ctypes.data_as(C.POINTER(C.c_int)), i.ctypes.data_as(_lib.i64p),
+                  j.ctypes.data_as(_lib.i64p))
+        self.n_src = int(comp.size)
+
+    def source_table(self, n0: int, delta: np.ndarray) -> None:
+        dtype = self.state.hz.dtype
+        delta = np.ascontiguousarray(delta, dtype=dtype)
+        _lib.call("etl_plan_source_table", self.handle, int(n0), int(delta.shape[0]),
+                  C.c_void_p(delta.ctypes.data))
+
+    def add_monitor(self, component: str, flat_index: np.ndarray, acc: DeviceArray) -> int:
+        idx = np.ascontiguousarray(flat_index, dtype=np.int64)
+        n_f = acc.shape[0]
+        out = C.c_int()
+        _lib.call("etl_plan_add_monitor", self.handle, COMPONENT_CODE[component], int(idx.size),
+                  idx.ctypes.data_as(_lib.i64p), int(n_f), C.c_void_p(acc.ptr), C.byref(out))
+        return out.value
+
+    def phasor_table(self, mon_id: int, n0: int, table: np.ndarray) -> None:
+        table = np.ascontiguousarray(table)
+        _lib.call("etl_plan_phasor_table", self.handle, int(mon_id), int(n0),
+                  int(table.shape[0]), C.c_void_p(table.ctypes.data))
+
+    def advance(self, n0: int, count: int) -> int:
+        parity = C.c_int()
+        _lib.call("etl_plan_advance", self.handle, int(n0), int(count), C.byref(parity))
+        self.parity = parity.value
+        return self.parity
+
+    def time(self, n0: int, count: int) -> tuple[float, float]:
+        """(average fused-kernel duration, total) in ms over ``count`` steps, HIP events."""
+        k, t, parity = C.c_double(), C.c_double(), C.c_int()
+        _lib.call("etl_plan_time", self.handle, int(n0), int(count), C.byref(k), C.byref(t),
+                  C.byref(parity))
+        self.parity = parity.value
+        return k.value, t.value
+
+    def bind(self) -> None:
+        """Point the state's field attributes at the buffers holding the current state."""
+        for name in FIELDS:
+            setattr(self.state, name, self.pairs[name][self.parity])

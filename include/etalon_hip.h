@@ -1,0 +1,159 @@
+/*
+ * etalon_hip.h — C ABI of libetalon_hip.so, the MI355X (gfx950) backend of the
+ * etalon 2D TE FDTD solver.
+ *
+ * Every function returns 0 on success and a negative ETL_E* code on failure; the
+ * message of the last failure on the calling thread is read with etl_last_error().
+ * All "device pointers" are addresses on the library's device (etl_init), such as
+ * those returned by etl_malloc().  No function takes or returns a torch type.
+ * Calls are not thread-safe on one handle; the library issues all work on one HIP
+ * stream per process (default priority), and etl_synchronize() waits for it.
+ *
+ * Precision codes: ETL_F32 (REAL = float, complex = 2 x float interleaved) and
+ * ETL_F64 (REAL = double).  Arrays are C-ordered (x, y) with y fastest:
+ *   hz (nx, ny)   ex (nx, ny + 1)   ey (nx + 1, ny)
+ * Complex accumulators are (n_f, n_nodes) complex, (re, im) interleaved.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * etalon/<file>:<line> in the reference tree.
+ */
+#ifndef ETALON_HIP_H
+#define ETALON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ETL_F32 0
+#define ETL_F64 1
+
+#define ETL_OK 0
+#define ETL_EARG (-1)     /* bad argument (size, null pointer, precision) */
+#define ETL_EHIP (-2)     /* a HIP runtime call failed */
+#define ETL_ESTATE (-3)   /* call out of order (no init, tables missing, ...) */
+#define ETL_ENOMEM (-4)   /* device allocation failed */
+
+/* ------------------------------------------------------------------ runtime */
+
+/* Message of the last failure on this thread (NUL-terminated, truncated to n). */
+int etl_last_error(char* buf, size_t n);
+/* Select HIP device `device` and create the library stream. Idempotent.
+ * Replaces the device probing of etalon/fdtd/backends/cupy_backend.py:36-59. */
+int etl_init(int device);
+/* "<device name> gfx<arch> CUs=<n> mem=<bytes>" of the initialised device. */
+int etl_device_info(char* buf, size_t n);
+/* Number of HIP devices visible (0 when there is no GPU); needs no etl_init. */
+int etl_device_count(int* count);
+
+/* ------------------------------------------------------------------ memory
+ * Backend.to_device / to_host / synchronize of the backend protocol,
+ * etalon/fdtd/backends/cupy_backend.py:61-68 (cp.asarray / cp.asnumpy). */
+int etl_malloc(void** ptr, size_t bytes);
+int etl_free(void* ptr);
+int etl_memcpy_h2d(void* dst, const void* src, size_t bytes);  /* synchronous */
+int etl_memcpy_d2h(void* dst, const void* src, size_t bytes);  /* synchronous */
+int etl_memcpy_d2d(void* dst, const void* src, size_t bytes);  /* stream-ordered */
+int etl_zero(void* dst, size_t bytes);                         /* fill kernel, stream-ordered */
+int etl_synchronize(void);
+
+/* ------------------------------------------------------------------ protocol primitives
+ * One launch each, in place, exactly the operations of the NumPy reference backend. */
+
+/* Hz^(n-1/2) -> Hz^(n+1/2) with the CPML psi_Hz,x / psi_Hz,y recurrences.
+ * Replaces NumpyBackend.update_h_te etalon/fdtd/backends/numpy_backend.py:41-49 and
+ * kernel update_h_te etalon/fdtd/backends/cuda/update_te.cu:14-39 (same argument order). */
+int etl_update_h_te(int precision, void* hz, const void* ex, const void* ey,
+                    void* psi_hz_x, void* psi_hz_y,
+                    const void* bx_h, const void* cx_h, const void* ikx_h,
+                    const void* by_h, const void* cy_h, const void* iky_h,
+                    double db, double inv_dx, double inv_dy, int nx, int ny);
+
+/* Ex (j = 1..ny-1), Ey (i = 1..nx-1) from Hz^(n+1/2); outer tangential E untouched.
+ * Replaces NumpyBackend.update_e_te numpy_backend.py:51-61 and kernel update_e_te
+ * cuda/update_te.cu:41-75. */
+int etl_update_e_te(int precision, void* ex, void* ey, const void* hz,
+                    void* psi_ex_y, void* psi_ey_x,
+                    const void* cb_ex, const void* cb_ey,
+                    const void* bx_e, const void* cx_e, const void* ikx_e,
+                    const void* by_e, const void* cy_e, const void* iky_e,
+                    double inv_dx, double inv_dy, int nx, int ny);
+
+/* acc[f, k] += field.flat[index[k]] * phasor[f]  (complex acc and phasor).
+ * Replaces NumpyBackend.accumulate_dft numpy_backend.py:63-67, kernel cuda/dft.cu:8-23. */
+int etl_accumulate_dft(int precision, void* acc, const void* field, const int64_t* index,
+                       const void* phasor, int n_f, int n_nodes);
+
+/* field[i[k] * pitch + j[k]] -= delta[k] for k < n (distinct nodes).
+ * Replaces the scatter `field[i, j] -= delta` of FDTDSolver._inject etalon/fdtd/solver.py:83-89. */
+int etl_scatter_sub(int precision, void* field, int64_t pitch, const int64_t* i,
+                    const int64_t* j, const void* delta, int n);
+/* out[k] = field[i[k] * pitch + j[k]] (materialises a fancy-indexed read). */
+int etl_gather(int precision, void* out, const void* field, int64_t pitch, const int64_t* i,
+               const int64_t* j, int n);
+
+/* ------------------------------------------------------------------ fused time stepping
+ * A plan advances the state many steps per call with one fused kernel per step
+ * (H half-step, M injection, E half-step, J injection, monitor trace) and a DFT
+ * kernel per trace chunk, replayed from a hipGraph.  It replaces the host loop of
+ * FDTDSolver.run / step, etalon/fdtd/solver.py:120-128,155-160: the result after
+ * etl_plan_advance(n) is bit-identical to n calls of step(). */
+
+typedef struct EtlPlan* etl_plan_t;
+
+typedef struct EtlTeDesc {
+    int precision;
+    int nx, ny;
+    double db, inv_dx, inv_dy;
+    /* Ping-pong field pairs: [0] holds the current state, [1] is scratch of the
+     * same shape (zero-filled by the caller).  The seven arrays are, in order,
+     * ex, ey, hz, psi_ex_y, psi_ey_x, psi_hz_x, psi_hz_y. */
+    void* field[7][2];
+    const void* cb_ex;
+    const void* cb_ey;
+    /* CPML profiles, etalon/fdtd/state.py:15-49 naming. */
+    const void *bx_e, *cx_e, *ikx_e, *bx_h, *cx_h, *ikx_h;
+    const void *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
+} EtlTeDesc;
+
+/* Tuning knobs of the fused kernel (0 = library default / autotuned). */
+typedef struct EtlTune {
+    int band_rows;     /* rows marched by one wave */
+    int vec;           /* REAL elements per lane along y (1, 2 or 4) */
+    int use_graph;     /* replay steps from a hipGraph (1) or launch directly (0) */
+    int trace_chunk;   /* steps per DFT flush */
+    int temporal;      /* steps fused per launch (1 = none) */
+} EtlTune;
+
+/* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
+int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out);
+int etl_plan_destroy(etl_plan_t plan);
+/* Point sources: node (comp, i, j) receives field -= delta[n][k] in step n; a node
+ * listed more than once is updated once per listing, in list order. */
+int etl_plan_set_sources(etl_plan_t plan, int n_nodes, const int* comp, const int64_t* i,
+                         const int64_t* j);
+/* Host table delta[(n - n0) * n_nodes + k] of REAL for steps [n0, n0 + rows). */
+int etl_plan_source_table(etl_plan_t plan, int64_t n0, int rows, const void* delta);
+/* A monitor component: flat C-order node indices into its component array, and its
+ * device accumulator acc (n_f, n_nodes) complex.  Returns the monitor id in *id. */
+int etl_plan_add_monitor(etl_plan_t plan, int comp, int n_nodes, const int64_t* flat_index,
+                         int n_f, void* acc, int* id);
+/* Host phasor rows (complex REAL, (rows, n_f)) for steps [n0, n0 + rows):
+ * ph[n][f] is the factor of the H sample of step n when comp = Hz and of the E
+ * sample otherwise (etalon/fdtd/solver.py:91-110). */
+int etl_plan_phasor_table(etl_plan_t plan, int id, int64_t n0, int rows, const void* ph);
+/* Advance steps [n, n + count); the tables must cover them.  On return *parity
+ * tells which buffer of each pair holds the state (0: field[k][0]). */
+int etl_plan_advance(etl_plan_t plan, int64_t n, int64_t count, int* parity);
+/* Time `count` steps from step n on the library stream with HIP events: the
+ * average duration of one fused stencil launch (ms) and of the whole call. */
+int etl_plan_time(etl_plan_t plan, int64_t n, int64_t count, double* kernel_ms,
+                  double* total_ms, int* parity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ETALON_HIP_H */

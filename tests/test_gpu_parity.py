@@ -1,0 +1,142 @@
+"""HIP kernels vs the NumPy oracle, bit for bit (the reference's G1–G4 equality suite,
+tests/gpu/test_cupy_equality.py:36-180, tightened from rtol to bitwise equality).
+
+Every comparison is on the bit pattern (±0 distinguished) in float32 and float64,
+through the C ABI of libetalon_hip.so.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import te_numpy as ref
+from tests.sims import bitwise_equal, full_sim, small_sim, vacuum_pulse
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = [("single", np.float32), ("double", np.float64)]
+FIELDS = ("ex", "ey", "hz", "psi_ex_y", "psi_ey_x", "psi_hz_x", "psi_hz_y")
+
+
+def random_state(dtype, seed=1, nx=67, ny=53):
+    """Random fields, ψ and CPML profiles on every node, Δx ≠ Δy (G1/G2 of the reference)."""
+    rng = np.random.default_rng(seed)
+    dx, dy = 0.05, 0.04
+    g = ref.Grid(nx, ny, dx, dy, ref.time_step(0.99, dx, dy), 0, 0.0, 0.0, 0.0)
+    eps = {c: rng.uniform(1.0, 12.0, g.shape(c)) for c in ("Ex", "Ey")}
+
+    def prof(n):
+        return (rng.uniform(0.2, 1.0, n + 1), -rng.uniform(0, 0.8, n + 1), rng.uniform(0.3, 1.0, n + 1),
+                rng.uniform(0.2, 1.0, n), -rng.uniform(0, 0.8, n), rng.uniform(0.3, 1.0, n))
+
+    s = ref.State(g, eps, prof(nx), prof(ny), dtype)
+    for name in FIELDS:
+        v = rng.standard_normal(getattr(s, name).shape).astype(dtype)
+        if name == "hz":
+            v *= dtype(1 / 377.0)
+        setattr(s, name, v)
+    return s
+
+
+def to_hip_state(s):
+    from etalon_amd.hip.backend import to_device
+    from etalon_amd.state import TEState
+
+    arrays = {k: to_device(getattr(s, k)) for k in (*FIELDS, "cb_ex", "cb_ey",
+              "bx_e", "cx_e", "ikx_e", "bx_h", "cx_h", "ikx_h",
+              "by_e", "cy_e", "iky_e", "by_h", "cy_h", "iky_h")}
+    return TEState(**arrays, db=s.db, inv_dx=s.inv_dx, inv_dy=s.inv_dy, nx=s.nx, ny=s.ny)
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+@pytest.mark.parametrize("update", ["h", "e"])
+def test_g1_g2_update_kernels_bitwise(hip, precision, dtype, update):
+    cpu = random_state(dtype)
+    gpu = to_hip_state(cpu)
+    (ref.update_h if update == "h" else ref.update_e)(cpu)
+    (hip.update_h_te if update == "h" else hip.update_e_te)(gpu)
+    for name in FIELDS:
+        assert bitwise_equal(getattr(gpu, name).get(), getattr(cpu, name)), (update, name)
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_g3_dft_kernel_bitwise(hip, precision, dtype):
+    rng = np.random.default_rng(2)
+    cdt = np.complex64 if dtype == np.float32 else np.complex128
+    field = rng.standard_normal((67, 54)).astype(dtype)
+    index = rng.choice(field.size, size=301, replace=False).astype(np.int64)
+    ph = (np.exp(1j * rng.uniform(0, 2 * np.pi, 13)) * 1e-16).astype(cdt)
+    acc = ((rng.standard_normal((13, 301)) + 1j * rng.standard_normal((13, 301))) * 1e-15).astype(cdt)
+    acc_gpu = hip.to_device(acc)
+    ref.dft(acc, field, index, ph)
+    hip.accumulate_dft(acc_gpu, hip.to_device(field), hip.to_device(index), hip.to_device(ph))
+    assert bitwise_equal(acc_gpu.get(), acc)
+
+
+def _compare_solver(gpu_solver, cpu_solver, nonzero_monitors=True):
+    for name, value in cpu_solver.fields().items():
+        assert np.abs(value).max() > 0, name
+        assert bitwise_equal(gpu_solver.fields()[name], value), name
+    for mg, acc in zip(gpu_solver.monitors, cpu_solver.acc):
+        label = f"{mg.sampling.monitor}/{mg.sampling.component}"
+        if nonzero_monitors:
+            assert np.abs(acc).max() > 0, label
+        assert bitwise_equal(mg.acc.get(), acc), label
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_g4_full_run_step_path(hip, precision, dtype):
+    from etalon_amd.solver import FDTDSolver
+
+    sim = full_sim(precision)
+    gpu, cpu = FDTDSolver(sim), ref.Solver(sim)
+    for _ in range(300):
+        gpu.step()
+    cpu.run(300)
+    _compare_solver(gpu, cpu)
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_g4_full_run_fused_path(hip, precision, dtype):
+    """600 steps through the fused plan in uneven chunks (odd counts flip the buffers)."""
+    from etalon_amd.solver import FDTDSolver
+
+    sim = full_sim(precision)
+    gpu, cpu = FDTDSolver(sim), ref.Solver(sim)
+    for count in (1, 63, 64, 129, 343):
+        gpu.advance(count)
+    cpu.run(600)
+    assert gpu.n == cpu.n == 600
+    _compare_solver(gpu, cpu)
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_fused_and_step_paths_interleave(hip, precision, dtype):
+    from etalon_amd.solver import FDTDSolver
+
+    sim = small_sim(precision=precision)
+    gpu, cpu = FDTDSolver(sim), ref.Solver(sim)
+    gpu.advance(37)
+    for _ in range(5):
+        gpu.step()
+    gpu.advance(58)
+    cpu.run(100)
+    _compare_solver(gpu, cpu)
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_vacuum_pulse_2000_steps_bitwise(hip, precision, dtype):
+    """184² grid (8 µm + 12-layer CPML), courant 0.5, 4 point monitors × 11 λ, 2000 steps."""
+    from etalon_amd.solver import FDTDSolver
+
+    sim = vacuum_pulse(precision)
+    gpu = FDTDSolver(sim)
+    assert (gpu.grid.nx, gpu.grid.ny, gpu.n_steps) == (184, 184, 2000)
+    result = gpu.run()
+    cpu = ref.Solver(sim).run()
+    # Ex on the source's axis (y = 0) is exactly zero by mirror symmetry.
+    _compare_solver(gpu, cpu, nonzero_monitors=False)
+    assert sum(np.abs(acc).max() > 0 for acc in cpu.acc) >= 10
+    values = cpu.monitor_values()
+    for mon in sim.monitors:
+        for comp in ("Ex", "Ey", "Hz"):
+            assert bitwise_equal(result[mon.name][comp].values, values[(mon.name, comp)])
