@@ -107,7 +107,8 @@ def run(precision: str, steps: int, warmup: int) -> dict:
     cells = nx * ny
     # Kernel-level timing: HIP events around every fused launch, same plan and stream.
     plan = solver._plan
-    n_time = min(500, max(steps, 64))
+    depth = plan.resolved_tune().temporal
+    n_time = 512  # a multiple of the trace chunk: every launch has the full depth
     delta, phasors = solver._tables(solver.n, n_time)
     if delta is not None:
         plan.source_table(solver.n, delta)
@@ -116,7 +117,7 @@ def run(precision: str, steps: int, warmup: int) -> dict:
     kernel_ms, total_ms = plan.time(solver.n, n_time)
     plan.bind()
     solver.n += n_time
-    bytes_per_launch = BYTES_PER_CELL[precision] * cells
+    bytes_per_launch = BYTES_PER_CELL[precision] * cells * depth
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
     return {
         "cells": (nx, ny),
@@ -124,6 +125,8 @@ def run(precision: str, steps: int, warmup: int) -> dict:
         "value": cells * steps / wall / 1e9,
         "ms_per_step": wall / steps * 1e3,
         "kernel_ms": kernel_ms,
+        "depth": depth,
+        "tune": plan.resolved_tune().__dict__,
         "events_total_ms_per_step": total_ms / n_time,
         "achieved_gbs": achieved,
         "device": solver.backend.device,
@@ -178,10 +181,12 @@ def main() -> int:
         "roofline": roof,
         "kernel": {
             "name": "fused_step_kernel",
-            "avg_ms": round(main_run["kernel_ms"], 5),
+            "avg_ms_per_launch": round(main_run["kernel_ms"], 5),
+            "steps_per_launch": main_run["depth"],
+            "tune": main_run["tune"],
             "events_ms_per_step": round(main_run["events_total_ms_per_step"], 5),
             "bytes_per_cell_update": BYTES_PER_CELL[args.precision],
-            "cells_per_launch": nx * ny,
+            "cell_updates_per_launch": nx * ny * main_run["depth"],
         },
         "device": main_run["device"],
     }

@@ -1,4 +1,4 @@
-// Kernel arguments of the fused time step (see te_fused.hip).
+// Kernel arguments of the fused, temporally blocked time step (see te_fused.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,18 +19,20 @@ struct FusedArgs {
     const R *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
     R db, inv_dx, inv_dy;
     int nx, ny;
-    int band;           // rows per wave
+    int band;           // rows per tile
+    int wo;             // output columns per tile (64 * V - 2 * deepest T)
     int n_bands, n_tj;  // tiles along x and along y
     int tiles_per_xcd;
     // CPML strips: index k of an axis is in the strip iff k < lo or k >= hi.
     int xh_lo, xh_hi, xe_lo, xe_hi, yh_lo, yh_hi, ye_lo, ye_hi;
-    // Point sources: node (comp, i, j); delta table rows of n_src REALs.
+    // Point sources: n_src nodes (comp, i, j, 0) on the device; delta table rows of
+    // n_src REALs, one row per step.
     int n_src;
-    int src_comp[kMaxSources];
-    int src_i[kMaxSources];
-    int src_j[kMaxSources];
+    const int4* src;
     const R* delta;
-    // Monitor trace: per component a CSR over (row * n_tj + tile_column) of entries
+    // Per tile: bit 0 = monitored nodes in its rows, bit 1 = a source in its window.
+    const unsigned char* tile_flags;
+    // Monitor trace: per component a CSR over (row * n_tj + tile column) of entries
     // (column, slot); the trace holds one row of n_slots REALs per step of a chunk.
     int has_mon;
     const int* mon_ptr[3];
@@ -39,9 +41,12 @@ struct FusedArgs {
     int n_slots;
 };
 
+// Output columns of a tile for vector width v and temporal depth t.
+inline int tile_out_cols(int v, int t) { return 64 * v - 2 * t; }
+
 template <typename R>
-int launch_fused_step(const FusedArgs<R>& a, int vec, const int* counter, int kstep, int parity,
-                      hipStream_t stream);
+int launch_fused(const FusedArgs<R>& a, int vec, int depth, const int* counter, int kstep0,
+                 int parity, hipStream_t stream);
 template <typename R>
 int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq,
                      const int* counter, const int4* pairs, int n_pairs, R* const* accs,

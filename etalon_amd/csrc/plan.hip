@@ -4,8 +4,8 @@
 // 83-128,155-160).  The per-step host work of the reference — evaluating the source
 // pulse and the DFT phasors in float64 — is done by the caller ahead of time into
 // tables (same NumPy expressions, so the REAL values are identical), and the plan
-// replays the steps on the device: one fused kernel per step, one DFT kernel per
-// trace chunk, and a counter bump, captured once into a hipGraph per buffer parity.
+// replays the steps on the device: one fused kernel per T steps, one DFT kernel per
+// trace chunk and a counter bump, captured once into a hipGraph per buffer parity.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -20,7 +20,7 @@ __global__ void set_counter_kernel(int* counter, int value) {
     if (threadIdx.x == 0) counter[0] = value;
 }
 
-// flag[0] |= 1 if psi is non-zero anywhere the fused kernel assumes it stays +0:
+// flag[0] = 1 if psi is non-zero anywhere the fused kernel assumes it stays +0:
 // outside the CPML strips, and on the outer tangential-E rows/columns of psi_E.
 template <typename R>
 __global__ void psi_check_kernel(const R* __restrict__ psi, int rows, int cols, int lo_r, int hi_r,
@@ -36,6 +36,8 @@ __global__ void psi_check_kernel(const R* __restrict__ psi, int rows, int cols, 
     if ((!in_strip || on_edge) && psi[k] != R(0)) flag[0] = 1;
 }
 
+// Strip of an axis: every index whose profile differs from (1, 0, 1) lies in
+// [0, lo) or [hi, n).
 template <typename R>
 void strip_bounds(const std::vector<R>& b, const std::vector<R>& c, const std::vector<R>& ik,
                   int& lo, int& hi) {
@@ -68,7 +70,8 @@ struct EtlPlan {
 
     // sources
     int n_src = 0;
-    int src_comp[kMaxSources] = {0}, src_i[kMaxSources] = {0}, src_j[kMaxSources] = {0};
+    std::vector<int4> src;  // comp, i, j, 0
+    int4* src_dev = nullptr;
     void* delta = nullptr;
     int64_t delta_n0 = 0;
     int delta_rows = 0;
@@ -84,10 +87,11 @@ struct EtlPlan {
         int rows = -1;
     };
     std::vector<Mon> mons;
-    bool mon_dirty = true;
+    bool dirty = true;  // monitor CSR / tile flags need a rebuild
     int n_slots = 0, nq = 0, n_pairs = 0;
     int* mon_ptr[3] = {nullptr, nullptr, nullptr};
     int2* mon_ent[3] = {nullptr, nullptr, nullptr};
+    unsigned char* tile_flags = nullptr;
     void* trace = nullptr;
     int4* pairs = nullptr;
     void** accs = nullptr;
@@ -99,7 +103,8 @@ struct EtlPlan {
     hipGraph_t graph_src[2] = {nullptr, nullptr};
 
     int elem() const { return precision == ETL_F32 ? 4 : 8; }
-    int n_tj() const { return (ny + 1 + 64 * tune.vec - 1) / (64 * tune.vec); }
+    int wo() const { return tile_out_cols(tune.vec, tune.temporal); }
+    int n_tj() const { return (ny + 1 + wo() - 1) / wo(); }
     int n_bands() const { return (nx + tune.band_rows - 1) / tune.band_rows; }
     int rows_of(int comp) const { return comp == 1 ? nx + 1 : nx; }
     int pitch_of(int comp) const { return comp == 0 ? ny + 1 : ny; }
@@ -116,7 +121,7 @@ void free_graphs(EtlPlan* p) {
     }
 }
 
-void free_monitors(EtlPlan* p) {
+void free_layout(EtlPlan* p) {
     for (int c = 0; c < 3; ++c) {
         etl_free(p->mon_ptr[c]);
         etl_free(p->mon_ent[c]);
@@ -126,9 +131,11 @@ void free_monitors(EtlPlan* p) {
     etl_free(p->trace);
     etl_free(p->pairs);
     etl_free(p->accs);
+    etl_free(p->tile_flags);
     p->trace = nullptr;
     p->pairs = nullptr;
     p->accs = nullptr;
+    p->tile_flags = nullptr;
 }
 
 int upload(void** dst, const void* src, size_t bytes) {
@@ -137,17 +144,22 @@ int upload(void** dst, const void* src, size_t bytes) {
     return etl_memcpy_h2d(*dst, src, bytes);
 }
 
-// Device CSR of monitored nodes, DFT pairs and the trace buffer.
-int build_monitors(EtlPlan* p) {
-    free_monitors(p);
+// Device CSR of monitored nodes per (row, tile column), DFT pairs, the trace buffer
+// and the per-tile flags; rebuilt whenever sources, monitors or tiling change.
+int build_layout(EtlPlan* p) {
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    free_layout(p);
     free_graphs(p);
     const int ntj = p->n_tj();
-    const int tw = 64 * p->tune.vec;
+    const int nb = p->n_bands();
+    const int wo = p->wo();
+    const int T = p->tune.temporal;
+    const int W = 64 * p->tune.vec;
     int slot = 0, q = 0;
-    std::vector<std::vector<int>> cnt(3);
     std::vector<std::vector<std::pair<int64_t, int2>>> ent(3);  // (cell, (col, slot))
     std::vector<int4> pairs;
     std::vector<void*> accs;
+    std::vector<unsigned char> flags(size_t(nb) * ntj, 0);
     for (size_t m = 0; m < p->mons.size(); ++m) {
         auto& mon = p->mons[m];
         mon.slot_base = slot;
@@ -157,7 +169,10 @@ int build_monitors(EtlPlan* p) {
             const int64_t fl = mon.flat[k];
             const int row = int(fl / pitch);
             const int col = int(fl - int64_t(row) * pitch);
-            ent[mon.comp].push_back({int64_t(row) * ntj + col / tw, make_int2(col, slot + k)});
+            const int tj = col / wo;
+            ent[mon.comp].push_back({int64_t(row) * ntj + tj, make_int2(col, slot + k)});
+            const int tb = std::min(row, p->nx - 1) / p->tune.band_rows;
+            flags[size_t(tb) * ntj + tj] |= 1;
         }
         for (int f = 0; f < mon.n_f; ++f)
             for (int k = 0; k < mon.n_nodes; ++k)
@@ -165,6 +180,19 @@ int build_monitors(EtlPlan* p) {
         accs.push_back(mon.acc);
         slot += mon.n_nodes;
         q += mon.n_f;
+    }
+    // A source matters to every tile whose window (rows from i0 - T, columns
+    // [o0 - T, o0 - T + W)) contains its node.
+    for (const int4& s : p->src) {
+        for (int tb = 0; tb < nb; ++tb) {
+            const int i0 = tb * p->tune.band_rows;
+            const int i1 = std::min(i0 + p->tune.band_rows, p->nx);
+            if (s.y < std::max(i0 - T, 0) || s.y > i1 + T) continue;
+            for (int tj = 0; tj < ntj; ++tj) {
+                const int cs = tj * wo - T;
+                if (s.z >= cs && s.z < cs + W) flags[size_t(tb) * ntj + tj] |= 2;
+            }
+        }
     }
     p->n_slots = slot;
     p->nq = q;
@@ -184,6 +212,7 @@ int build_monitors(EtlPlan* p) {
         if ((rc = upload((void**)&p->mon_ent[c], flat.data(), flat.size() * sizeof(int2))))
             return rc;
     }
+    if ((rc = upload((void**)&p->tile_flags, flags.data(), flags.size()))) return rc;
     if (p->n_pairs) {
         if ((rc = upload((void**)&p->pairs, pairs.data(), pairs.size() * sizeof(int4)))) return rc;
         if ((rc = upload((void**)&p->accs, accs.data(), accs.size() * sizeof(void*)))) return rc;
@@ -191,7 +220,7 @@ int build_monitors(EtlPlan* p) {
                              size_t(p->tune.trace_chunk) * std::max(1, p->n_slots) * p->elem())))
             return rc;
     }
-    p->mon_dirty = false;
+    p->dirty = false;
     return ETL_OK;
 }
 
@@ -222,6 +251,7 @@ FusedArgs<R> make_args(const EtlPlan* p) {
     a.nx = p->nx;
     a.ny = p->ny;
     a.band = p->tune.band_rows;
+    a.wo = p->wo();
     a.n_bands = p->n_bands();
     a.n_tj = p->n_tj();
     a.tiles_per_xcd = (a.n_bands * a.n_tj + 7) / 8;
@@ -234,12 +264,9 @@ FusedArgs<R> make_args(const EtlPlan* p) {
     a.ye_lo = p->strips[6];
     a.ye_hi = p->strips[7];
     a.n_src = p->n_src;
-    for (int s = 0; s < p->n_src; ++s) {
-        a.src_comp[s] = p->src_comp[s];
-        a.src_i[s] = p->src_i[s];
-        a.src_j[s] = p->src_j[s];
-    }
+    a.src = p->src_dev;
     a.delta = (const R*)p->delta;
+    a.tile_flags = p->tile_flags;
     a.has_mon = p->n_slots > 0;
     for (int c = 0; c < 3; ++c) {
         a.mon_ptr[c] = p->mon_ptr[c];
@@ -250,19 +277,26 @@ FusedArgs<R> make_args(const EtlPlan* p) {
     return a;
 }
 
-// Enqueue `count` (<= trace_chunk) steps starting at buffer parity `parity`, then the
-// DFT of their trace rows and the counter bump.  `events` (optional) receives a
-// start/stop pair around every fused launch.
+// Enqueue `count` (<= trace_chunk) steps starting at buffer parity `parity`: fused
+// launches of depth T, then depth-1 launches for the remainder, the DFT of the trace
+// rows and the counter bump.  Returns the number of fused launches in *launches.
+// `events` (optional) receives a start/stop pair around every fused launch.
 template <typename R>
-int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events) {
+int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
     const FusedArgs<R> a = make_args<R>(p);
     hipStream_t s = ctx().stream;
-    for (int k = 0; k < count; ++k) {
-        if (events) ETL_HIP(hipEventRecord(events[2 * k], s));
-        int rc = launch_fused_step<R>(a, p->tune.vec, p->counter, k, parity ^ (k & 1), s);
+    const int T = p->tune.temporal;
+    int k = 0, n = 0;
+    while (k < count) {
+        const int depth = (count - k >= T) ? T : 1;
+        if (events) ETL_HIP(hipEventRecord(events[2 * n], s));
+        int rc = launch_fused<R>(a, p->tune.vec, depth, p->counter, k, parity ^ (n & 1), s);
         if (rc) return rc;
-        if (events) ETL_HIP(hipEventRecord(events[2 * k + 1], s));
+        if (events) ETL_HIP(hipEventRecord(events[2 * n + 1], s));
+        k += depth;
+        ++n;
     }
+    *launches = n;
     if (p->n_pairs) {
         int rc = launch_trace_dft<R>((const R*)p->trace, p->n_slots, count, (const R*)p->ph,
                                      p->nq, p->counter, p->pairs, p->n_pairs, (R* const*)p->accs,
@@ -272,16 +306,17 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events) {
     return launch_bump(p->counter, count, s);
 }
 
-int enqueue(EtlPlan* p, int count, int parity, hipEvent_t* events) {
-    return p->precision == ETL_F32 ? enqueue_block<float>(p, count, parity, events)
-                                   : enqueue_block<double>(p, count, parity, events);
+int enqueue(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
+    return p->precision == ETL_F32 ? enqueue_block<float>(p, count, parity, events, launches)
+                                   : enqueue_block<double>(p, count, parity, events, launches);
 }
 
 int ensure_graph(EtlPlan* p, int parity) {
     if (p->graph[parity]) return ETL_OK;
     hipStream_t s = ctx().stream;
     ETL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue(p, p->tune.trace_chunk, parity, nullptr);
+    int launches = 0;
+    int rc = enqueue(p, p->tune.trace_chunk, parity, nullptr, &launches);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -323,11 +358,11 @@ int check_tables(EtlPlan* p, int64_t n, int64_t count) {
     return ETL_OK;
 }
 
-int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events) {
+int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_launch) {
     int rc = check_tables(p, n, count);
     if (rc) return rc;
-    if (p->mon_dirty) {
-        if ((rc = build_monitors(p))) return rc;
+    if (p->dirty) {
+        if ((rc = build_layout(p))) return rc;
     }
     const int64_t n0 = p->n_src > 0 ? p->delta_n0 : (p->mons.empty() ? n : p->mons[0].n0);
     const int64_t start = n - n0;
@@ -337,19 +372,25 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events) {
     ETL_LAUNCHED();
     const int K = p->tune.trace_chunk;
     int64_t done = 0;
+    int total = 0;
     if (events == nullptr && p->tune.use_graph) {
         while (count - done >= K) {
             if ((rc = ensure_graph(p, p->parity))) return rc;
             ETL_HIP(hipGraphLaunch(p->graph[p->parity], s));
-            done += K;  // K is even: parity unchanged
+            done += K;  // an even number of launches: parity unchanged
+            total += K / p->tune.temporal;
         }
     }
     while (done < count) {
         const int blk = int(std::min<int64_t>(K, count - done));
-        if ((rc = enqueue(p, blk, p->parity, events ? events + 2 * done : nullptr))) return rc;
-        p->parity ^= (blk & 1);
+        int launches = 0;
+        if ((rc = enqueue(p, blk, p->parity, events ? events + 2 * total : nullptr, &launches)))
+            return rc;
+        p->parity ^= (launches & 1);
         done += blk;
+        total += launches;
     }
+    if (n_launch) *n_launch = total;
     return ETL_OK;
 }
 
@@ -375,17 +416,20 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
     if (tune) t = *tune;
     const bool f32 = desc->precision == ETL_F32;
     if (t.vec == 0) t.vec = f32 ? 4 : 2;
-    if (t.band_rows == 0) t.band_rows = 8;
+    if (t.temporal == 0) t.temporal = 4;
+    if (t.band_rows == 0) t.band_rows = 32;
     if (t.trace_chunk == 0) t.trace_chunk = 64;
-    if (t.temporal == 0) t.temporal = 1;
     if (tune == nullptr) t.use_graph = 1;
     ETL_REQUIRE(t.vec == 1 || t.vec == 2 || t.vec == 4, "etl_plan_create: vec must be 1, 2 or 4");
+    ETL_REQUIRE(t.temporal == 1 || t.temporal == 2 || t.temporal == 4,
+                "etl_plan_create: temporal depth must be 1, 2 or 4");
+    ETL_REQUIRE(tile_out_cols(t.vec, t.temporal) >= 32,
+                "etl_plan_create: vec %d too narrow for temporal depth %d", t.vec, t.temporal);
     ETL_REQUIRE(t.band_rows >= 1 && t.band_rows <= 4096, "etl_plan_create: band_rows %d",
                 t.band_rows);
-    ETL_REQUIRE(t.trace_chunk >= 2 && t.trace_chunk <= 4096 && t.trace_chunk % 2 == 0,
-                "etl_plan_create: trace_chunk must be even, in [2, 4096]");
-    ETL_REQUIRE(t.temporal == 1, "etl_plan_create: temporal blocking depth %d not available",
-                t.temporal);
+    ETL_REQUIRE(t.trace_chunk >= 2 && t.trace_chunk <= 4096 &&
+                    t.trace_chunk % (2 * t.temporal) == 0,
+                "etl_plan_create: trace_chunk must be a multiple of 2 * temporal, <= 4096");
 
     EtlPlan* p = new EtlPlan();
     p->precision = desc->precision;
@@ -427,10 +471,10 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
         struct {
             int field, rows, cols, lo_r, hi_r, lo_c, hi_c, er, ec;
         } chk[4] = {
-            {3, nx, ny + 1, 0, nx, p->strips[6], p->strips[7], 0, 1},        // psi_ex_y
-            {4, nx + 1, ny, p->strips[2], p->strips[3], 0, ny, 1, 0},        // psi_ey_x
-            {5, nx, ny, p->strips[0], p->strips[1], 0, ny, 0, 0},            // psi_hz_x
-            {6, nx, ny, 0, nx, p->strips[4], p->strips[5], 0, 0},            // psi_hz_y
+            {3, nx, ny + 1, 0, nx, p->strips[6], p->strips[7], 0, 1},  // psi_ex_y
+            {4, nx + 1, ny, p->strips[2], p->strips[3], 0, ny, 1, 0},  // psi_ey_x
+            {5, nx, ny, p->strips[0], p->strips[1], 0, ny, 0, 0},      // psi_hz_x
+            {6, nx, ny, 0, nx, p->strips[4], p->strips[5], 0, 0},      // psi_hz_y
         };
         for (int k = 0; k < 4 && rc == ETL_OK; ++k) {
             const auto& q = chk[k];
@@ -469,7 +513,8 @@ int etl_plan_destroy(etl_plan_t p) {
     ETL_CHECK_INIT();
     ETL_HIP(hipStreamSynchronize(ctx().stream));
     free_graphs(p);
-    free_monitors(p);
+    free_layout(p);
+    etl_free(p->src_dev);
     etl_free(p->delta);
     etl_free(p->ph);
     etl_free(p->counter);
@@ -484,17 +529,30 @@ int etl_plan_set_sources(etl_plan_t p, int n_nodes, const int* comp, const int64
     ETL_REQUIRE(n_nodes >= 0 && n_nodes <= kMaxSources,
                 "etl_plan_set_sources: %d source nodes (at most %d)", n_nodes, kMaxSources);
     ETL_REQUIRE(n_nodes == 0 || (comp && i && j), "etl_plan_set_sources: null array");
+    std::vector<int4> src;
     for (int s = 0; s < n_nodes; ++s) {
         ETL_REQUIRE(comp[s] >= 0 && comp[s] <= 2, "etl_plan_set_sources: component %d", comp[s]);
         ETL_REQUIRE(i[s] >= 0 && i[s] < p->rows_of(comp[s]) && j[s] >= 0 &&
                         j[s] < p->pitch_of(comp[s]),
                     "etl_plan_set_sources: node (%lld, %lld) outside component %d",
                     (long long)i[s], (long long)j[s], comp[s]);
-        p->src_comp[s] = comp[s];
-        p->src_i[s] = int(i[s]);
-        p->src_j[s] = int(j[s]);
+        // Outer tangential E never changes in the update; a source there would be
+        // the only writer, which the fused step does not model.
+        const bool outer = (comp[s] == 0 && (j[s] == 0 || j[s] == p->ny)) ||
+                           (comp[s] == 1 && (i[s] == 0 || i[s] == p->nx));
+        ETL_REQUIRE(!outer, "etl_plan_set_sources: node on the outer tangential-E boundary");
+        src.push_back(make_int4(comp[s], int(i[s]), int(j[s]), 0));
     }
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    etl_free(p->src_dev);
+    p->src_dev = nullptr;
+    if (n_nodes) {
+        int rc = upload((void**)&p->src_dev, src.data(), src.size() * sizeof(int4));
+        if (rc) return rc;
+    }
+    p->src = src;
     p->n_src = n_nodes;
+    p->dirty = true;
     free_graphs(p);
     return ETL_OK;
 }
@@ -506,6 +564,7 @@ int etl_plan_source_table(etl_plan_t p, int64_t n0, int rows, const void* delta)
     if (p->n_src == 0) return ETL_OK;
     ETL_REQUIRE(delta != nullptr, "etl_plan_source_table: null table");
     const size_t bytes = size_t(rows) * p->n_src * p->elem();
+    ETL_HIP(hipStreamSynchronize(ctx().stream));  // the previous table may be in use
     if (bytes > p->delta_bytes) {
         etl_free(p->delta);
         p->delta = nullptr;
@@ -514,7 +573,6 @@ int etl_plan_source_table(etl_plan_t p, int64_t n0, int rows, const void* delta)
         p->delta_bytes = bytes;
         free_graphs(p);
     }
-    ETL_HIP(hipStreamSynchronize(ctx().stream));  // the previous table may be in use
     int rc = etl_memcpy_h2d(p->delta, delta, bytes);
     if (rc) return rc;
     p->delta_n0 = n0;
@@ -536,12 +594,18 @@ int etl_plan_add_monitor(etl_plan_t p, int comp, int n_nodes, const int64_t* fla
     m.n_f = n_f;
     m.acc = acc;
     m.flat.assign(flat_index, flat_index + n_nodes);
-    for (int64_t f : m.flat)
+    const int pitch = p->pitch_of(comp);
+    for (int64_t f : m.flat) {
         ETL_REQUIRE(f >= 0 && f < size, "etl_plan_add_monitor: index %lld outside component %d",
                     (long long)f, comp);
+        const int64_t row = f / pitch;
+        ETL_REQUIRE(!(comp == 1 && row == p->nx),
+                    "etl_plan_add_monitor: node on the outer tangential-E boundary");
+    }
     m.slot_base = m.q_base = 0;
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
     p->mons.push_back(m);
-    p->mon_dirty = true;
+    p->dirty = true;
     *id = int(p->mons.size()) - 1;
     // The combined phasor table changes shape: drop it.
     etl_free(p->ph);
@@ -558,12 +622,12 @@ int etl_plan_phasor_table(etl_plan_t p, int id, int64_t n0, int rows, const void
     ETL_REQUIRE(id >= 0 && id < int(p->mons.size()), "etl_plan_phasor_table: no monitor %d", id);
     ETL_REQUIRE(rows >= 0 && n0 >= 0 && (ph || rows == 0), "etl_plan_phasor_table: bad table");
     int rc;
-    if (p->mon_dirty) {
-        if ((rc = build_monitors(p))) return rc;
+    if (p->dirty) {
+        if ((rc = build_layout(p))) return rc;
     }
     const size_t cplx = size_t(2) * p->elem();
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
     if (rows > p->ph_rows_cap) {
-        ETL_HIP(hipStreamSynchronize(ctx().stream));
         etl_free(p->ph);
         p->ph = nullptr;
         if ((rc = etl_malloc(&p->ph, size_t(rows) * p->nq * cplx))) return rc;
@@ -572,7 +636,6 @@ int etl_plan_phasor_table(etl_plan_t p, int id, int64_t n0, int rows, const void
         free_graphs(p);
     }
     auto& m = p->mons[id];
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
     if (rows > 0)
         ETL_HIP(hipMemcpy2DAsync((char*)p->ph + size_t(m.q_base) * cplx, size_t(p->nq) * cplx, ph,
                                  size_t(m.n_f) * cplx, size_t(m.n_f) * cplx, size_t(rows),
@@ -583,12 +646,18 @@ int etl_plan_phasor_table(etl_plan_t p, int id, int64_t n0, int rows, const void
     return ETL_OK;
 }
 
+int etl_plan_tune(etl_plan_t p, EtlTune* out) {
+    ETL_REQUIRE(p && out, "etl_plan_tune: null argument");
+    *out = p->tune;
+    return ETL_OK;
+}
+
 int etl_plan_advance(etl_plan_t p, int64_t n, int64_t count, int* parity) {
     ETL_CHECK_INIT();
     ETL_REQUIRE(p != nullptr, "etl_plan_advance: null plan");
     ETL_REQUIRE(n >= 0 && count >= 0, "etl_plan_advance: bad step range");
     if (count > 0) {
-        int rc = run_steps(p, n, count, nullptr);
+        int rc = run_steps(p, n, count, nullptr, nullptr);
         if (rc) return rc;
     }
     if (parity) *parity = p->parity;
@@ -601,26 +670,28 @@ int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, double* kernel_ms, dou
     ETL_REQUIRE(p != nullptr, "etl_plan_time: null plan");
     ETL_REQUIRE(count >= 1 && count <= 100000, "etl_plan_time: count %lld outside [1, 100000]",
                 (long long)count);
-    std::vector<hipEvent_t> ev(2 * count);
+    std::vector<hipEvent_t> ev(2 * count);  // at most one launch per step
     for (auto& e : ev) ETL_HIP(hipEventCreate(&e));
     hipEvent_t t0, t1;
     ETL_HIP(hipEventCreate(&t0));
     ETL_HIP(hipEventCreate(&t1));
     hipStream_t s = ctx().stream;
     ETL_HIP(hipEventRecord(t0, s));
-    int rc = run_steps(p, n, count, ev.data());
+    int launches = 0;
+    int rc = run_steps(p, n, count, ev.data(), &launches);
     if (rc == ETL_OK) {
         ETL_HIP(hipEventRecord(t1, s));
         ETL_HIP(hipEventSynchronize(t1));
         double sum = 0.0;
-        for (int64_t k = 0; k < count; ++k) {
+        for (int k = 0; k < launches; ++k) {
             float ms = 0.f;
             ETL_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
             sum += ms;
         }
         float tot = 0.f;
         ETL_HIP(hipEventElapsedTime(&tot, t0, t1));
-        if (kernel_ms) *kernel_ms = sum / double(count);
+        // Average duration of one fused launch (a launch of depth T covers T steps).
+        if (kernel_ms) *kernel_ms = sum / double(std::max(launches, 1));
         if (total_ms) *total_ms = tot;
         if (parity) *parity = p->parity;
     }

@@ -1,43 +1,53 @@
-// Fused 2D TE time step for gfx950: one launch advances Ex, Ey, Hz by one full step.
+// Fused, temporally blocked 2D TE time stepping for gfx950.
 //
-// Per step n (etalon/fdtd/solver.py:120-128):  H update -> M injection (t = nΔt) ->
-// E update -> J injection (t = (n+½)Δt); the monitored node values of Hz (after M)
-// and of Ex, Ey (after J) are written to a trace buffer, and a separate DFT kernel
-// folds a chunk of trace rows into the accumulators in step order.
+// One launch advances the fields T full steps.  Per step n (etalon/fdtd/solver.py:
+// 120-128): H update -> M injection (t = nΔt) -> E update -> J injection
+// (t = (n+½)Δt); the monitored values of Hz (after M) and of Ex, Ey (after J) go to
+// a trace buffer that a separate DFT kernel folds into the accumulators in step
+// order.  Every node update performs the reference's operations in the reference's
+// order (etalon/fdtd/backends/numpy_backend.py:41-61), so the results are bit-identical
+// to T single steps whatever T, the tile shape or the launch order.
 //
-// Work decomposition.  One wave (64 lanes) owns a tile of `band` rows (x) by 64*V
-// columns (y).  Lane l holds V consecutive columns c = c0 + l*V .. c+V-1 and marches
-// down the rows; Ey of the next row and Hz of the previous row stay in registers,
-// so each field value is loaded once.  Neighbours along y come from the adjacent
-// lane (ds_bpermute); the wave's first lane recomputes the Hz value of the column to
-// the left of the tile (halo), and before its first row the wave recomputes the Hz
-// row above the band.  Reads come from buffer A and writes go to buffer B
-// (ping-pong), so the halo values recomputed here are the old ones whatever order
-// the tiles run in.  Outer tangential E (Ex at j = 0, ny; Ey at i = 0, nx) is copied.
+// Decomposition.  One wave (64 lanes) owns an output tile of `band` rows (x) by
+// WO <= 64*V - 2*T columns (y).  Lane l holds the V consecutive columns
+// cs + l*V .. + V-1 of a window that extends T columns past the tile on each side
+// (cs = first output column - T).  The wave marches down rows r = i0 - T ..
+// i1 + T - 2 and, at row r, performs step t on row r - t (a skew of one row per
+// step): step t needs E^t of rows r-t and r-t+1 — the latter produced moments
+// before by step t-1 — and H^(t+½) of row r-t-1, kept from the previous row.  Each
+// row's state between steps (fields, psi, cb) stays in registers, so every array is
+// read from memory once and Ex, Ey, Hz written once per T steps.  Row r + 1 is
+// loaded while row r is computed, and no other load sits between them: gfx950
+// retires vector loads in order, so any load waited on inside the step loop would
+// also drain the prefetch.  Values outside the window are unknown; they
+// corrupt one column per half-step from each side and one row per step from the
+// top, which the T-wide halos absorb; the physical boundaries (PEC tangential E,
+// copied unchanged) need no halo.  Reads come from buffer A, writes go to buffer B,
+// so tiles never see each other's results within a launch.
 //
-// CPML.  psi is read and written only in the border strips where the 1D profiles
-// differ from (b, c, 1/kappa) = (1, 0, 1); elsewhere psi is +0 and the update
-// reduces to  d*1 + (+0)  exactly as in the reference (the +0 keeps the sign of zero).
+// CPML.  psi lives only in the border strips, where the 1D profiles differ from
+// (b, c, 1/kappa) = (1, 0, 1); there it is carried in registers between steps like
+// the fields.  Elsewhere psi is +0 and the update reduces to d*1 + (+0), exactly the
+// reference's arithmetic (the +0 fixes the sign of a zero difference).
 //
-// Tile order.  Workgroups are dispatched round-robin over the 8 XCDs; block b is
-// mapped to tile (b % 8) * tiles_per_xcd + b / 8, so each XCD sweeps one contiguous
-// slab of rows and the halo rows shared by vertically adjacent bands stay in that
-// XCD's L2.
+// Tile order.  Workgroups are dispatched round-robin over the 8 XCDs; block b takes
+// tile (b % 8) * tiles_per_xcd + b / 8, so each XCD sweeps a contiguous slab of
+// rows and the halo rows shared by vertically adjacent tiles stay in its L2.
 #include "common.h"
 #include "fused.h"
 
 namespace etl {
 namespace {
 
-// V consecutive REALs with only the alignment of one REAL: rows of Ex have an odd
-// pitch (ny + 1), and gfx950 global loads/stores need only dword alignment.
+// V consecutive REALs with the alignment of one REAL: Ex rows have an odd pitch
+// (ny + 1) and gfx950 global loads/stores need only dword alignment.
 template <typename R, int V>
 using Vec = R __attribute__((ext_vector_type(V), aligned(sizeof(R))));
 
-// Load V consecutive elements p[c..c+V-1], those with index >= limit read as 0.
+// dst[k] = p[c + k] for the elements with 0 <= c + k < limit (others 0).
 template <typename R, int V>
 __device__ __forceinline__ void load_row(R (&dst)[V], const R* __restrict__ p, int c, int limit) {
-    if (c + V <= limit) {
+    if (c >= 0 && c + V <= limit) {
         if constexpr (V == 1) {
             dst[0] = p[c];
         } else {
@@ -47,13 +57,15 @@ __device__ __forceinline__ void load_row(R (&dst)[V], const R* __restrict__ p, i
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < V; ++k) dst[k] = (c + k < limit) ? p[c + k] : R(0);
+        for (int k = 0; k < V; ++k) dst[k] = (c + k >= 0 && c + k < limit) ? p[c + k] : R(0);
     }
 }
 
+// p[c + k] = src[k] for the elements with lo <= c + k < hi.
 template <typename R, int V>
-__device__ __forceinline__ void store_row(R* __restrict__ p, const R (&src)[V], int c, int limit) {
-    if (c + V <= limit) {
+__device__ __forceinline__ void store_row(R* __restrict__ p, const R (&src)[V], int c, int lo,
+                                          int hi) {
+    if (c >= lo && c + V <= hi) {
         if constexpr (V == 1) {
             p[c] = src[0];
         } else {
@@ -65,31 +77,38 @@ __device__ __forceinline__ void store_row(R* __restrict__ p, const R (&src)[V], 
     } else {
 #pragma unroll
         for (int k = 0; k < V; ++k)
-            if (c + k < limit) p[c + k] = src[k];
+            if (c + k >= lo && c + k < hi) p[c + k] = src[k];
     }
 }
 
-template <typename R>
-__device__ __forceinline__ R shfl_from_next(R x) {  // lane l receives lane l+1's x
-    return __shfl_down(x, 1, 64);
-}
-template <typename R>
-__device__ __forceinline__ R shfl_from_prev(R x) {  // lane l receives lane l-1's x
-    return __shfl_up(x, 1, 64);
+// Elements whose mask is false read as 0 (no memory access).
+template <typename R, int V>
+__device__ __forceinline__ void load_masked(R (&dst)[V], const R* __restrict__ p, int c,
+                                            const bool (&m)[V]) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) dst[k] = m[k] ? p[c + k] : R(0);
 }
 
-// Subtract this step's deltas of the sources of component `comp` lying on row i.
+template <typename R, int V>
+__device__ __forceinline__ void store_masked(R* __restrict__ p, const R (&src)[V], int c,
+                                             const bool (&m)[V]) {
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+        if (m[k]) p[c + k] = src[k];
+}
+
+// Subtract this step's deltas of the sources of component `comp` lying on row i,
+// in source order.  x - (+0) == x bit for bit, so the untouched elements are
+// unchanged; the select keeps v[] in registers (no dynamic index).
 template <typename R, int V>
 __device__ __forceinline__ void inject_row(const FusedArgs<R>& a, const R* __restrict__ drow,
                                            int comp, int i, int c, R (&v)[V]) {
-#pragma unroll
-    for (int s = 0; s < kMaxSources; ++s) {
-        if (s < a.n_src && a.src_comp[s] == comp && a.src_i[s] == i) {
-            const int js = a.src_j[s] - c;
+    for (int s = 0; s < a.n_src; ++s) {
+        const int4 sd = a.src[s];  // comp, i, j
+        if (sd.x == comp && sd.y == i) {
+            const int js = sd.z - c;
             if (js >= 0 && js < V) {
                 const R dl = drow[s];
-                // x - (+0) == x bit for bit, so the untouched elements are unchanged;
-                // a select per element keeps v[] in registers (no dynamic index).
 #pragma unroll
                 for (int k = 0; k < V; ++k) v[k] = v[k] - ((k == js) ? dl : R(0));
             }
@@ -97,20 +116,11 @@ __device__ __forceinline__ void inject_row(const FusedArgs<R>& a, const R* __res
     }
 }
 
-template <typename R>
-__device__ __forceinline__ R inject_one(const FusedArgs<R>& a, const R* __restrict__ drow, int comp,
-                                        int i, int j, R v) {
-#pragma unroll
-    for (int s = 0; s < kMaxSources; ++s)
-        if (s < a.n_src && a.src_comp[s] == comp && a.src_i[s] == i && a.src_j[s] == j)
-            v = v - drow[s];
-    return v;
-}
-
-// Write the monitored values of row i of component `comp` into the trace row.
+// Write the monitored values of row i of component `comp` into trace row `trow`.
 template <typename R, int V>
 __device__ __forceinline__ void trace_row(const FusedArgs<R>& a, R* __restrict__ lds, int comp,
-                                          int i, int tj, int c0, const R (&v)[V], R* trow) {
+                                          int i, int tj, int cs, const R (&v)[V],
+                                          R* __restrict__ trow) {
     const int* ptr = a.mon_ptr[comp];
     if (ptr == nullptr) return;
     const int64_t cell = int64_t(i) * a.n_tj + tj;
@@ -123,30 +133,33 @@ __device__ __forceinline__ void trace_row(const FusedArgs<R>& a, R* __restrict__
     __syncthreads();
     const int2* ent = a.mon_ent[comp];
     for (int q = s + lane; q < e; q += 64) {
-        const int2 en = ent[q];
-        trow[en.y] = lds[en.x - c0];
+        const int2 en = ent[q];  // column, slot
+        trow[en.y] = lds[en.x - cs];
     }
     __syncthreads();
 }
 
-template <typename R, int V>
-__global__ __launch_bounds__(64) void fused_step_kernel(FusedArgs<R> a,
-                                                        const int* __restrict__ counter,
-                                                        int kstep, int parity) {
+template <typename R, int V, int T>
+__global__ __launch_bounds__(64) void fused_kernel(const FusedArgs<R> a,
+                                                   const int* __restrict__ counter, int kstep0,
+                                                   int parity) {
+    constexpr int W = 64 * V;
+    const int WO = a.wo;  // the same tiles for every depth T; WO + 2T <= W
     const int b = blockIdx.x;
     const int tile = (b & 7) * a.tiles_per_xcd + (b >> 3);
     if (tile >= a.n_bands * a.n_tj) return;
     const int tb = tile / a.n_tj;
     const int tj = tile - tb * a.n_tj;
     const int lane = threadIdx.x;
-    const int c0 = tj * 64 * V;
-    const int c = c0 + lane * V;
-    const int nx = a.nx, ny = a.ny;
-    const int px = ny + 1;  // Ex pitch
+    const int nx = a.nx, ny = a.ny, px = ny + 1;
+    const int o0 = tj * WO;  // first output column of the tile
+    const int cs = o0 - T;   // first column of the window
+    const int c = cs + lane * V;
     const int i0 = tb * a.band;
     const int i1 = min(i0 + a.band, nx);
+    const int rs = max(i0 - T, 0);
 
-    __shared__ R lds[64 * V];
+    __shared__ R lds[W];
 
     const R* __restrict__ exA = a.f[0][parity];
     const R* __restrict__ eyA = a.f[1][parity];
@@ -163,23 +176,25 @@ __global__ __launch_bounds__(64) void fused_step_kernel(FusedArgs<R> a,
     R* __restrict__ phxB = const_cast<R*>(a.f[5][parity ^ 1]);
     R* __restrict__ phyB = const_cast<R*>(a.f[6][parity ^ 1]);
 
-    const int trow_index = *counter + kstep;  // row of the source / phasor tables
-    const R* __restrict__ drow = a.delta + int64_t(trow_index) * a.n_src;
-    R* __restrict__ trow = a.trace + int64_t(kstep) * a.n_slots;
+    const int nbase = *counter + kstep0;  // table row of this launch's first step
+    const int flags = a.tile_flags ? int(a.tile_flags[tile]) : 3;
+    const bool has_src = (flags & 2) && a.n_src > 0;
+    const bool has_mon = (flags & 1) && a.has_mon;
 
     const R db = a.db, idx = a.inv_dx, idy = a.inv_dy;
     const R zero = R(0);
 
-    // Column profiles of this lane's columns (only read inside the y strips).
-    const bool tile_ys_h = (c0 - 1 < a.yh_lo) || (c0 + 64 * V > a.yh_hi);
-    const bool tile_ys_e = (c0 < a.ye_lo) || (c0 + 64 * V > a.ye_hi);
-    bool ysh[V], yse[V];
+    // ---- per-lane column masks and y-strip profiles ----
+    const int own_lo = o0, own_hi = o0 + WO;
+    bool mH[V], ysh[V], yse[V];
     R byh[V], cyh[V], ikyh[V], bye[V], cye[V], ikye[V];
+    const bool tile_ys = (cs < max(a.yh_lo, a.ye_lo)) || (cs + W > min(a.yh_hi, a.ye_hi));
 #pragma unroll
     for (int k = 0; k < V; ++k) {
         const int j = c + k;
-        ysh[k] = tile_ys_h && (j < a.yh_lo || j >= a.yh_hi) && j < ny;
-        yse[k] = tile_ys_e && (j < a.ye_lo || j >= a.ye_hi) && j >= 1 && j <= ny - 1;
+        mH[k] = (j >= 0 && j < ny);  // Hz / Ey column
+        ysh[k] = tile_ys && mH[k] && (j < a.yh_lo || j >= a.yh_hi);
+        yse[k] = tile_ys && j >= 1 && j <= ny - 1 && (j < a.ye_lo || j >= a.ye_hi);
         byh[k] = ysh[k] ? a.by_h[j] : R(1);
         cyh[k] = ysh[k] ? a.cy_h[j] : R(0);
         ikyh[k] = ysh[k] ? a.iky_h[j] : R(1);
@@ -187,195 +202,224 @@ __global__ __launch_bounds__(64) void fused_step_kernel(FusedArgs<R> a,
         cye[k] = yse[k] ? a.cy_e[j] : R(0);
         ikye[k] = yse[k] ? a.iky_e[j] : R(1);
     }
-    // Left halo column of the tile (first lane only).
-    const int cw = c0 - 1;
-    const bool has_left = (tj > 0);
-    const bool ysw = has_left && (cw < a.yh_lo || cw >= a.yh_hi);
-    R byw = R(1), cyw = R(0), ikyw = R(1);
-    if (lane == 0 && ysw) {
-        byw = a.by_h[cw];
-        cyw = a.cy_h[cw];
-        ikyw = a.iky_h[cw];
+    auto xs_h = [&](int i) { return i < a.xh_lo || i >= a.xh_hi; };
+    auto xs_e = [&](int i) { return i >= 1 && i <= nx - 1 && (i < a.xe_lo || i >= a.xe_hi); };
+
+    // ---- state of step t on its current row r - t (slot t) ----
+    R sex[T][V], sey[T][V], shz[T][V];                 // E^t, H^(t-½): step t's inputs
+    R hpv[T][V];                                       // H^(t+½) of the row above
+    R scbx[T][V], scby[T][V];                          // cb_ex, cb_ey of step t's row
+    R spyh[T][V], spxh[T][V], spye[T][V], spxe[T][V];  // psi inputs of step t
+    R eyn0[V];                                         // Ey^0 of row r + 1
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            sex[t][k] = sey[t][k] = shz[t][k] = hpv[t][k] = scbx[t][k] = scby[t][k] = R(0);
+            spyh[t][k] = spxh[t][k] = spye[t][k] = spxe[t][k] = R(0);
+        }
+
+    // row rs into slot 0
+    load_row<R, V>(sex[0], exA + int64_t(rs) * px, c, px);
+    load_row<R, V>(shz[0], hzA + int64_t(rs) * ny, c, ny);
+    load_row<R, V>(sey[0], eyA + int64_t(rs) * ny, c, ny);
+    load_row<R, V>(eyn0, eyA + int64_t(rs + 1) * ny, c, ny);
+    load_row<R, V>(scbx[0], a.cb_ex + int64_t(rs) * px, c, px);
+    load_row<R, V>(scby[0], a.cb_ey + int64_t(rs) * ny, c, ny);
+    if (tile_ys) {
+        load_masked<R, V>(spyh[0], phyA + int64_t(rs) * ny, c, ysh);
+        load_masked<R, V>(spye[0], pexA + int64_t(rs) * px, c, yse);
     }
+    if (xs_h(rs)) load_row<R, V>(spxh[0], phxA + int64_t(rs) * ny, c, ny);
+    if (xs_e(rs)) load_row<R, V>(spxe[0], peyA + int64_t(rs) * ny, c, ny);
 
-    // Hz update of one node; psi is read from A and, when `store`, written to B.
-    auto h_node = [&](R hz, R d, R e, bool ys, R by, R cy, R iky, bool xs, R bx, R cx, R ikx,
-                      int64_t k_hz, bool store) -> R {
-        R ty, tx;
-        if (ys) {
-            const R py = by * phyA[k_hz] + cy * d;
-            if (store) phyB[k_hz] = py;
-            ty = d * iky + py;
-        } else {
-            ty = d + zero;
-        }
-        if (xs) {
-            const R pxv = bx * phxA[k_hz] + cx * e;
-            if (store) phxB[k_hz] = pxv;
-            tx = e * ikx + pxv;
-        } else {
-            tx = e + zero;
-        }
-        return hz + db * (ty - tx);
-    };
-
-    R hprev[V];  // Hz^(n+1/2) of the previous row
-    R eyC[V];    // Ey^n of the current row
-    R eyCw = R(0);
-
-    // ---- prologue: Ey of the first row, and the Hz row above the band (halo) ----
-    load_row<R, V>(eyC, eyA + int64_t(i0) * ny, c, ny);
-    if (lane == 0 && has_left) eyCw = eyA[int64_t(i0) * ny + cw];
-    if (i0 > 0) {
-        const int ih = i0 - 1;
-        R exh[V], eyh[V], hzh[V];
-        load_row<R, V>(exh, exA + int64_t(ih) * px, c, px);
-        load_row<R, V>(eyh, eyA + int64_t(ih) * ny, c, ny);
-        load_row<R, V>(hzh, hzA + int64_t(ih) * ny, c, ny);
-        R exr = shfl_from_next(exh[0]);
-        if (lane == 63 && c + V <= ny) exr = exA[int64_t(ih) * px + c + V];
-        const bool xs = (ih < a.xh_lo || ih >= a.xh_hi);
-        R bx = R(1), cx = R(0), ikx = R(1);
-        if (xs) {
-            bx = a.bx_h[ih];
-            cx = a.cx_h[ih];
-            ikx = a.ikx_h[ih];
-        }
+    const int r_end = i1 + T - 1;
+    for (int r = rs; r < r_end; ++r) {
+        // ---- prefetch row r + 1 for step 0 of the next row ----
+        const int rn = r + 1;
+        R pex[V], phz[V], pey[V], ppyh[V], ppxh[V], ppye[V], ppxe[V], pcbx[V], pcby[V];
 #pragma unroll
-        for (int k = 0; k < V; ++k) {
-            const R exn = (k + 1 < V) ? exh[k + 1] : exr;
-            const R d = (exn - exh[k]) * idy;
-            const R e = (eyC[k] - eyh[k]) * idx;
-            hprev[k] = (c + k < ny)
-                           ? h_node(hzh[k], d, e, ysh[k], byh[k], cyh[k], ikyh[k], xs, bx, cx,
-                                    ikx, int64_t(ih) * ny + c + k, false)
-                           : R(0);
-        }
-        if (a.n_src) inject_row<R, V>(a, drow, 2, ih, c, hprev);
-    } else {
-#pragma unroll
-        for (int k = 0; k < V; ++k) hprev[k] = R(0);
-    }
-
-    // ---- march down the band ----
-    for (int i = i0; i < i1; ++i) {
-        R ex[V], hz[V], eyN[V], cbx[V], cby[V] = {};
-        const int64_t rx = int64_t(i) * px;
-        const int64_t rh = int64_t(i) * ny;
-        load_row<R, V>(ex, exA + rx, c, px);
-        load_row<R, V>(hz, hzA + rh, c, ny);
-        load_row<R, V>(eyN, eyA + rh + ny, c, ny);
-        load_row<R, V>(cbx, a.cb_ex + rx, c, px);
-        if (i >= 1) load_row<R, V>(cby, a.cb_ey + rh, c, ny);
-        R exr = shfl_from_next(ex[0]);
-        if (lane == 63 && c + V <= ny) exr = exA[rx + c + V];
-
-        const bool xsh = (i < a.xh_lo || i >= a.xh_hi);
-        R bx = R(1), cx = R(0), ikx = R(1);
-        if (xsh) {
-            bx = a.bx_h[i];
-            cx = a.cx_h[i];
-            ikx = a.ikx_h[i];
-        }
-
-        // H half-step
-        R hn[V];
-#pragma unroll
-        for (int k = 0; k < V; ++k) {
-            const R exn = (k + 1 < V) ? ex[k + 1] : exr;
-            const R d = (exn - ex[k]) * idy;
-            const R e = (eyN[k] - eyC[k]) * idx;
-            hn[k] = (c + k < ny) ? h_node(hz[k], d, e, ysh[k], byh[k], cyh[k], ikyh[k], xsh, bx,
-                                          cx, ikx, rh + c + k, true)
-                                 : R(0);
-        }
-        if (a.n_src) inject_row<R, V>(a, drow, 2, i, c, hn);
-
-        // Hz^(n+1/2) at column c - 1
-        R hl = shfl_from_prev(hn[V - 1]);
-        R eyNw = R(0);
-        if (lane == 0 && has_left) {
-            const R exw = exA[rx + cw];
-            const R hzw = hzA[rh + cw];
-            eyNw = eyA[rh + ny + cw];
-            const R d = (ex[0] - exw) * idy;
-            const R e = (eyNw - eyCw) * idx;
-            hl = h_node(hzw, d, e, ysw, byw, cyw, ikyw, xsh, bx, cx, ikx, rh + cw, false);
-            if (a.n_src) hl = inject_one(a, drow, 2, i, cw, hl);
-        }
-
-        // E half-step
-        const bool xse = (i < a.xe_lo || i >= a.xe_hi);
-        R bxe = R(1), cxe = R(0), ikxe = R(1);
-        if (xse) {
-            bxe = a.bx_e[i];
-            cxe = a.cx_e[i];
-            ikxe = a.ikx_e[i];
-        }
-        R exo[V], eyo[V];
-#pragma unroll
-        for (int k = 0; k < V; ++k) {
-            const int j = c + k;
-            // Ex(i, j), j in [1, ny - 1]
-            if (j >= 1 && j <= ny - 1) {
-                const R hleft = (k == 0) ? hl : hn[k - 1];
-                const R d = (hn[k] - hleft) * idy;
-                R t;
-                if (yse[k]) {
-                    const R p = bye[k] * pexA[rx + j] + cye[k] * d;
-                    pexB[rx + j] = p;
-                    t = d * ikye[k] + p;
-                } else {
-                    t = d + zero;
-                }
-                exo[k] = ex[k] + cbx[k] * t;
-            } else {
-                exo[k] = ex[k];
+        for (int k = 0; k < V; ++k)
+            pex[k] = phz[k] = pey[k] = ppyh[k] = ppxh[k] = ppye[k] = ppxe[k] = pcbx[k] =
+                pcby[k] = R(0);
+        if (rn < nx) {
+            load_row<R, V>(pex, exA + int64_t(rn) * px, c, px);
+            load_row<R, V>(phz, hzA + int64_t(rn) * ny, c, ny);
+            load_row<R, V>(pcbx, a.cb_ex + int64_t(rn) * px, c, px);
+            load_row<R, V>(pcby, a.cb_ey + int64_t(rn) * ny, c, ny);
+            if (tile_ys) {
+                load_masked<R, V>(ppyh, phyA + int64_t(rn) * ny, c, ysh);
+                load_masked<R, V>(ppye, pexA + int64_t(rn) * px, c, yse);
             }
-            // Ey(i, j), i in [1, nx - 1]
-            if (i >= 1 && j < ny) {
-                const R d = (hn[k] - hprev[k]) * idx;
-                R t;
-                if (xse) {
-                    const R p = bxe * peyA[rh + j] + cxe * d;
-                    peyB[rh + j] = p;
-                    t = d * ikxe + p;
-                } else {
-                    t = d + zero;
-                }
-                eyo[k] = eyC[k] - cby[k] * t;
-            } else {
-                eyo[k] = eyC[k];
-            }
+            if (xs_h(rn)) load_row<R, V>(ppxh, phxA + int64_t(rn) * ny, c, ny);
+            if (xs_e(rn)) load_row<R, V>(ppxe, peyA + int64_t(rn) * ny, c, ny);
         }
-        if (a.n_src) {
-            inject_row<R, V>(a, drow, 0, i, c, exo);
-            inject_row<R, V>(a, drow, 1, i, c, eyo);
-        }
-
-        store_row<R, V>(hzB + rh, hn, c, ny);
-        store_row<R, V>(exB + rx, exo, c, px);
-        store_row<R, V>(eyB + rh, eyo, c, ny);
-
-        if (a.has_mon) {
-            trace_row<R, V>(a, lds, 2, i, tj, c0, hn, trow);
-            trace_row<R, V>(a, lds, 0, i, tj, c0, exo, trow);
-            trace_row<R, V>(a, lds, 1, i, tj, c0, eyo, trow);
-        }
+        if (rn + 1 <= nx) load_row<R, V>(pey, eyA + int64_t(rn + 1) * ny, c, ny);
 
 #pragma unroll
-        for (int k = 0; k < V; ++k) {
-            hprev[k] = hn[k];
-            eyC[k] = eyN[k];
+        for (int t = 0; t < T; ++t) {
+            const int rho = r - t;
+            if (rho < rs || rho >= nx) continue;  // wave-uniform
+            const int64_t rx = int64_t(rho) * px;
+            const int64_t rh = int64_t(rho) * ny;
+            // E^t of row rho + 1: from memory for step 0, else step t-1's output of this
+            // row; Ey row nx is outer tangential E and never changes.
+            R eyN[V];
+            if (t == 0) {
+#pragma unroll
+                for (int k = 0; k < V; ++k) eyN[k] = eyn0[k];
+            } else if (rho + 1 == nx) {
+                load_row<R, V>(eyN, eyA + int64_t(nx) * ny, c, ny);
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; ++k) eyN[k] = sey[t > 0 ? t - 1 : 0][k];
+            }
+
+            // ---- H half-step on row rho ----
+            const bool xh = xs_h(rho);
+            R bx = R(1), cx = R(0), ikx = R(1);
+            if (xh) {
+                bx = a.bx_h[rho];
+                cx = a.cx_h[rho];
+                ikx = a.ikx_h[rho];
+            }
+            const R exr = __shfl_down(sex[t][0], 1, 64);  // lane 63: outside the window
+            R hn[V];
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const R exn = (k + 1 < V) ? sex[t][k + 1] : exr;
+                const R d = (exn - sex[t][k]) * idy;
+                const R e = (eyN[k] - sey[t][k]) * idx;
+                R ty, tx;
+                if (ysh[k]) {
+                    const R py = byh[k] * spyh[t][k] + cyh[k] * d;
+                    spyh[t][k] = py;
+                    ty = d * ikyh[k] + py;
+                } else {
+                    ty = d + zero;
+                }
+                if (xh) {
+                    const R pxv = bx * spxh[t][k] + cx * e;
+                    spxh[t][k] = pxv;
+                    tx = e * ikx + pxv;
+                } else {
+                    tx = e + zero;
+                }
+                hn[k] = shz[t][k] + db * (ty - tx);
+            }
+            const R* drow = a.delta + int64_t(nbase + t) * a.n_src;
+            if (has_src) inject_row<R, V>(a, drow, 2, rho, c, hn);
+
+            // ---- E half-step on row rho ----
+            const R hl = __shfl_up(hn[V - 1], 1, 64);  // lane 0: outside the window
+            const bool xe = xs_e(rho);
+            R bxe = R(1), cxe = R(0), ikxe = R(1);
+            if (xe) {
+                bxe = a.bx_e[rho];
+                cxe = a.cx_e[rho];
+                ikxe = a.ikx_e[rho];
+            }
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const int j = c + k;
+                if (j >= 1 && j <= ny - 1) {  // Ex(rho, j); j = 0, ny stay unchanged
+                    const R hleft = (k == 0) ? hl : hn[k - 1];
+                    const R d = (hn[k] - hleft) * idy;
+                    R tt;
+                    if (yse[k]) {
+                        const R p = bye[k] * spye[t][k] + cye[k] * d;
+                        spye[t][k] = p;
+                        tt = d * ikye[k] + p;
+                    } else {
+                        tt = d + zero;
+                    }
+                    sex[t][k] = sex[t][k] + scbx[t][k] * tt;
+                }
+                if (rho >= 1) {  // Ey(rho, j), rho <= nx - 1; row 0 stays unchanged
+                    const R d = (hn[k] - hpv[t][k]) * idx;
+                    R tt;
+                    if (xe) {
+                        const R p = bxe * spxe[t][k] + cxe * d;
+                        spxe[t][k] = p;
+                        tt = d * ikxe + p;
+                    } else {
+                        tt = d + zero;
+                    }
+                    sey[t][k] = sey[t][k] - scby[t][k] * tt;
+                }
+            }
+            if (has_src) {
+                inject_row<R, V>(a, drow, 0, rho, c, sex[t]);
+                inject_row<R, V>(a, drow, 1, rho, c, sey[t]);
+            }
+            const bool owned_row = rho >= i0 && rho < i1;
+            if (has_mon && owned_row) {
+                R* trow = a.trace + int64_t(kstep0 + t) * a.n_slots;
+                trace_row<R, V>(a, lds, 2, rho, tj, cs, hn, trow);
+                trace_row<R, V>(a, lds, 0, rho, tj, cs, sex[t], trow);
+                trace_row<R, V>(a, lds, 1, rho, tj, cs, sey[t], trow);
+            }
+            if (t == T - 1 && owned_row) {
+                store_row<R, V>(hzB + rh, hn, c, max(own_lo, 0), min(own_hi, ny));
+                store_row<R, V>(exB + rx, sex[t], c, max(own_lo, 0), min(own_hi, px));
+                store_row<R, V>(eyB + rh, sey[t], c, max(own_lo, 0), min(own_hi, ny));
+                bool mo[V], moy[V], moe[V];
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const bool own = (c + k >= own_lo && c + k < own_hi);
+                    mo[k] = own && mH[k];
+                    moy[k] = own && ysh[k];
+                    moe[k] = own && yse[k];
+                }
+                if (tile_ys) {
+                    store_masked<R, V>(phyB + rh, spyh[t], c, moy);
+                    store_masked<R, V>(pexB + rx, spye[t], c, moe);
+                }
+                if (xh) store_masked<R, V>(phxB + rh, spxh[t], c, mo);
+                if (xe) store_masked<R, V>(peyB + rh, spxe[t], c, mo);
+            }
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                shz[t][k] = hn[k];  // step t+1's H input after the shift below
+                hpv[t][k] = hn[k];
+            }
         }
-        eyCw = eyNw;
+
+        // ---- shift: step t's outputs become step t+1's inputs; row r+1 enters slot 0 ----
+#pragma unroll
+        for (int t = T - 1; t >= 1; --t)
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                sex[t][k] = sex[t - 1][k];
+                sey[t][k] = sey[t - 1][k];
+                shz[t][k] = shz[t - 1][k];
+                scbx[t][k] = scbx[t - 1][k];
+                scby[t][k] = scby[t - 1][k];
+                spyh[t][k] = spyh[t - 1][k];
+                spxh[t][k] = spxh[t - 1][k];
+                spye[t][k] = spye[t - 1][k];
+                spxe[t][k] = spxe[t - 1][k];
+            }
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            sex[0][k] = pex[k];
+            shz[0][k] = phz[k];
+            sey[0][k] = eyn0[k];
+            eyn0[k] = pey[k];
+            scbx[0][k] = pcbx[k];
+            scby[0][k] = pcby[k];
+            spyh[0][k] = ppyh[k];
+            spxh[0][k] = ppxh[k];
+            spye[0][k] = ppye[k];
+            spxe[0][k] = ppxe[k];
+        }
     }
 
-    // ---- last band: Ey row nx is outer tangential E (copied) ----
+    // ---- last band: Ey row nx (outer tangential E) is copied ----
     if (i1 == nx) {
-        if (a.n_src) inject_row<R, V>(a, drow, 1, nx, c, eyC);
-        store_row<R, V>(eyB + int64_t(nx) * ny, eyC, c, ny);
-        if (a.has_mon) trace_row<R, V>(a, lds, 1, nx, tj, c0, eyC, trow);
+        R eyl[V];
+        load_row<R, V>(eyl, eyA + int64_t(nx) * ny, c, ny);
+        store_row<R, V>(eyB + int64_t(nx) * ny, eyl, c, max(own_lo, 0), min(own_hi, ny));
     }
 }
 
@@ -408,33 +452,47 @@ __global__ void bump_counter_kernel(int* counter, int by) {
     if (threadIdx.x == 0) counter[0] = counter[0] + by;
 }
 
-}  // namespace
-
-template <typename R>
-int launch_fused_step(const FusedArgs<R>& a, int vec, const int* counter, int kstep, int parity,
-                      hipStream_t stream) {
-    const int n_tiles = a.n_bands * a.n_tj;
-    const int grid = 8 * a.tiles_per_xcd;
-    (void)n_tiles;
-    switch (vec) {
+template <typename R, int V>
+int launch_v(const FusedArgs<R>& a, int depth, const int* counter, int kstep0, int parity,
+             hipStream_t stream) {
+    const dim3 grid(unsigned(8 * a.tiles_per_xcd)), block(64);
+    switch (depth) {
         case 1:
-            hipLaunchKernelGGL((fused_step_kernel<R, 1>), dim3(grid), dim3(64), 0, stream, a,
-                               counter, kstep, parity);
+            hipLaunchKernelGGL((fused_kernel<R, V, 1>), grid, block, 0, stream, a, counter, kstep0,
+                               parity);
             break;
         case 2:
-            hipLaunchKernelGGL((fused_step_kernel<R, 2>), dim3(grid), dim3(64), 0, stream, a,
-                               counter, kstep, parity);
+            hipLaunchKernelGGL((fused_kernel<R, V, 2>), grid, block, 0, stream, a, counter, kstep0,
+                               parity);
             break;
         case 4:
-            hipLaunchKernelGGL((fused_step_kernel<R, 4>), dim3(grid), dim3(64), 0, stream, a,
-                               counter, kstep, parity);
+            hipLaunchKernelGGL((fused_kernel<R, V, 4>), grid, block, 0, stream, a, counter, kstep0,
+                               parity);
             break;
         default:
-            set_error("fused step: unsupported vector width %d", vec);
+            set_error("fused step: unsupported temporal depth %d", depth);
             return ETL_EARG;
     }
     ETL_LAUNCHED();
     return ETL_OK;
+}
+
+}  // namespace
+
+template <typename R>
+int launch_fused(const FusedArgs<R>& a, int vec, int depth, const int* counter, int kstep0,
+                 int parity, hipStream_t stream) {
+    switch (vec) {
+        case 1:
+            return launch_v<R, 1>(a, depth, counter, kstep0, parity, stream);
+        case 2:
+            return launch_v<R, 2>(a, depth, counter, kstep0, parity, stream);
+        case 4:
+            return launch_v<R, 4>(a, depth, counter, kstep0, parity, stream);
+        default:
+            set_error("fused step: unsupported vector width %d", vec);
+            return ETL_EARG;
+    }
 }
 
 template <typename R>
@@ -454,10 +512,10 @@ int launch_bump(int* counter, int by, hipStream_t stream) {
     return ETL_OK;
 }
 
-template int launch_fused_step<float>(const FusedArgs<float>&, int, const int*, int, int,
-                                      hipStream_t);
-template int launch_fused_step<double>(const FusedArgs<double>&, int, const int*, int, int,
-                                       hipStream_t);
+template int launch_fused<float>(const FusedArgs<float>&, int, int, const int*, int, int,
+                                 hipStream_t);
+template int launch_fused<double>(const FusedArgs<double>&, int, int, const int*, int, int,
+                                  hipStream_t);
 template int launch_trace_dft<float>(const float*, int, int, const float*, int, const int*,
                                      const int4*, int, float* const*, hipStream_t);
 template int launch_trace_dft<double>(const double*, int, int, const double*, int, const int*,

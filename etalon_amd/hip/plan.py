@@ -23,11 +23,14 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h
 
 @dataclass
 class Tune:
+    """Fused-kernel shape; 0 = the library default (float32: vec 4, float64: vec 2;
+    temporal depth 4, 32-row bands, 64-step trace chunks)."""
+
     band_rows: int = 0
     vec: int = 0
     use_graph: int = 1
     trace_chunk: int = 0
-    temporal: int = 1
+    temporal: int = 0
 
     def as_struct(self) -> _lib.EtlTune:
         return _lib.EtlTune(self.band_rows, self.vec, self.use_graph, self.trace_chunk,
@@ -109,8 +112,15 @@ class FusedPlan:
         self.parity = parity.value
         return self.parity
 
+    def resolved_tune(self) -> Tune:
+        """The tuning in effect (library defaults filled in)."""
+        t = _lib.EtlTune()
+        _lib.call("etl_plan_tune", self.handle, C.byref(t))
+        return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal)
+
     def time(self, n0: int, count: int) -> tuple[float, float]:
-        """(average fused-kernel duration, total) in ms over ``count`` steps, HIP events."""
+        """(average duration of one fused launch, total) in ms over ``count`` steps,
+        from HIP events on the library stream; a launch covers ``temporal`` steps."""
         k, t, parity = C.c_double(), C.c_double(), C.c_int()
         _lib.call("etl_plan_time", self.handle, int(n0), int(count), C.byref(k), C.byref(t),
                   C.byref(parity))

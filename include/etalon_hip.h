@@ -144,6 +144,8 @@ int etl_plan_add_monitor(etl_plan_t plan, int comp, int n_nodes, const int64_t* 
  * ph[n][f] is the factor of the H sample of step n when comp = Hz and of the E
  * sample otherwise (etalon/fdtd/solver.py:91-110). */
 int etl_plan_phasor_table(etl_plan_t plan, int id, int64_t n0, int rows, const void* ph);
+/* The tuning the plan resolved (library defaults filled in). */
+int etl_plan_tune(etl_plan_t plan, EtlTune* out);
 /* Advance steps [n, n + count); the tables must cover them.  On return *parity
  * tells which buffer of each pair holds the state (0: field[k][0]). */
 int etl_plan_advance(etl_plan_t plan, int64_t n, int64_t count, int* parity);

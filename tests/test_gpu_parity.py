@@ -140,3 +140,34 @@ def test_vacuum_pulse_2000_steps_bitwise(hip, precision, dtype):
     for mon in sim.monitors:
         for comp in ("Ex", "Ey", "Hz"):
             assert bitwise_equal(result[mon.name][comp].values, values[(mon.name, comp)])
+
+
+_ORACLE_CACHE = {}
+
+
+def _oracle_full(precision, steps):
+    key = (precision, steps)
+    if key not in _ORACLE_CACHE:
+        _ORACLE_CACHE[key] = ref.Solver(full_sim(precision)).run(steps)
+    return _ORACLE_CACHE[key]
+
+
+# (vec, temporal depth, band rows): every shape is its own code path
+TUNES = [(1, 1, 1), (2, 2, 3), (2, 4, 7), (1, 4, 16), (4, 1, 5), (4, 2, 8), (2, 4, 32), (4, 4, 32)]
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+@pytest.mark.parametrize(("vec", "depth", "band"), TUNES)
+def test_fused_tile_shapes_bitwise(hip, precision, dtype, vec, depth, band):
+    """Temporal blocking depth, lane width and band height never change a bit."""
+    from etalon_amd.hip.plan import Tune
+    from etalon_amd.solver import FDTDSolver
+
+    if precision == "double" and vec == 4 and depth == 4:
+        pytest.skip("float64 with vec 4 and depth 4 exceeds the register budget (not offered)")
+    sim = full_sim(precision)
+    gpu = FDTDSolver(sim, tune=Tune(band_rows=band, vec=vec, temporal=depth,
+                                    trace_chunk=8 * depth, use_graph=1))
+    for count in (3, 64, 129, 204):  # 400 steps: graph chunks, depth-1 tails, odd launches
+        gpu.advance(count)
+    _compare_solver(gpu, _oracle_full(precision, 400))
