@@ -26,7 +26,7 @@ ARCH = "gfx950"
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero",
     "-fno-fast-math",

@@ -31,6 +31,10 @@ Context& ctx();
 int require_init();
 // Stream-ordered zero fill with a kernel (safe inside hipGraph capture).
 int zero_async(void* dst, size_t bytes);
+// Valid device memory that etl_malloc places before and after every array.
+constexpr size_t kGuard = 256;
+// Bytes asked for when `p` is the start of a live etl_malloc array, else 0.
+size_t guarded_bytes(const void* p);
 
 }  // namespace etl
 

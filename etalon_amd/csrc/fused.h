@@ -30,13 +30,12 @@ struct FusedArgs {
     int n_src;
     const int4* src;
     const R* delta;
-    // Per tile: bit 0 = monitored nodes in its rows, bit 1 = a source in its window.
-    const unsigned char* tile_flags;
-    // Monitor trace: per component a CSR over (row * n_tj + tile column) of entries
-    // (column, slot); the trace holds one row of n_slots REALs per step of a chunk.
+    // Monitor trace: per component a CSR over tiles of entries (row, column, slot,
+    // 0) of the tile's owned nodes; the trace holds one row of n_slots REALs per step
+    // of a chunk.
     int has_mon;
-    const int* mon_ptr[3];
-    const int2* mon_ent[3];
+    const int* tile_mon[3];
+    const int4* tile_ent[3];
     R* trace;
     int n_slots;
 };
@@ -44,9 +43,45 @@ struct FusedArgs {
 // Output columns of a tile for vector width v and temporal depth t.
 inline int tile_out_cols(int v, int t) { return 64 * v - 2 * t; }
 
+// Tile descriptors (first row, tile column, monitor cell, 0) split by kernel
+// variant: interior tiles carry no psi and are `band` rows high; strip tiles touch a
+// CPML strip and carry psi; extra tiles also inject sources or trace monitors.
+// Strip and extra tiles are kSubBand rows high: they are few, so splitting them
+// shortens the longest wave of a launch.
+constexpr int kSubBand = 4;
+struct TileList {
+    const int4* interior;
+    int n_interior;
+    const int4* strip;
+    int n_strip;
+    const int4* extra;
+    int n_extra;
+};
+
+// Register-tile shapes compiled into the library: (precision, vec, depth, band rows).
+// Every shape of depth T > 1 comes with its depth-1 sibling for step-count tails.
+struct TileShape {
+    int precision, vec, depth, band;
+};
+constexpr TileShape kTileShapes[] = {
+    {0, 1, 1, 8}, {0, 1, 2, 8},  {0, 1, 4, 8},  {0, 1, 1, 16}, {0, 1, 2, 16}, {0, 1, 4, 16},
+    {0, 2, 1, 8}, {0, 2, 2, 8},  {0, 2, 4, 8},  {1, 1, 1, 8},  {1, 1, 2, 8},  {1, 1, 4, 8},
+    {1, 1, 1, 16}, {1, 1, 2, 16},
+};
+inline bool tile_shape_supported(int precision, int vec, int depth, int band) {
+    for (const TileShape& s : kTileShapes)
+        if (s.precision == precision && s.vec == vec && s.depth == depth && s.band == band)
+            return true;
+    return false;
+}
+
+// Launch one fused step of depth T over all tiles: the interior variant on s_int, the
+// strip variant on s_strip and the extra variant on s_extra (the caller orders the
+// streams).
 template <typename R>
-int launch_fused(const FusedArgs<R>& a, int vec, int depth, const int* counter, int kstep0,
-                 int parity, hipStream_t stream);
+int launch_fused(const FusedArgs<R>& a, const TileList& tl, int vec, int depth, int band,
+                 const int* counter, int kstep0, int parity, hipStream_t s_int,
+                 hipStream_t s_strip, hipStream_t s_extra);
 template <typename R>
 int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq,
                      const int* counter, const int4* pairs, int n_pairs, R* const* accs,

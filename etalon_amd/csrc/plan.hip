@@ -89,9 +89,12 @@ struct EtlPlan {
     std::vector<Mon> mons;
     bool dirty = true;  // monitor CSR / tile flags need a rebuild
     int n_slots = 0, nq = 0, n_pairs = 0;
-    int* mon_ptr[3] = {nullptr, nullptr, nullptr};
-    int2* mon_ent[3] = {nullptr, nullptr, nullptr};
-    unsigned char* tile_flags = nullptr;
+    int* tile_mon[3] = {nullptr, nullptr, nullptr};   // per component: CSR over tiles
+    int4* tile_ent[3] = {nullptr, nullptr, nullptr};  // (row, column, slot, 0)
+    int4* tiles_dev = nullptr;  // interior, then strip, then extra tile descriptors
+    int n_interior = 0, n_strip = 0, n_extra = 0;
+    hipStream_t s_strip = nullptr, s_extra = nullptr;  // beside the interior tiles
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
     void* trace = nullptr;
     int4* pairs = nullptr;
     void** accs = nullptr;
@@ -123,19 +126,19 @@ void free_graphs(EtlPlan* p) {
 
 void free_layout(EtlPlan* p) {
     for (int c = 0; c < 3; ++c) {
-        etl_free(p->mon_ptr[c]);
-        etl_free(p->mon_ent[c]);
-        p->mon_ptr[c] = nullptr;
-        p->mon_ent[c] = nullptr;
+        etl_free(p->tile_mon[c]);
+        etl_free(p->tile_ent[c]);
+        p->tile_mon[c] = nullptr;
+        p->tile_ent[c] = nullptr;
     }
     etl_free(p->trace);
     etl_free(p->pairs);
     etl_free(p->accs);
-    etl_free(p->tile_flags);
+    etl_free(p->tiles_dev);
+    p->tiles_dev = nullptr;
     p->trace = nullptr;
     p->pairs = nullptr;
     p->accs = nullptr;
-    p->tile_flags = nullptr;
 }
 
 int upload(void** dst, const void* src, size_t bytes) {
@@ -144,8 +147,14 @@ int upload(void** dst, const void* src, size_t bytes) {
     return etl_memcpy_h2d(*dst, src, bytes);
 }
 
-// Device CSR of monitored nodes per (row, tile column), DFT pairs, the trace buffer
-// and the per-tile flags; rebuilt whenever sources, monitors or tiling change.
+// Tile descriptors of the three kernel variants, the monitor CSR over extra tiles,
+// DFT pairs and the trace buffer; rebuilt whenever sources, monitors or tiling change.
+//
+// A full tile is `band` rows by WO columns.  Its window — rows [i0 - T, i0 + band + T]
+// and columns [o0 - T, o0 - T + W) — decides its variant: interior when it meets no
+// CPML strip, no monitored node it owns and no source; otherwise it is split into
+// kSubBand-row sub-tiles, each extra when it owns a monitored node or its own window
+// holds a source, else strip.
 int build_layout(EtlPlan* p) {
     ETL_HIP(hipStreamSynchronize(ctx().stream));
     free_layout(p);
@@ -155,11 +164,17 @@ int build_layout(EtlPlan* p) {
     const int wo = p->wo();
     const int T = p->tune.temporal;
     const int W = 64 * p->tune.vec;
-    int slot = 0, q = 0;
-    std::vector<std::vector<std::pair<int64_t, int2>>> ent(3);  // (cell, (col, slot))
+    const int band = p->tune.band_rows;
+    const int nx = p->nx;
+
+    // monitored nodes: slots, DFT pairs
+    struct Node {
+        int comp, row, col, slot;
+    };
+    std::vector<Node> nodes;
     std::vector<int4> pairs;
     std::vector<void*> accs;
-    std::vector<unsigned char> flags(size_t(nb) * ntj, 0);
+    int slot = 0, q = 0;
     for (size_t m = 0; m < p->mons.size(); ++m) {
         auto& mon = p->mons[m];
         mon.slot_base = slot;
@@ -168,11 +183,7 @@ int build_layout(EtlPlan* p) {
         for (int k = 0; k < mon.n_nodes; ++k) {
             const int64_t fl = mon.flat[k];
             const int row = int(fl / pitch);
-            const int col = int(fl - int64_t(row) * pitch);
-            const int tj = col / wo;
-            ent[mon.comp].push_back({int64_t(row) * ntj + tj, make_int2(col, slot + k)});
-            const int tb = std::min(row, p->nx - 1) / p->tune.band_rows;
-            flags[size_t(tb) * ntj + tj] |= 1;
+            nodes.push_back({mon.comp, row, int(fl - int64_t(row) * pitch), slot + k});
         }
         for (int f = 0; f < mon.n_f; ++f)
             for (int k = 0; k < mon.n_nodes; ++k)
@@ -181,18 +192,51 @@ int build_layout(EtlPlan* p) {
         slot += mon.n_nodes;
         q += mon.n_f;
     }
-    // A source matters to every tile whose window (rows from i0 - T, columns
-    // [o0 - T, o0 - T + W)) contains its node.
-    for (const int4& s : p->src) {
-        for (int tb = 0; tb < nb; ++tb) {
-            const int i0 = tb * p->tune.band_rows;
-            const int i1 = std::min(i0 + p->tune.band_rows, p->nx);
-            if (s.y < std::max(i0 - T, 0) || s.y > i1 + T) continue;
-            for (int tj = 0; tj < ntj; ++tj) {
-                const int cs = tj * wo - T;
-                if (s.z >= cs && s.z < cs + W) flags[size_t(tb) * ntj + tj] |= 2;
+
+    const int* st = p->strips;
+    const int x_lo = std::max(st[0], st[2]), x_hi = std::min(st[1], st[3]);
+    const int y_lo = std::max(st[4], st[6]), y_hi = std::min(st[5], st[7]);
+    auto owns_node = [&](int r0, int r1, int tj) {
+        for (const Node& n : nodes)
+            if (n.row >= r0 && n.row < r1 && n.col / wo == tj) return true;
+        return false;
+    };
+    auto sees_source = [&](int r0, int r1, int tj) {
+        const int cs = tj * wo - T;
+        for (const int4& s : p->src)
+            if (s.y >= r0 - T && s.y <= r1 + T && s.z >= cs && s.z < cs + W) return true;
+        return false;
+    };
+    std::vector<int4> interior, strip, extra;
+    for (int tb = 0; tb < nb; ++tb) {
+        const int i0 = tb * band;
+        const int i1 = std::min(i0 + band, nx);
+        const bool rows_touch = (i0 - T < x_lo) || (i1 + T > x_hi);
+        for (int tj = 0; tj < ntj; ++tj) {
+            const int cs = tj * wo - T;
+            const bool cols_touch = (cs < y_lo) || (cs + W > y_hi);
+            if (!rows_touch && !cols_touch && !owns_node(i0, i1, tj) && !sees_source(i0, i1, tj)) {
+                interior.push_back(make_int4(i0, tj, 0, 0));
+                continue;
+            }
+            for (int r0 = i0; r0 < i1; r0 += kSubBand) {
+                const int r1 = std::min(r0 + kSubBand, nx);
+                if (owns_node(r0, r1, tj) || sees_source(r0, r1, tj))
+                    extra.push_back(make_int4(r0, tj, int(extra.size()), 0));
+                else
+                    strip.push_back(make_int4(r0, tj, 0, 0));
             }
         }
+    }
+    // monitor entries per extra sub-tile (row, column, slot)
+    std::vector<std::vector<std::pair<int, int4>>> ent(3);
+    for (const Node& n : nodes) {
+        const int tj = n.col / wo;
+        for (const int4& e : extra)
+            if (e.y == tj && n.row >= e.x && n.row < e.x + kSubBand) {
+                ent[n.comp].push_back({e.z, make_int4(n.row, n.col, n.slot, 0)});
+                break;
+            }
     }
     p->n_slots = slot;
     p->nq = q;
@@ -200,19 +244,24 @@ int build_layout(EtlPlan* p) {
     int rc;
     for (int c = 0; c < 3; ++c) {
         if (ent[c].empty()) continue;
-        const int64_t cells = int64_t(p->rows_of(c)) * ntj;
         std::stable_sort(ent[c].begin(), ent[c].end(),
                          [](const auto& a, const auto& b) { return a.first < b.first; });
-        std::vector<int> ptr(cells + 1, 0);
+        std::vector<int> ptr(extra.size() + 1, 0);
         for (auto& e : ent[c]) ptr[e.first + 1]++;
-        for (int64_t k = 0; k < cells; ++k) ptr[k + 1] += ptr[k];
-        std::vector<int2> flat(ent[c].size());
+        for (size_t k = 0; k < extra.size(); ++k) ptr[k + 1] += ptr[k];
+        std::vector<int4> flat(ent[c].size());
         for (size_t k = 0; k < ent[c].size(); ++k) flat[k] = ent[c][k].second;
-        if ((rc = upload((void**)&p->mon_ptr[c], ptr.data(), ptr.size() * sizeof(int)))) return rc;
-        if ((rc = upload((void**)&p->mon_ent[c], flat.data(), flat.size() * sizeof(int2))))
+        if ((rc = upload((void**)&p->tile_mon[c], ptr.data(), ptr.size() * sizeof(int)))) return rc;
+        if ((rc = upload((void**)&p->tile_ent[c], flat.data(), flat.size() * sizeof(int4))))
             return rc;
     }
-    if ((rc = upload((void**)&p->tile_flags, flags.data(), flags.size()))) return rc;
+    std::vector<int4> tiles(interior);
+    tiles.insert(tiles.end(), strip.begin(), strip.end());
+    tiles.insert(tiles.end(), extra.begin(), extra.end());
+    if ((rc = upload((void**)&p->tiles_dev, tiles.data(), tiles.size() * sizeof(int4)))) return rc;
+    p->n_interior = int(interior.size());
+    p->n_strip = int(strip.size());
+    p->n_extra = int(extra.size());
     if (p->n_pairs) {
         if ((rc = upload((void**)&p->pairs, pairs.data(), pairs.size() * sizeof(int4)))) return rc;
         if ((rc = upload((void**)&p->accs, accs.data(), accs.size() * sizeof(void*)))) return rc;
@@ -266,11 +315,10 @@ FusedArgs<R> make_args(const EtlPlan* p) {
     a.n_src = p->n_src;
     a.src = p->src_dev;
     a.delta = (const R*)p->delta;
-    a.tile_flags = p->tile_flags;
     a.has_mon = p->n_slots > 0;
     for (int c = 0; c < 3; ++c) {
-        a.mon_ptr[c] = p->mon_ptr[c];
-        a.mon_ent[c] = p->mon_ent[c];
+        a.tile_mon[c] = p->tile_mon[c];
+        a.tile_ent[c] = p->tile_ent[c];
     }
     a.trace = (R*)p->trace;
     a.n_slots = p->n_slots;
@@ -285,13 +333,28 @@ template <typename R>
 int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
     const FusedArgs<R> a = make_args<R>(p);
     hipStream_t s = ctx().stream;
+    const TileList tl{p->tiles_dev,
+                      p->n_interior,
+                      p->tiles_dev + p->n_interior,
+                      p->n_strip,
+                      p->tiles_dev + p->n_interior + p->n_strip,
+                      p->n_extra};
     const int T = p->tune.temporal;
     int k = 0, n = 0;
     while (k < count) {
         const int depth = (count - k >= T) ? T : 1;
         if (events) ETL_HIP(hipEventRecord(events[2 * n], s));
-        int rc = launch_fused<R>(a, p->tune.vec, depth, p->counter, k, parity ^ (n & 1), s);
+        // fork: strip and extra tiles run on their own streams, joined before the next step
+        ETL_HIP(hipEventRecord(p->ev_fork, s));
+        ETL_HIP(hipStreamWaitEvent(p->s_strip, p->ev_fork, 0));
+        ETL_HIP(hipStreamWaitEvent(p->s_extra, p->ev_fork, 0));
+        int rc = launch_fused<R>(a, tl, p->tune.vec, depth, p->tune.band_rows, p->counter, k,
+                                 parity ^ (n & 1), s, p->s_strip, p->s_extra);
         if (rc) return rc;
+        ETL_HIP(hipEventRecord(p->ev_join, p->s_strip));
+        ETL_HIP(hipEventRecord(p->ev_join2, p->s_extra));
+        ETL_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+        ETL_HIP(hipStreamWaitEvent(s, p->ev_join2, 0));
         if (events) ETL_HIP(hipEventRecord(events[2 * n + 1], s));
         k += depth;
         ++n;
@@ -415,18 +478,15 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
     EtlTune t{};
     if (tune) t = *tune;
     const bool f32 = desc->precision == ETL_F32;
-    if (t.vec == 0) t.vec = f32 ? 4 : 2;
-    if (t.temporal == 0) t.temporal = 4;
-    if (t.band_rows == 0) t.band_rows = 32;
+    if (t.vec == 0) t.vec = 1;
+    if (t.temporal == 0) t.temporal = f32 ? 4 : 2;
+    if (t.band_rows == 0) t.band_rows = 8;
     if (t.trace_chunk == 0) t.trace_chunk = 64;
     if (tune == nullptr) t.use_graph = 1;
-    ETL_REQUIRE(t.vec == 1 || t.vec == 2 || t.vec == 4, "etl_plan_create: vec must be 1, 2 or 4");
-    ETL_REQUIRE(t.temporal == 1 || t.temporal == 2 || t.temporal == 4,
-                "etl_plan_create: temporal depth must be 1, 2 or 4");
-    ETL_REQUIRE(tile_out_cols(t.vec, t.temporal) >= 32,
-                "etl_plan_create: vec %d too narrow for temporal depth %d", t.vec, t.temporal);
-    ETL_REQUIRE(t.band_rows >= 1 && t.band_rows <= 4096, "etl_plan_create: band_rows %d",
-                t.band_rows);
+    ETL_REQUIRE(tile_shape_supported(desc->precision, t.vec, t.temporal, t.band_rows) &&
+                    tile_shape_supported(desc->precision, t.vec, 1, t.band_rows),
+                "etl_plan_create: no compiled tile shape vec=%d temporal=%d band_rows=%d",
+                t.vec, t.temporal, t.band_rows);
     ETL_REQUIRE(t.trace_chunk >= 2 && t.trace_chunk <= 4096 &&
                     t.trace_chunk % (2 * t.temporal) == 0,
                 "etl_plan_create: trace_chunk must be a multiple of 2 * temporal, <= 4096");
@@ -461,7 +521,36 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
         }
         return ETL_OK;
     };
+    // The fused kernel loads whole vectors that overhang rows by up to 64*vec
+    // elements... clamped to the row end, so by less than vec elements, and starts up
+    // to `temporal` elements before a row: every array must carry etl_malloc's guard
+    // bands and be at least its shape's size.
+    {
+        const size_t el = p->elem();
+        const size_t sz_ex = size_t(nx) * (ny + 1) * el, sz_ey = size_t(nx + 1) * ny * el,
+                     sz_hz = size_t(nx) * ny * el;
+        const size_t need[7] = {sz_ex, sz_ey, sz_hz, sz_ex, sz_ey, sz_hz, sz_hz};
+        bool ok = guarded_bytes(desc->cb_ex) >= sz_ex && guarded_bytes(desc->cb_ey) >= sz_ey;
+        for (int k = 0; k < 7; ++k)
+            for (int b = 0; b < 2; ++b) ok = ok && guarded_bytes(desc->field[k][b]) >= need[k];
+        if (!ok) {
+            delete p;
+            set_error("etl_plan_create: field and cb arrays must be etl_malloc allocations of "
+                      "their full shape (the fused kernel reads into their guard bands)");
+            return ETL_EARG;
+        }
+    }
     int rc = f32 ? bounds(float(0), p->strips) : bounds(double(0), p->strips);
+    if (rc == ETL_OK) {
+        if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_join2, hipEventDisableTiming) != hipSuccess) {
+            set_error("etl_plan_create: stream/event creation failed");
+            rc = ETL_EHIP;
+        }
+    }
     if (rc == ETL_OK) rc = etl_malloc((void**)&p->counter, 256);
     // psi must be +0 wherever the fused kernel does not touch it.
     if (rc == ETL_OK) {
@@ -501,6 +590,11 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
     }
     if (rc != ETL_OK) {
         etl_free(p->counter);
+        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+        if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+        if (p->ev_join2) (void)hipEventDestroy(p->ev_join2);
+        if (p->s_strip) (void)hipStreamDestroy(p->s_strip);
+        if (p->s_extra) (void)hipStreamDestroy(p->s_extra);
         delete p;
         return rc;
     }
@@ -518,6 +612,14 @@ int etl_plan_destroy(etl_plan_t p) {
     etl_free(p->delta);
     etl_free(p->ph);
     etl_free(p->counter);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->ev_join2) (void)hipEventDestroy(p->ev_join2);
+    for (hipStream_t side : {p->s_strip, p->s_extra})
+        if (side) {
+            ETL_HIP(hipStreamSynchronize(side));
+            (void)hipStreamDestroy(side);
+        }
     delete p;
     return ETL_OK;
 }

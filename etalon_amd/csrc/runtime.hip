@@ -3,7 +3,9 @@
 // Memory: a small caching allocator keyed by exact (rounded) size, so that the
 // per-step host uploads of the backend protocol (one tiny delta array per source
 // per step, etalon/fdtd/solver.py:87) do not call hipMalloc/hipFree each step
-// (hipFree synchronises the device).
+// (hipFree synchronises the device).  Every block carries kGuard bytes of valid
+// device memory before and after the array, so the fused kernels can issue whole
+// vector loads that overhang a row by a few elements without branching.
 #include <cstdarg>
 #include <map>
 #include <mutex>
@@ -39,8 +41,13 @@ int require_init() {
 
 namespace {
 std::mutex g_pool_mutex;
-std::multimap<size_t, void*> g_free;           // size -> block, cached for reuse
-std::unordered_map<void*, size_t> g_live;      // block -> size
+struct Block {
+    void* base;    // start of the hipMalloc block
+    size_t size;   // size class (bytes usable between the guards)
+    size_t bytes;  // bytes asked for
+};
+std::multimap<size_t, void*> g_free;           // size class -> base, cached for reuse
+std::unordered_map<void*, Block> g_live;       // user pointer (base + kGuard) -> block
 size_t g_cached_bytes = 0;
 constexpr size_t kCacheLimit = size_t(1) << 30;  // keep at most 1 GiB of freed blocks
 
@@ -65,6 +72,12 @@ __global__ void fill_zero_bytes_kernel(uint8_t* __restrict__ p, size_t n) {
     if (k < n) p[k] = 0;
 }
 }  // namespace
+
+size_t guarded_bytes(const void* p) {
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    auto it = g_live.find(const_cast<void*>(p));
+    return it == g_live.end() ? 0 : it->second.bytes;
+}
 
 int zero_async(void* dst, size_t bytes) {
     if (bytes == 0) return ETL_OK;
@@ -146,15 +159,16 @@ int etl_malloc(void** ptr, size_t bytes) {
         std::lock_guard<std::mutex> lock(g_pool_mutex);
         auto it = g_free.find(size);
         if (it != g_free.end()) {
-            *ptr = it->second;
+            void* base = it->second;
             g_free.erase(it);
             g_cached_bytes -= size;
-            g_live[*ptr] = size;
+            *ptr = static_cast<char*>(base) + kGuard;
+            g_live[*ptr] = Block{base, size, bytes};
             return ETL_OK;
         }
     }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, size);
+    hipError_t e = hipMalloc(&p, size + 2 * kGuard);
     if (e != hipSuccess) {
         // Release the cache and retry once.
         std::lock_guard<std::mutex> lock(g_pool_mutex);
@@ -162,15 +176,15 @@ int etl_malloc(void** ptr, size_t bytes) {
         for (auto& kv : g_free) (void)hipFree(kv.second);
         g_free.clear();
         g_cached_bytes = 0;
-        e = hipMalloc(&p, size);
+        e = hipMalloc(&p, size + 2 * kGuard);
         if (e != hipSuccess) {
             set_error("etl_malloc: hipMalloc(%zu) failed: %s", size, hipGetErrorString(e));
             return ETL_ENOMEM;
         }
     }
     std::lock_guard<std::mutex> lock(g_pool_mutex);
-    g_live[p] = size;
-    *ptr = p;
+    *ptr = static_cast<char*>(p) + kGuard;
+    g_live[*ptr] = Block{p, size, bytes};
     return ETL_OK;
 }
 
@@ -180,17 +194,17 @@ int etl_free(void* ptr) {
     std::lock_guard<std::mutex> lock(g_pool_mutex);
     auto it = g_live.find(ptr);
     ETL_REQUIRE(it != g_live.end(), "etl_free: %p was not allocated by etl_malloc", ptr);
-    size_t size = it->second;
+    const Block blk = it->second;
     g_live.erase(it);
-    if (g_cached_bytes + size <= kCacheLimit) {
+    if (g_cached_bytes + blk.size <= kCacheLimit) {
         // Stream-ordered reuse: every later use of this block is issued on the same
         // stream, after the work that used it before.
-        g_free.emplace(size, ptr);
-        g_cached_bytes += size;
+        g_free.emplace(blk.size, blk.base);
+        g_cached_bytes += blk.size;
         return ETL_OK;
     }
     ETL_HIP(hipStreamSynchronize(ctx().stream));
-    ETL_HIP(hipFree(ptr));
+    ETL_HIP(hipFree(blk.base));
     return ETL_OK;
 }
 

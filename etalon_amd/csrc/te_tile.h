@@ -1,0 +1,414 @@
+// Fused, temporally blocked 2D TE time stepping for gfx950: the register-tile kernel.
+//
+// One launch advances the fields T full steps.  Per step n (etalon/fdtd/solver.py:
+// 120-128): H update -> M injection (t = nΔt) -> E update -> J injection
+// (t = (n+½)Δt); the monitored values of Hz (after M) and of Ex, Ey (after J) go to
+// a trace buffer that a separate DFT kernel folds into the accumulators in step
+// order.  Every node update performs the reference's operations in the reference's
+// order (etalon/fdtd/backends/numpy_backend.py:41-61), so the results are bit-identical
+// to T single steps whatever T or the tile shape.
+//
+// Decomposition.  One wave (64 lanes) owns an output tile of BR rows (x), from the
+// row its descriptor names, by
+// WO = 64*V - 2*T columns (y).  It loads a window of RW = BR + 2T rows by 64*V
+// columns — lane l holds columns o0 - T + l*V .. + V-1 of every window row in
+// registers (Ey one row more) — in a single burst of independent vector loads,
+// performs the T steps entirely in registers, and stores its BR x WO output.  Row
+// neighbours are registers of the same lane, column neighbours come from the
+// adjacent lane (ds_bpermute).  Values beyond the window are unknown: they corrupt
+// one row and one column per half-step from each side, which the T-wide halo
+// absorbs, so the owned cells are exact.  Physical boundaries need no halo: tangential
+// E on the outer boundary (Ex at j = 0, ny; Ey at i = 0, nx) is never updated, so
+// rows and columns outside the grid never reach inside it.  Reads come from buffer A
+// and writes go to buffer B (ping-pong): tiles never see each other's writes.
+//
+// All register indices are compile-time constants (rows are unrolled), so the tile
+// lives in VGPRs and the only waits are the one after the load burst.  Thousands of
+// tiles per launch keep many loads in flight per SIMD.
+//
+// Variants.  STRIP tiles touch a CPML strip: they load, carry and store psi where
+// the 1D profiles differ from (b, c, 1/kappa) = (1, 0, 1); elsewhere psi is +0 and the
+// update reduces to d*1 + (+0), exactly the reference's arithmetic (the +0 fixes the
+// sign of a zero difference).  EXTRA tiles also inject sources and trace monitors.
+#pragma once
+
+#include "common.h"
+#include "fused.h"
+
+namespace etl {
+
+// V consecutive REALs with the alignment of one REAL: Ex rows have an odd pitch
+// (ny + 1) and gfx950 global loads/stores need only dword alignment.
+template <typename R, int V>
+using Vec = R __attribute__((ext_vector_type(V), aligned(sizeof(R))));
+
+// dst[k] = p[row * pitch + min(c, limit - 1) + k]: always one full vector load.
+// Arrays come from etl_malloc, which keeps kGuard valid bytes before and after each
+// one, and c >= -T, so the overhang past a row (or before the first) stays in
+// memory; elements outside the row are garbage the caller never uses.
+template <typename R, int V>
+__device__ __forceinline__ void ldv(R (&dst)[V], const R* __restrict__ base, int row, int pitch,
+                                    int c, int limit) {
+    const R* p = base + int64_t(row) * pitch + min(c, limit - 1);
+    if constexpr (V == 1) {
+        dst[0] = p[0];
+    } else {
+        const Vec<R, V> x = *reinterpret_cast<const Vec<R, V>*>(p);
+#pragma unroll
+        for (int k = 0; k < V; ++k) dst[k] = x[k];
+    }
+}
+
+// p[c + k] = src[k] for the elements with lo <= c + k < hi.
+template <typename R, int V>
+__device__ __forceinline__ void store_row(R* __restrict__ p, const R (&src)[V], int c, int lo,
+                                          int hi) {
+    if (c >= lo && c + V <= hi) {
+        if constexpr (V == 1) {
+            p[c] = src[0];
+        } else {
+            Vec<R, V> x;
+#pragma unroll
+            for (int k = 0; k < V; ++k) x[k] = src[k];
+            *reinterpret_cast<Vec<R, V>*>(p + c) = x;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+            if (c + k >= lo && c + k < hi) p[c + k] = src[k];
+    }
+}
+
+template <typename R, int V>
+__device__ __forceinline__ void store_masked(R* __restrict__ p, const R (&src)[V], int c,
+                                             const bool (&m)[V]) {
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+        if (m[k]) p[c + k] = src[k];
+}
+
+// Broadcast lane `l` of v to the whole wave (v_readlane; l is wave-uniform).
+template <typename R>
+__device__ __forceinline__ R lane_value(R v, int l) {
+    if constexpr (sizeof(R) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffffLL), l);
+        const int hi = __builtin_amdgcn_readlane(int(b >> 32), l);
+        return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+    }
+}
+
+// Point sources of a tile: lane s holds source s (comp, row, column) and its
+// deltas for the T steps of the launch; `rows[comp]` has bit l set when window row
+// l carries a source of that component.
+template <typename R, int T>
+struct TileSources {
+    int n = 0;
+    int comp = 0, row = 0, col = 0;
+    R delta[T];
+    unsigned long long rows[3] = {0, 0, 0};
+};
+
+// Subtract this step's deltas of the sources of component `comp` on row i, in
+// source order.  x - (+0) == x bit for bit, so the untouched elements are
+// unchanged; the select keeps v[] in registers (no dynamic index).
+template <typename R, int V, int T>
+__device__ __forceinline__ void inject_row(const TileSources<R, T>& src, int t, int comp, int i,
+                                           int c, R (&v)[V]) {
+    for (int s = 0; s < src.n; ++s) {
+        if (__builtin_amdgcn_readlane(src.comp, s) != comp ||
+            __builtin_amdgcn_readlane(src.row, s) != i)
+            continue;
+        const int js = __builtin_amdgcn_readlane(src.col, s) - c;
+        const R dl = lane_value(src.delta[t], s);
+#pragma unroll
+        for (int k = 0; k < V; ++k) v[k] = v[k] - ((k == js) ? dl : R(0));
+    }
+}
+
+// Monitored values of one component's owned rows into the trace row: the rows go
+// to LDS, then the lanes copy the tile's entries (row, column, slot).
+template <typename R, int V, int RW, int T, int BR>
+__device__ __forceinline__ void trace_tile(const int4* __restrict__ ent, int e0, int e1,
+                                           R* __restrict__ lds, const R (&f)[RW][V], int i0,
+                                           int cs, R* __restrict__ trow) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int l = 0; l < BR; ++l)
+#pragma unroll
+        for (int k = 0; k < V; ++k) lds[(l * 64 + lane) * V + k] = f[T + l][k];
+    __syncthreads();
+    for (int q = e0 + lane; q < e1; q += 64) {
+        const int4 en = ent[q];  // row, column, slot
+        trow[en.z] = lds[(en.x - i0) * 64 * V + (en.y - cs)];
+    }
+    __syncthreads();
+}
+
+template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
+__global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
+                                                  const int4* __restrict__ tiles, int n_tiles,
+                                                  const int* __restrict__ counter, int kstep0,
+                                                  int parity) {
+    constexpr int W = 64 * V;
+    constexpr int RW = BR + 2 * T;  // window rows (Ey: one more)
+    const int b = blockIdx.x;
+    const int per_xcd = (n_tiles + 7) >> 3;
+    const int li = (b & 7) * per_xcd + (b >> 3);
+    if (li >= n_tiles) return;
+    const int4 td = tiles[li];  // first row, tile column, monitor cell, 0
+    const int tj = td.y;
+    const int lane = threadIdx.x;
+    const int nx = a.nx, ny = a.ny, px = ny + 1;
+    const int o0 = tj * a.wo;  // first output column
+    const int cs = o0 - T;     // first window column
+    const int c = cs + lane * V;
+    const int i0 = td.x;
+    const int g0 = i0 - T;  // global row of window row 0
+
+    const R* __restrict__ exA = a.f[0][parity];
+    const R* __restrict__ eyA = a.f[1][parity];
+    const R* __restrict__ hzA = a.f[2][parity];
+    const R* __restrict__ pexA = a.f[3][parity];  // psi_ex_y
+    const R* __restrict__ peyA = a.f[4][parity];  // psi_ey_x
+    const R* __restrict__ phxA = a.f[5][parity];  // psi_hz_x
+    const R* __restrict__ phyA = a.f[6][parity];  // psi_hz_y
+    R* __restrict__ exB = const_cast<R*>(a.f[0][parity ^ 1]);
+    R* __restrict__ eyB = const_cast<R*>(a.f[1][parity ^ 1]);
+    R* __restrict__ hzB = const_cast<R*>(a.f[2][parity ^ 1]);
+    R* __restrict__ pexB = const_cast<R*>(a.f[3][parity ^ 1]);
+    R* __restrict__ peyB = const_cast<R*>(a.f[4][parity ^ 1]);
+    R* __restrict__ phxB = const_cast<R*>(a.f[5][parity ^ 1]);
+    R* __restrict__ phyB = const_cast<R*>(a.f[6][parity ^ 1]);
+
+    // ---- one burst of loads: the whole window ----
+    constexpr int PR = STRIP ? RW : 1;  // psi rows held (none in interior tiles)
+    R ex[RW][V], hz[RW][V], ey[RW + 1][V], cx[RW][V], cy[RW][V];
+    R phy[PR][V], phx[PR][V], pey[PR][V], pex[PR][V];
+#pragma unroll
+    for (int l = 0; l < RW; ++l) {
+        const int g = g0 + l;
+        const int gx = min(max(g, 0), nx - 1);  // rows of Ex, Hz, cb_ex, psi_hz, psi_ex
+        const int ge = min(max(g, 0), nx);      // rows of Ey, cb_ey, psi_ey
+        ldv<R, V>(ex[l], exA, gx, px, c, px);
+        ldv<R, V>(hz[l], hzA, gx, ny, c, ny);
+        ldv<R, V>(ey[l], eyA, ge, ny, c, ny);
+        ldv<R, V>(cx[l], a.cb_ex, gx, px, c, px);
+        ldv<R, V>(cy[l], a.cb_ey, ge, ny, c, ny);
+        if constexpr (STRIP) {
+            ldv<R, V>(phy[l], phyA, gx, ny, c, ny);
+            ldv<R, V>(phx[l], phxA, gx, ny, c, ny);
+            ldv<R, V>(pey[l], pexA, gx, px, c, px);
+            ldv<R, V>(pex[l], peyA, ge, ny, c, ny);
+        }
+    }
+    ldv<R, V>(ey[RW], eyA, min(max(g0 + RW, 0), nx), ny, c, ny);
+
+    const R db = a.db, idx = a.inv_dx, idy = a.inv_dy;
+    const R zero = R(0);
+
+    // y-strip membership and profiles of this lane's columns
+    bool mH[V], ysh[V], yse[V];
+    R byh[V], cyh[V], ikyh[V], bye[V], cye[V], ikye[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int j = c + k;
+        mH[k] = (j >= 0 && j < ny);
+        ysh[k] = STRIP && mH[k] && (j < a.yh_lo || j >= a.yh_hi);
+        yse[k] = STRIP && j >= 1 && j <= ny - 1 && (j < a.ye_lo || j >= a.ye_hi);
+        byh[k] = ysh[k] ? a.by_h[j] : R(1);
+        cyh[k] = ysh[k] ? a.cy_h[j] : R(0);
+        ikyh[k] = ysh[k] ? a.iky_h[j] : R(1);
+        bye[k] = yse[k] ? a.by_e[j] : R(1);
+        cye[k] = yse[k] ? a.cy_e[j] : R(0);
+        ikye[k] = yse[k] ? a.iky_e[j] : R(1);
+    }
+
+    // Row profiles of the x strips: lane l holds those of window row l (RW <= 64),
+    // read back with v_readlane where a row needs them.
+    R pbxh = R(1), pcxh = R(0), pikxh = R(1), pbxe = R(1), pcxe = R(0), pikxe = R(1);
+    if constexpr (STRIP) {
+        const int gl = g0 + min(lane, RW - 1);
+        const int gh = min(max(gl, 0), nx - 1), ge = min(max(gl, 0), nx);
+        pbxh = a.bx_h[gh];
+        pcxh = a.cx_h[gh];
+        pikxh = a.ikx_h[gh];
+        pbxe = a.bx_e[ge];
+        pcxe = a.cx_e[ge];
+        pikxe = a.ikx_e[ge];
+    }
+
+    // Sources and monitor entries of an extra tile.
+    TileSources<R, T> src;
+    int mon_lo[3] = {0, 0, 0}, mon_hi[3] = {0, 0, 0};
+    if constexpr (EXTRA) {
+        const int nbase = *counter + kstep0;  // table row of this launch's first step
+        src.n = a.n_src;
+        if (lane < a.n_src) {
+            const int4 sd = a.src[lane];
+            src.comp = sd.x;
+            src.row = sd.y;
+            src.col = sd.z;
+#pragma unroll
+            for (int t = 0; t < T; ++t) src.delta[t] = a.delta[int64_t(nbase + t) * a.n_src + lane];
+        }
+        for (int s = 0; s < src.n; ++s) {
+            const int l = __builtin_amdgcn_readlane(src.row, s) - g0;
+            if (l >= 0 && l < RW) src.rows[__builtin_amdgcn_readlane(src.comp, s)] |= 1ull << l;
+        }
+        if (a.has_mon) {
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) {
+                if (a.tile_mon[comp] == nullptr) continue;
+                mon_lo[comp] = a.tile_mon[comp][td.z];
+                mon_hi[comp] = a.tile_mon[comp][td.z + 1];
+            }
+        }
+    }
+    __shared__ R lds[EXTRA ? BR * W : 1];
+
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        // ---- H half-step, window rows [t, RW - t) ----
+#pragma unroll
+        for (int l = t; l < RW - t; ++l) {
+            const int g = g0 + l;
+            bool xh = false;
+            R bx = R(1), cxp = R(0), ikx = R(1);
+            if constexpr (STRIP) {
+                xh = g < a.xh_lo || g >= a.xh_hi;
+                if (xh) {
+                    bx = lane_value(pbxh, l);
+                    cxp = lane_value(pcxh, l);
+                    ikx = lane_value(pikxh, l);
+                }
+            }
+            const R exr = __shfl_down(ex[l][0], 1, 64);  // lane 63: outside the window
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const R exn = (k + 1 < V) ? ex[l][k + 1] : exr;
+                const R d = (exn - ex[l][k]) * idy;
+                const R e = (ey[l + 1][k] - ey[l][k]) * idx;
+                R ty, tx;
+                if (STRIP && ysh[k]) {
+                    const R py = byh[k] * phy[STRIP ? l : 0][k] + cyh[k] * d;
+                    phy[STRIP ? l : 0][k] = py;
+                    ty = d * ikyh[k] + py;
+                } else {
+                    ty = d + zero;
+                }
+                if (STRIP && xh) {
+                    const R pxv = bx * phx[STRIP ? l : 0][k] + cxp * e;
+                    phx[STRIP ? l : 0][k] = pxv;
+                    tx = e * ikx + pxv;
+                } else {
+                    tx = e + zero;
+                }
+                hz[l][k] = hz[l][k] + db * (ty - tx);
+            }
+            if constexpr (EXTRA) {
+                if ((src.rows[2] >> l) & 1) inject_row<R, V, T>(src, t, 2, g, c, hz[l]);
+            }
+        }
+        // ---- E half-step: Ex rows [t, RW - t), Ey rows [t + 1, RW - t) ----
+#pragma unroll
+        for (int l = t; l < RW - t; ++l) {
+            const int g = g0 + l;
+            const R hl = __shfl_up(hz[l][V - 1], 1, 64);  // lane 0: outside the window
+            const bool ex_row = (g >= 0 && g <= nx - 1);
+            const bool ey_row = (l >= t + 1) && (g >= 1 && g <= nx - 1);
+            bool xe = false;
+            R bxe = R(1), cxe = R(0), ikxe = R(1);
+            if constexpr (STRIP) {
+                xe = ey_row && (g < a.xe_lo || g >= a.xe_hi);
+                if (xe) {
+                    bxe = lane_value(pbxe, l);
+                    cxe = lane_value(pcxe, l);
+                    ikxe = lane_value(pikxe, l);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const int j = c + k;
+                if (ex_row && j >= 1 && j <= ny - 1) {  // Ex(g, j); j = 0, ny stay unchanged
+                    const R hleft = (k == 0) ? hl : hz[l][k - 1];
+                    const R d = (hz[l][k] - hleft) * idy;
+                    R tt;
+                    if (STRIP && yse[k]) {
+                        const R p = bye[k] * pey[STRIP ? l : 0][k] + cye[k] * d;
+                        pey[STRIP ? l : 0][k] = p;
+                        tt = d * ikye[k] + p;
+                    } else {
+                        tt = d + zero;
+                    }
+                    ex[l][k] = ex[l][k] + cx[l][k] * tt;
+                }
+                if (ey_row) {  // Ey(g, j), g in [1, nx - 1]; rows 0 and nx stay unchanged
+                    const R d = (hz[l][k] - hz[l > 0 ? l - 1 : 0][k]) * idx;
+                    R tt;
+                    if (STRIP && xe) {
+                        const R p = bxe * pex[STRIP ? l : 0][k] + cxe * d;
+                        pex[STRIP ? l : 0][k] = p;
+                        tt = d * ikxe + p;
+                    } else {
+                        tt = d + zero;
+                    }
+                    ey[l][k] = ey[l][k] - cy[l][k] * tt;
+                }
+            }
+            if constexpr (EXTRA) {
+                if ((src.rows[0] >> l) & 1) inject_row<R, V, T>(src, t, 0, g, c, ex[l]);
+                if ((src.rows[1] >> l) & 1) inject_row<R, V, T>(src, t, 1, g, c, ey[l]);
+            }
+        }
+        if constexpr (EXTRA) {
+            R* trow = a.trace + int64_t(kstep0 + t) * a.n_slots;
+            if (mon_lo[2] < mon_hi[2])
+                trace_tile<R, V, RW, T, BR>(a.tile_ent[2], mon_lo[2], mon_hi[2], lds, hz, i0, cs,
+                                            trow);
+            if (mon_lo[0] < mon_hi[0])
+                trace_tile<R, V, RW, T, BR>(a.tile_ent[0], mon_lo[0], mon_hi[0], lds, ex, i0, cs,
+                                            trow);
+            if (mon_lo[1] < mon_hi[1])
+                trace_tile<R, V, RW + 1, T, BR>(a.tile_ent[1], mon_lo[1], mon_hi[1], lds, ey, i0,
+                                                cs, trow);
+        }
+    }
+
+    // ---- store the owned rows [i0, i0 + BR) and columns [o0, o0 + WO) ----
+    const int lo = max(o0, 0), hi = o0 + a.wo;
+#pragma unroll
+    for (int l = T; l < T + BR; ++l) {
+        const int g = g0 + l;
+        if (g >= nx) break;
+        store_row<R, V>(hzB + int64_t(g) * ny, hz[l], c, lo, min(hi, ny));
+        store_row<R, V>(exB + int64_t(g) * px, ex[l], c, lo, min(hi, px));
+        store_row<R, V>(eyB + int64_t(g) * ny, ey[l], c, lo, min(hi, ny));
+        if constexpr (STRIP) {
+            bool mo[V], moy[V], moe[V];
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const bool own = (c + k >= lo && c + k < hi);
+                mo[k] = own && mH[k];
+                moy[k] = own && ysh[k];
+                moe[k] = own && yse[k];
+            }
+            store_masked<R, V>(phyB + int64_t(g) * ny, phy[STRIP ? l : 0], c, moy);
+            store_masked<R, V>(pexB + int64_t(g) * px, pey[STRIP ? l : 0], c, moe);
+            if (g < a.xh_lo || g >= a.xh_hi)
+                store_masked<R, V>(phxB + int64_t(g) * ny, phx[STRIP ? l : 0], c, mo);
+            if (g >= 1 && (g < a.xe_lo || g >= a.xe_hi))
+                store_masked<R, V>(peyB + int64_t(g) * ny, pex[STRIP ? l : 0], c, mo);
+        }
+    }
+    // the last band copies Ey row nx (outer tangential E)
+    if (i0 + BR >= nx) {
+        R eyl[V];
+        ldv<R, V>(eyl, eyA, nx, ny, c, ny);
+        store_row<R, V>(eyB + int64_t(nx) * ny, eyl, c, lo, min(hi, ny));
+    }
+}
+
+}  // namespace etl

@@ -1,0 +1,67 @@
+// Monitor DFT from the trace buffer, and the step counter of fused plans.
+//
+// The fused kernels write each monitored node's value of every step into a trace
+// row; this kernel folds a chunk of rows into the complex accumulators, one thread
+// per (node, frequency), in step order: acc = acc + v*ph, re and im separately — the
+// operations of NumpyBackend.accumulate_dft (etalon/fdtd/backends/numpy_backend.py:
+// 63-67) applied step after step, so the accumulators are bit-identical.
+#include "common.h"
+#include "fused.h"
+
+namespace etl {
+namespace {
+
+// acc[pair] += sum over the chunk's rows, in step order, of trace * phasor.
+template <typename R>
+__global__ __launch_bounds__(256) void trace_dft_kernel(const R* __restrict__ trace, int n_slots,
+                                                        int rows, const R* __restrict__ ph, int nq,
+                                                        const int* __restrict__ counter,
+                                                        const int4* __restrict__ pairs,
+                                                        int n_pairs, R* const* __restrict__ accs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    const int4 pr = pairs[p];  // slot, q, monitor, complex element index
+    R* acc = accs[pr.z] + 2 * int64_t(pr.w);
+    R re = acc[0], im = acc[1];
+    const int base = *counter;
+    const R* __restrict__ php = ph + 2 * (int64_t(base) * nq + pr.y);
+    for (int r = 0; r < rows; ++r) {
+        const R v = trace[int64_t(r) * n_slots + pr.x];
+        const R pre = php[2 * int64_t(r) * nq];
+        const R pim = php[2 * int64_t(r) * nq + 1];
+        re = re + v * pre;
+        im = im + v * pim;
+    }
+    acc[0] = re;
+    acc[1] = im;
+}
+
+__global__ void bump_counter_kernel(int* counter, int by) {
+    if (threadIdx.x == 0) counter[0] = counter[0] + by;
+}
+
+}  // namespace
+
+template <typename R>
+int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq,
+                     const int* counter, const int4* pairs, int n_pairs, R* const* accs,
+                     hipStream_t stream) {
+    if (n_pairs == 0 || rows == 0) return ETL_OK;
+    hipLaunchKernelGGL(trace_dft_kernel<R>, dim3((n_pairs + 255) / 256), dim3(256), 0, stream,
+                       trace, n_slots, rows, ph, nq, counter, pairs, n_pairs, accs);
+    ETL_LAUNCHED();
+    return ETL_OK;
+}
+
+int launch_bump(int* counter, int by, hipStream_t stream) {
+    hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(64), 0, stream, counter, by);
+    ETL_LAUNCHED();
+    return ETL_OK;
+}
+
+template int launch_trace_dft<float>(const float*, int, int, const float*, int, const int*,
+                                     const int4*, int, float* const*, hipStream_t);
+template int launch_trace_dft<double>(const double*, int, int, const double*, int, const int*,
+                                      const int4*, int, double* const*, hipStream_t);
+
+}  // namespace etl

@@ -23,8 +23,8 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h
 
 @dataclass
 class Tune:
-    """Fused-kernel shape; 0 = the library default (float32: vec 4, float64: vec 2;
-    temporal depth 4, 32-row bands, 64-step trace chunks)."""
+    """Fused-kernel shape; 0 = the library default (vec 1, 8-row tiles, temporal
+    depth 4 in float32 and 2 in float64, 64-step trace chunks)."""
 
     band_rows: int = 0
     vec: int = 0

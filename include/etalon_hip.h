@@ -103,6 +103,9 @@ int etl_gather(int precision, void* out, const void* field, int64_t pitch, const
 
 typedef struct EtlPlan* etl_plan_t;
 
+/* Every field and cb array of a plan must come from etl_malloc: the fused kernel
+ * issues whole vector loads that may start a few elements before a row and end a
+ * few past it, which etl_malloc's guard bands make safe. */
 typedef struct EtlTeDesc {
     int precision;
     int nx, ny;
@@ -118,13 +121,16 @@ typedef struct EtlTeDesc {
     const void *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
 } EtlTeDesc;
 
-/* Tuning knobs of the fused kernel (0 = library default / autotuned). */
+/* Shape of the fused kernel (0 = library default).  (vec, temporal, band_rows)
+ * must be a compiled register-tile shape: float32 vec 1 with band 8 or 16, or vec 2
+ * with band 8, at temporal depth 1, 2 or 4; float64 vec 1 with band 8 (depth 1, 2,
+ * 4) or band 16 (depth 1, 2). */
 typedef struct EtlTune {
-    int band_rows;     /* rows marched by one wave */
-    int vec;           /* REAL elements per lane along y (1, 2 or 4) */
+    int band_rows;     /* output rows of one wave's tile */
+    int vec;           /* REAL elements per lane along y */
     int use_graph;     /* replay steps from a hipGraph (1) or launch directly (0) */
-    int trace_chunk;   /* steps per DFT flush */
-    int temporal;      /* steps fused per launch (1 = none) */
+    int trace_chunk;   /* steps per DFT flush: a multiple of 2 * temporal */
+    int temporal;      /* steps fused per launch: 1 (none), 2 or 4 */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
