@@ -13,10 +13,12 @@ the guide at x = +45 µm (4 µm long, Hz, 21 wavelengths 1.50–1.60 µm), coura
 (fused kernel + monitor DFT), bracketed by device synchronisation.
 
 One JSON line on stdout: value = cell-updates/s over the whole grid (PML
-included) ÷ 1e9, a roofline object for the fused step kernel (32 B per cell-update
-in float32 — read and write Ex, Ey, Hz, read cb_ex, cb_ey — over its average
-duration from HIP events on the library stream, against 8 TB/s), and the NumPy
-oracle on the host (one core) on a bounded sample of the same grid.
+included) ÷ 1e9; a roofline object for the dominant kernel, the interior-tile
+kernel (algorithmic bytes: 32 B per cell-update in float32 — read and write Ex,
+Ey, Hz, read cb_ex, cb_ey — times the cells it advances times the steps fused per
+launch, over its average duration from HIP events on the stream it runs on,
+against 8 TB/s; "effective", since temporal blocking moves fewer bytes than that);
+and the NumPy oracle on the host (one core) on a bounded sample of the same grid.
 """
 
 from __future__ import annotations
@@ -35,6 +37,15 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"single": 32, "double": 64}
+# Reference-published throughput on this grid (2024^2 cells incl. CPML), RTX 4060 Ti,
+# CuPy kernels: README.md:57, bench/results/b2ff355.json:76-87, docs/plan-m0.md:232.
+REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
+# Bytes the interior-tile kernel moves per launch on this grid, from rocprofv3 PMC
+# runs of this script (profiles/tile_kernel_2024_pmc_traffic.md): FETCH_SIZE and
+# WRITE_SIZE in KiB per dispatch, keyed by (precision, vec, temporal, band_rows), and
+# the factors that a 256 MiB copy gives them (FETCH_SIZE counts half the bytes read).
+PMC_KIB = {("single", 1, 4, 8): (40183.4, 44627.4), ("double", 1, 4, 8): (86718.7, 90052.0)}
+FETCH_SCALE, WRITE_SCALE = 1.9998, 1.0
 
 
 def throughput_sim(precision: str):
@@ -105,30 +116,30 @@ def run(precision: str, steps: int, warmup: int) -> dict:
     solver.backend.synchronize()
     wall = time.perf_counter() - t0
     cells = nx * ny
-    # Kernel-level timing: HIP events around every fused launch, same plan and stream.
+    # Kernel-level timing with HIP events: the same plan, launched directly.
     plan = solver._plan
-    depth = plan.resolved_tune().temporal
     n_time = 512  # a multiple of the trace chunk: every launch has the full depth
     delta, phasors = solver._tables(solver.n, n_time)
     if delta is not None:
         plan.source_table(solver.n, delta)
     for mon_id, table in zip(solver._plan_ids, phasors):
         plan.phasor_table(mon_id, solver.n, table)
-    kernel_ms, total_ms = plan.time(solver.n, n_time)
+    tm = plan.time(solver.n, n_time)
     plan.bind()
     solver.n += n_time
-    bytes_per_launch = BYTES_PER_CELL[precision] * cells * depth
-    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    depth = tm["depth"]
+    per_cell = BYTES_PER_CELL[precision]
+    interior_bytes = per_cell * tm["interior_cells"] * depth  # per launch
     return {
         "cells": (nx, ny),
         "wall": wall,
         "value": cells * steps / wall / 1e9,
         "ms_per_step": wall / steps * 1e3,
-        "kernel_ms": kernel_ms,
-        "depth": depth,
+        "timing": tm,
+        "interior_bytes": interior_bytes,
+        "achieved_gbs": interior_bytes / (tm["interior_ms"] * 1e-3) / 1e9,
+        "launch_gbs": per_cell * cells * depth / (tm["launch_ms"] * 1e-3) / 1e9,
         "tune": plan.resolved_tune().__dict__,
-        "events_total_ms_per_step": total_ms / n_time,
-        "achieved_gbs": achieved,
         "device": solver.backend.device,
     }
 
@@ -140,7 +151,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", choices=("single", "double"), default="single")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the FDTD "
@@ -150,16 +161,28 @@ def main() -> int:
 
     main_run = run(args.precision, args.steps, args.warmup)
     nx, ny = main_run["cells"]
+    tune = main_run["tune"]
+    pmc = PMC_KIB.get((args.precision, tune["vec"], tune["temporal"], tune["band_rows"]))
+    traffic = None
+    if pmc is not None and (nx, ny) == (2024, 2024):
+        traffic = round((pmc[0] * FETCH_SCALE + pmc[1] * WRITE_SCALE) * 1024)
+    interior_s = main_run["timing"]["interior_ms"] * 1e-3
     roof = {
         "bound": "hbm",
         "achieved": round(main_run["achieved_gbs"], 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "note": "effective: algorithmic bytes of T unfused steps per launch; temporal blocking "
+                "moves `traffic` bytes (measured) instead",
     }
+    if traffic is not None:
+        roof["traffic_gbs"] = round(traffic / interior_s / 1e9, 1)
+        roof["traffic_frac"] = round(traffic / interior_s / 1e9 / HBM_PEAK_GBS, 4)
+    tm = main_run["timing"]
     line = {
-        "metric": "Gcell-updates/s (2D TE FDTD, fused step, 2024^2 cells incl. 12-layer CPML)",
+        "metric": "Gcell-updates/s & %HBM roofline",
         "value": round(main_run["value"], 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
@@ -168,7 +191,7 @@ def main() -> int:
         "ms_per_step": round(main_run["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": round(main_run["value"] / REFERENCE_GCELLS[args.precision], 2),
         "dtype": "f32" if args.precision == "single" else "f64",
         "data": "synthetic: strip waveguide geometry, zero initial fields, seeded-free",
         "config": {
@@ -180,14 +203,19 @@ def main() -> int:
         },
         "roofline": roof,
         "kernel": {
-            "name": "fused_step_kernel",
-            "avg_ms_per_launch": round(main_run["kernel_ms"], 5),
-            "steps_per_launch": main_run["depth"],
-            "tune": main_run["tune"],
-            "events_ms_per_step": round(main_run["events_total_ms_per_step"], 5),
+            "name": "tile_kernel (interior tiles)",
+            "avg_ms_per_launch": round(tm["interior_ms"], 5),
+            "steps_per_launch": tm["depth"],
+            "cells_per_step": tm["interior_cells"],
+            "algorithmic_bytes_per_launch": main_run["interior_bytes"],
             "bytes_per_cell_update": BYTES_PER_CELL[args.precision],
-            "cell_updates_per_launch": nx * ny * main_run["depth"],
+            "tune": main_run["tune"],
+            "fused_launch_ms": round(tm["launch_ms"], 5),
+            "fused_launch_effective_gbs": round(main_run["launch_gbs"], 1),
+            "events_ms_per_step": round(tm["total_ms"] / (tm["launches"] * tm["depth"]), 5),
         },
+        "baseline_note": "vs_baseline divides by the reference's published CuPy throughput on "
+                         "this grid (RTX 4060 Ti: 2.94 Gcell/s f32, 1.5 f64)",
         "device": main_run["device"],
     }
     if not args.no_cpu_baseline:

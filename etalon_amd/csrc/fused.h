@@ -83,8 +83,8 @@ int launch_fused(const FusedArgs<R>& a, const TileList& tl, int vec, int depth, 
                  const int* counter, int kstep0, int parity, hipStream_t s_int,
                  hipStream_t s_strip, hipStream_t s_extra);
 template <typename R>
-int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq,
-                     const int* counter, const int4* pairs, int n_pairs, R* const* accs,
+int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq, int* counter,
+                     const int4* pairs, int n_pairs, R* const* accs,
                      hipStream_t stream);
 int launch_bump(int* counter, int by, hipStream_t stream);
 

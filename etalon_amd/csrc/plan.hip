@@ -17,7 +17,10 @@ namespace etl {
 namespace {
 
 __global__ void set_counter_kernel(int* counter, int value) {
-    if (threadIdx.x == 0) counter[0] = value;
+    if (threadIdx.x == 0) {
+        counter[0] = value;
+        counter[1] = 0;  // block tally of the DFT kernel
+    }
 }
 
 // flag[0] = 1 if psi is non-zero anywhere the fused kernel assumes it stays +0:
@@ -76,6 +79,11 @@ struct EtlPlan {
     int64_t delta_n0 = 0;
     int delta_rows = 0;
     size_t delta_bytes = 0;
+    // Tables are staged on the host and uploaded in one contiguous copy each by the
+    // next advance (stale = staged but not yet on the device).
+    std::vector<unsigned char> delta_host, ph_host;  // ph_host: (ph_host_rows, nq) complex
+    int ph_host_rows = 0;
+    bool delta_stale = false, ph_stale = false;
 
     // monitors
     struct Mon {
@@ -328,7 +336,8 @@ FusedArgs<R> make_args(const EtlPlan* p) {
 // Enqueue `count` (<= trace_chunk) steps starting at buffer parity `parity`: fused
 // launches of depth T, then depth-1 launches for the remainder, the DFT of the trace
 // rows and the counter bump.  Returns the number of fused launches in *launches.
-// `events` (optional) receives a start/stop pair around every fused launch.
+// `events` (optional) receives four events per fused launch: fork, interior kernel
+// start and end (on the main stream, where only that kernel runs), join.
 template <typename R>
 int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
     const FusedArgs<R> a = make_args<R>(p);
@@ -343,35 +352,79 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
     int k = 0, n = 0;
     while (k < count) {
         const int depth = (count - k >= T) ? T : 1;
-        if (events) ETL_HIP(hipEventRecord(events[2 * n], s));
+        if (events) ETL_HIP(hipEventRecord(events[4 * n], s));
         // fork: strip and extra tiles run on their own streams, joined before the next step
         ETL_HIP(hipEventRecord(p->ev_fork, s));
         ETL_HIP(hipStreamWaitEvent(p->s_strip, p->ev_fork, 0));
         ETL_HIP(hipStreamWaitEvent(p->s_extra, p->ev_fork, 0));
+        if (events) ETL_HIP(hipEventRecord(events[4 * n + 1], s));
         int rc = launch_fused<R>(a, tl, p->tune.vec, depth, p->tune.band_rows, p->counter, k,
                                  parity ^ (n & 1), s, p->s_strip, p->s_extra);
         if (rc) return rc;
+        if (events) ETL_HIP(hipEventRecord(events[4 * n + 2], s));
         ETL_HIP(hipEventRecord(p->ev_join, p->s_strip));
         ETL_HIP(hipEventRecord(p->ev_join2, p->s_extra));
         ETL_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
         ETL_HIP(hipStreamWaitEvent(s, p->ev_join2, 0));
-        if (events) ETL_HIP(hipEventRecord(events[2 * n + 1], s));
+        if (events) ETL_HIP(hipEventRecord(events[4 * n + 3], s));
         k += depth;
         ++n;
     }
     *launches = n;
-    if (p->n_pairs) {
-        int rc = launch_trace_dft<R>((const R*)p->trace, p->n_slots, count, (const R*)p->ph,
-                                     p->nq, p->counter, p->pairs, p->n_pairs, (R* const*)p->accs,
-                                     s);
-        if (rc) return rc;
-    }
+    // the DFT kernel advances the step counter itself
+    if (p->n_pairs && count)
+        return launch_trace_dft<R>((const R*)p->trace, p->n_slots, count, (const R*)p->ph, p->nq,
+                                   p->counter, p->pairs, p->n_pairs, (R* const*)p->accs, s);
     return launch_bump(p->counter, count, s);
 }
 
 int enqueue(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
     return p->precision == ETL_F32 ? enqueue_block<float>(p, count, parity, events, launches)
                                    : enqueue_block<double>(p, count, parity, events, launches);
+}
+
+// Rows allocated for a table of `rows` rows: a power of two, at least 1024.
+int table_capacity(int rows) {
+    int cap = 1024;
+    while (cap < rows && cap < (1 << 30)) cap <<= 1;
+    return std::max(cap, rows);
+}
+
+// Upload the staged tables (after the steps in flight, which may read the old ones).
+// A larger device table gets a new address, which invalidates the captured graphs.
+int upload_tables(EtlPlan* p) {
+    if (!p->delta_stale && !p->ph_stale) return ETL_OK;
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    int rc;
+    if (p->delta_stale) {
+        const size_t bytes = p->delta_host.size();
+        if (bytes > p->delta_bytes) {
+            const size_t cap = size_t(table_capacity(p->delta_rows)) * p->n_src * p->elem();
+            etl_free(p->delta);
+            p->delta = nullptr;
+            if ((rc = etl_malloc(&p->delta, cap))) return rc;
+            p->delta_bytes = cap;
+            free_graphs(p);
+        }
+        if (bytes && (rc = etl_memcpy_h2d(p->delta, p->delta_host.data(), bytes))) return rc;
+        p->delta_stale = false;
+    }
+    if (p->ph_stale) {
+        const size_t row_bytes = size_t(p->nq) * 2 * p->elem();
+        if (p->ph_host_rows > p->ph_rows_cap) {
+            const int cap = table_capacity(p->ph_host_rows);
+            etl_free(p->ph);
+            p->ph = nullptr;
+            if ((rc = etl_malloc(&p->ph, size_t(cap) * row_bytes))) return rc;
+            p->ph_rows_cap = cap;
+            free_graphs(p);
+        }
+        if (!p->ph_host.empty() &&
+            (rc = etl_memcpy_h2d(p->ph, p->ph_host.data(), p->ph_host.size())))
+            return rc;
+        p->ph_stale = false;
+    }
+    return ETL_OK;
 }
 
 int ensure_graph(EtlPlan* p, int parity) {
@@ -422,11 +475,12 @@ int check_tables(EtlPlan* p, int64_t n, int64_t count) {
 }
 
 int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_launch) {
-    int rc = check_tables(p, n, count);
-    if (rc) return rc;
+    int rc;
     if (p->dirty) {
         if ((rc = build_layout(p))) return rc;
     }
+    if ((rc = upload_tables(p))) return rc;
+    if ((rc = check_tables(p, n, count))) return rc;
     const int64_t n0 = p->n_src > 0 ? p->delta_n0 : (p->mons.empty() ? n : p->mons[0].n0);
     const int64_t start = n - n0;
     ETL_REQUIRE(start >= 0 && start + count < (int64_t(1) << 31), "etl_plan_advance: step range");
@@ -447,7 +501,7 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_l
     while (done < count) {
         const int blk = int(std::min<int64_t>(K, count - done));
         int launches = 0;
-        if ((rc = enqueue(p, blk, p->parity, events ? events + 2 * total : nullptr, &launches)))
+        if ((rc = enqueue(p, blk, p->parity, events ? events + 4 * total : nullptr, &launches)))
             return rc;
         p->parity ^= (launches & 1);
         done += blk;
@@ -479,7 +533,7 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
     if (tune) t = *tune;
     const bool f32 = desc->precision == ETL_F32;
     if (t.vec == 0) t.vec = 1;
-    if (t.temporal == 0) t.temporal = f32 ? 4 : 2;
+    if (t.temporal == 0) t.temporal = 4;
     if (t.band_rows == 0) t.band_rows = 8;
     if (t.trace_chunk == 0) t.trace_chunk = 64;
     if (tune == nullptr) t.use_graph = 1;
@@ -666,19 +720,11 @@ int etl_plan_source_table(etl_plan_t p, int64_t n0, int rows, const void* delta)
     if (p->n_src == 0) return ETL_OK;
     ETL_REQUIRE(delta != nullptr, "etl_plan_source_table: null table");
     const size_t bytes = size_t(rows) * p->n_src * p->elem();
-    ETL_HIP(hipStreamSynchronize(ctx().stream));  // the previous table may be in use
-    if (bytes > p->delta_bytes) {
-        etl_free(p->delta);
-        p->delta = nullptr;
-        int rc = etl_malloc(&p->delta, bytes);
-        if (rc) return rc;
-        p->delta_bytes = bytes;
-        free_graphs(p);
-    }
-    int rc = etl_memcpy_h2d(p->delta, delta, bytes);
-    if (rc) return rc;
+    const unsigned char* b = static_cast<const unsigned char*>(delta);
+    p->delta_host.assign(b, b + bytes);
     p->delta_n0 = n0;
     p->delta_rows = rows;
+    p->delta_stale = true;
     return ETL_OK;
 }
 
@@ -713,6 +759,9 @@ int etl_plan_add_monitor(etl_plan_t p, int comp, int n_nodes, const int64_t* fla
     etl_free(p->ph);
     p->ph = nullptr;
     p->ph_rows_cap = 0;
+    p->ph_host.clear();
+    p->ph_host_rows = 0;
+    p->ph_stale = false;
     for (auto& mm : p->mons) mm.rows = -1;
     free_graphs(p);
     return ETL_OK;
@@ -728,23 +777,19 @@ int etl_plan_phasor_table(etl_plan_t p, int id, int64_t n0, int rows, const void
         if ((rc = build_layout(p))) return rc;
     }
     const size_t cplx = size_t(2) * p->elem();
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
-    if (rows > p->ph_rows_cap) {
-        etl_free(p->ph);
-        p->ph = nullptr;
-        if ((rc = etl_malloc(&p->ph, size_t(rows) * p->nq * cplx))) return rc;
-        p->ph_rows_cap = rows;
-        for (auto& mm : p->mons) mm.rows = -1;
-        free_graphs(p);
+    const size_t row_bytes = size_t(p->nq) * cplx;
+    if (rows > p->ph_host_rows) {  // rows are appended: the other monitors' columns stay
+        p->ph_host.resize(size_t(rows) * row_bytes);
+        p->ph_host_rows = rows;
     }
     auto& m = p->mons[id];
-    if (rows > 0)
-        ETL_HIP(hipMemcpy2DAsync((char*)p->ph + size_t(m.q_base) * cplx, size_t(p->nq) * cplx, ph,
-                                 size_t(m.n_f) * cplx, size_t(m.n_f) * cplx, size_t(rows),
-                                 hipMemcpyHostToDevice, ctx().stream));
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    const unsigned char* src = static_cast<const unsigned char*>(ph);
+    for (int r = 0; r < rows; ++r)
+        std::memcpy(p->ph_host.data() + size_t(r) * row_bytes + size_t(m.q_base) * cplx,
+                    src + size_t(r) * m.n_f * cplx, size_t(m.n_f) * cplx);
     m.n0 = n0;
     m.rows = rows;
+    p->ph_stale = true;
     return ETL_OK;
 }
 
@@ -766,13 +811,12 @@ int etl_plan_advance(etl_plan_t p, int64_t n, int64_t count, int* parity) {
     return ETL_OK;
 }
 
-int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, double* kernel_ms, double* total_ms,
-                  int* parity) {
+int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, EtlTiming* out, int* parity) {
     ETL_CHECK_INIT();
-    ETL_REQUIRE(p != nullptr, "etl_plan_time: null plan");
+    ETL_REQUIRE(p != nullptr && out != nullptr, "etl_plan_time: null argument");
     ETL_REQUIRE(count >= 1 && count <= 100000, "etl_plan_time: count %lld outside [1, 100000]",
                 (long long)count);
-    std::vector<hipEvent_t> ev(2 * count);  // at most one launch per step
+    std::vector<hipEvent_t> ev(4 * count);  // at most one launch per step
     for (auto& e : ev) ETL_HIP(hipEventCreate(&e));
     hipEvent_t t0, t1;
     ETL_HIP(hipEventCreate(&t0));
@@ -784,17 +828,23 @@ int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, double* kernel_ms, dou
     if (rc == ETL_OK) {
         ETL_HIP(hipEventRecord(t1, s));
         ETL_HIP(hipEventSynchronize(t1));
-        double sum = 0.0;
+        double all = 0.0, interior = 0.0;
         for (int k = 0; k < launches; ++k) {
-            float ms = 0.f;
-            ETL_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
-            sum += ms;
+            float a = 0.f, b = 0.f;
+            ETL_HIP(hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 3]));
+            ETL_HIP(hipEventElapsedTime(&b, ev[4 * k + 1], ev[4 * k + 2]));
+            all += a;
+            interior += b;
         }
         float tot = 0.f;
         ETL_HIP(hipEventElapsedTime(&tot, t0, t1));
-        // Average duration of one fused launch (a launch of depth T covers T steps).
-        if (kernel_ms) *kernel_ms = sum / double(std::max(launches, 1));
-        if (total_ms) *total_ms = tot;
+        const int L = std::max(launches, 1);
+        out->launch_ms = all / L;
+        out->interior_ms = p->n_interior ? interior / L : 0.0;
+        out->total_ms = tot;
+        out->interior_cells = int64_t(p->n_interior) * p->tune.band_rows * p->wo();
+        out->launches = launches;
+        out->depth = p->tune.temporal;
         if (parity) *parity = p->parity;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);

@@ -15,7 +15,7 @@
 // registers (Ey one row more) — in a single burst of independent vector loads,
 // performs the T steps entirely in registers, and stores its BR x WO output.  Row
 // neighbours are registers of the same lane, column neighbours come from the
-// adjacent lane (ds_bpermute).  Values beyond the window are unknown: they corrupt
+// adjacent lane (DPP wave shifts).  Values beyond the window are unknown: they corrupt
 // one row and one column per half-step from each side, which the T-wide halo
 // absorbs, so the owned cells are exact.  Physical boundaries need no halo: tangential
 // E on the outer boundary (Ex at j = 0, ny; Ey at i = 0, nx) is never updated, so
@@ -85,6 +85,29 @@ __device__ __forceinline__ void store_masked(R* __restrict__ p, const R (&src)[V
 #pragma unroll
     for (int k = 0; k < V; ++k)
         if (m[k]) p[c + k] = src[k];
+}
+
+// v of lane + 1 (wave_shl:1) and of lane - 1 (wave_shr:1): DPP moves inside the VALU,
+// no LDS round trip.  Lanes 63 and 0 receive a value outside the window.
+template <int CTRL, typename R>
+__device__ __forceinline__ R lane_shift(R v) {
+    if constexpr (sizeof(R) == 4) {
+        return __int_as_float(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_update_dpp(0, int(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xf, 0xf, false);
+        return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+    }
+}
+template <typename R>
+__device__ __forceinline__ R from_next_lane(R v) {
+    return lane_shift<0x130>(v);
+}
+template <typename R>
+__device__ __forceinline__ R from_prev_lane(R v) {
+    return lane_shift<0x138>(v);
 }
 
 // Broadcast lane `l` of v to the whole wave (v_readlane; l is wave-uniform).
@@ -285,7 +308,7 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
                     ikx = lane_value(pikxh, l);
                 }
             }
-            const R exr = __shfl_down(ex[l][0], 1, 64);  // lane 63: outside the window
+            const R exr = from_next_lane(ex[l][0]);  // lane 63: outside the window
 #pragma unroll
             for (int k = 0; k < V; ++k) {
                 const R exn = (k + 1 < V) ? ex[l][k + 1] : exr;
@@ -316,9 +339,10 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
 #pragma unroll
         for (int l = t; l < RW - t; ++l) {
             const int g = g0 + l;
-            const R hl = __shfl_up(hz[l][V - 1], 1, 64);  // lane 0: outside the window
-            const bool ex_row = (g >= 0 && g <= nx - 1);
-            const bool ey_row = (l >= t + 1) && (g >= 1 && g <= nx - 1);
+            const R hl = from_prev_lane(hz[l][V - 1]);  // lane 0: outside the window
+            // Interior tiles lie inside rows [1, nx - 1] and columns [1, ny - 1].
+            const bool ex_row = !STRIP || (g >= 0 && g <= nx - 1);
+            const bool ey_row = (l >= t + 1) && (!STRIP || (g >= 1 && g <= nx - 1));
             bool xe = false;
             R bxe = R(1), cxe = R(0), ikxe = R(1);
             if constexpr (STRIP) {
@@ -332,7 +356,9 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
 #pragma unroll
             for (int k = 0; k < V; ++k) {
                 const int j = c + k;
-                if (ex_row && j >= 1 && j <= ny - 1) {  // Ex(g, j); j = 0, ny stay unchanged
+                // Ex(g, j); j = 0, ny stay unchanged (a select, not a branch)
+                if (ex_row) {
+                    const bool upd = !STRIP || (j >= 1 && j <= ny - 1);
                     const R hleft = (k == 0) ? hl : hz[l][k - 1];
                     const R d = (hz[l][k] - hleft) * idy;
                     R tt;
@@ -343,7 +369,8 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
                     } else {
                         tt = d + zero;
                     }
-                    ex[l][k] = ex[l][k] + cx[l][k] * tt;
+                    const R exn = ex[l][k] + cx[l][k] * tt;
+                    ex[l][k] = upd ? exn : ex[l][k];
                 }
                 if (ey_row) {  // Ey(g, j), g in [1, nx - 1]; rows 0 and nx stay unchanged
                     const R d = (hz[l][k] - hz[l > 0 ? l - 1 : 0][k]) * idx;

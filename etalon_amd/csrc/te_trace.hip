@@ -11,29 +11,57 @@
 namespace etl {
 namespace {
 
-// acc[pair] += sum over the chunk's rows, in step order, of trace * phasor.
+// acc[pair] += sum over the chunk's rows, in step order, of trace * phasor.  The
+// rows are read in batches of B independent loads before the sequential fold, so
+// the memory latency is paid once per batch.  counter[0] is the step of the chunk's
+// first row; the last block to finish advances it by `rows` (every block has read it
+// by then) and clears the block tally counter[1] for the next chunk.
 template <typename R>
 __global__ __launch_bounds__(256) void trace_dft_kernel(const R* __restrict__ trace, int n_slots,
                                                         int rows, const R* __restrict__ ph, int nq,
-                                                        const int* __restrict__ counter,
+                                                        int* __restrict__ counter,
                                                         const int4* __restrict__ pairs,
                                                         int n_pairs, R* const* __restrict__ accs) {
+    constexpr int B = sizeof(R) == 4 ? 32 : 16;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
-    const int4 pr = pairs[p];  // slot, q, monitor, complex element index
-    R* acc = accs[pr.z] + 2 * int64_t(pr.w);
-    R re = acc[0], im = acc[1];
-    const int base = *counter;
-    const R* __restrict__ php = ph + 2 * (int64_t(base) * nq + pr.y);
-    for (int r = 0; r < rows; ++r) {
-        const R v = trace[int64_t(r) * n_slots + pr.x];
-        const R pre = php[2 * int64_t(r) * nq];
-        const R pim = php[2 * int64_t(r) * nq + 1];
-        re = re + v * pre;
-        im = im + v * pim;
+    const int base = __atomic_load_n(counter, __ATOMIC_RELAXED);
+    if (p < n_pairs) {
+        const int4 pr = pairs[p];  // slot, q, monitor, complex element index
+        R* acc = accs[pr.z] + 2 * int64_t(pr.w);
+        R re = acc[0], im = acc[1];
+        const R* __restrict__ php = ph + 2 * (int64_t(base) * nq + pr.y);
+        const R* __restrict__ tp = trace + pr.x;
+        int r = 0;
+        for (; r + B <= rows; r += B) {
+            R v[B], pre[B], pim[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                v[b] = tp[int64_t(r + b) * n_slots];
+                pre[b] = php[2 * int64_t(r + b) * nq];
+                pim[b] = php[2 * int64_t(r + b) * nq + 1];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                re = re + v[b] * pre[b];
+                im = im + v[b] * pim[b];
+            }
+        }
+        for (; r < rows; ++r) {
+            const R v = tp[int64_t(r) * n_slots];
+            re = re + v * php[2 * int64_t(r) * nq];
+            im = im + v * php[2 * int64_t(r) * nq + 1];
+        }
+        acc[0] = re;
+        acc[1] = im;
     }
-    acc[0] = re;
-    acc[1] = im;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned done = atomicAdd(reinterpret_cast<unsigned*>(counter + 1), 1u);
+        if (done + 1 == gridDim.x) {
+            counter[1] = 0;
+            __atomic_store_n(counter, base + rows, __ATOMIC_RELAXED);
+        }
+    }
 }
 
 __global__ void bump_counter_kernel(int* counter, int by) {
@@ -43,8 +71,8 @@ __global__ void bump_counter_kernel(int* counter, int by) {
 }  // namespace
 
 template <typename R>
-int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq,
-                     const int* counter, const int4* pairs, int n_pairs, R* const* accs,
+int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq, int* counter,
+                     const int4* pairs, int n_pairs, R* const* accs,
                      hipStream_t stream) {
     if (n_pairs == 0 || rows == 0) return ETL_OK;
     hipLaunchKernelGGL(trace_dft_kernel<R>, dim3((n_pairs + 255) / 256), dim3(256), 0, stream,
@@ -59,9 +87,9 @@ int launch_bump(int* counter, int by, hipStream_t stream) {
     return ETL_OK;
 }
 
-template int launch_trace_dft<float>(const float*, int, int, const float*, int, const int*,
+template int launch_trace_dft<float>(const float*, int, int, const float*, int, int*,
                                      const int4*, int, float* const*, hipStream_t);
-template int launch_trace_dft<double>(const double*, int, int, const double*, int, const int*,
+template int launch_trace_dft<double>(const double*, int, int, const double*, int, int*,
                                       const int4*, int, double* const*, hipStream_t);
 
 }  // namespace etl

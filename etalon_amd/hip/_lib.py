@@ -50,6 +50,17 @@ class EtlTune(C.Structure):
     ]
 
 
+class EtlTiming(C.Structure):
+    _fields_ = [
+        ("launch_ms", C.c_double),
+        ("interior_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("interior_cells", C.c_int64),
+        ("launches", C.c_int),
+        ("depth", C.c_int),
+    ]
+
+
 # name -> argtypes; every function returns int
 _SIGNATURES = {
     "etl_last_error": [C.c_char_p, C.c_size_t],
@@ -78,8 +89,7 @@ _SIGNATURES = {
     "etl_plan_phasor_table": [vp, C.c_int, C.c_int64, C.c_int, vp],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],
     "etl_plan_advance": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)],
-    "etl_plan_time": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                      C.POINTER(C.c_int)],
+    "etl_plan_time": [vp, C.c_int64, C.c_int64, C.POINTER(EtlTiming), C.POINTER(C.c_int)],
 }
 
 

@@ -24,7 +24,7 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h
 @dataclass
 class Tune:
     """Fused-kernel shape; 0 = the library default (vec 1, 8-row tiles, temporal
-    depth 4 in float32 and 2 in float64, 64-step trace chunks)."""
+    depth 4, 64-step trace chunks)."""
 
     band_rows: int = 0
     vec: int = 0
@@ -118,14 +118,15 @@ class FusedPlan:
         _lib.call("etl_plan_tune", self.handle, C.byref(t))
         return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal)
 
-    def time(self, n0: int, count: int) -> tuple[float, float]:
-        """(average duration of one fused launch, total) in ms over ``count`` steps,
-        from HIP events on the library stream; a launch covers ``temporal`` steps."""
-        k, t, parity = C.c_double(), C.c_double(), C.c_int()
-        _lib.call("etl_plan_time", self.handle, int(n0), int(count), C.byref(k), C.byref(t),
-                  C.byref(parity))
+    def time(self, n0: int, count: int) -> dict:
+        """HIP-event timing of ``count`` steps launched directly: ``launch_ms`` (average
+        fused launch, all tile kinds), ``interior_ms`` (the interior-tile kernel alone,
+        which does most of the work), ``total_ms``, ``interior_cells`` (Hz cells that
+        kernel advances per step), ``launches`` and ``depth`` (steps per launch)."""
+        t, parity = _lib.EtlTiming(), C.c_int()
+        _lib.call("etl_plan_time", self.handle, int(n0), int(count), C.byref(t), C.byref(parity))
         self.parity = parity.value
-        return k.value, t.value
+        return {name: getattr(t, name) for name, _ in _lib.EtlTiming._fields_}
 
     def bind(self) -> None:
         """Point the state's field attributes at the buffers holding the current state."""

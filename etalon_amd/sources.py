@@ -48,12 +48,35 @@ class PointInjection:
         return self.coefficient * pulse_signal(self.pulse, self.sample_time(n, dt))
 
     def delta_table(self, n0: int, rows: int, dt: float) -> np.ndarray:
-        """delta(n) for n in [n0, n0 + rows), shape (rows, nodes), one host evaluation
-        per step exactly as the per-step loop does it (same scalar ufunc calls)."""
-        out = np.empty((rows, self.coefficient.size))
-        for r in range(rows):
-            out[r] = self.delta(n0 + r, dt)
+        """delta(n) for n in [n0, n0 + rows), shape (rows, nodes), float64.
+
+        Evaluated over the whole time column at once, with the operations of
+        ``delta(n)`` per step.  Per step, ``s`` is a NumPy float64 scalar and ``s**2``
+        is the scalar power (C ``pow``), which can differ by one ulp from the array
+        square, so the squares are taken per element the same way.  Whether exp and
+        sin give the same bits for an array as for one element depends on the NumPy
+        build and the CPU, so sampled rows are checked against the per-step
+        evaluation and any difference switches to one evaluation per step.
+        """
+        steps = np.arange(n0, n0 + rows, dtype=np.float64)
+        t = steps * dt if self.is_magnetic else (steps + 0.5) * dt
+        p = self.pulse
+        s = t - p.delay
+        s2 = np.fromiter((x**2 for x in s), dtype=np.float64, count=s.size)
+        g = p.amplitude * np.exp(-s2 / (2.0 * p.width**2)) * np.sin(2.0 * np.pi * p.frequency * s)
+        out = self.coefficient[None, :] * g[:, None]
+        if rows and not _matches_per_step(self, out, n0, dt):
+            for r in range(rows):
+                out[r] = self.delta(n0 + r, dt)
         return out
+
+
+def _matches_per_step(inj: PointInjection, table: np.ndarray, n0: int, dt: float) -> bool:
+    rows = table.shape[0]
+    sample = sorted(set(range(min(rows, 16))) | set(range(0, rows, max(1, rows // 64)))
+                    | {rows - 1})
+    return all(np.array_equal(table[r].view(np.int64), inj.delta(n0 + r, dt).view(np.int64))
+               for r in sample)
 
 
 def point_injection(source, grid: YeeGrid2D, eps_r: dict) -> PointInjection:

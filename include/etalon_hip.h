@@ -121,10 +121,11 @@ typedef struct EtlTeDesc {
     const void *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
 } EtlTeDesc;
 
-/* Shape of the fused kernel (0 = library default).  (vec, temporal, band_rows)
- * must be a compiled register-tile shape: float32 vec 1 with band 8 or 16, or vec 2
- * with band 8, at temporal depth 1, 2 or 4; float64 vec 1 with band 8 (depth 1, 2,
- * 4) or band 16 (depth 1, 2). */
+/* Shape of the fused kernel (0 = library default: vec 1, band 8, temporal 4,
+ * graph on, 64-step trace chunks).  (vec, temporal, band_rows) must be a compiled
+ * register-tile shape: float32 vec 1 with band 8 or 16, or vec 2 with band 8, at
+ * temporal depth 1, 2 or 4; float64 vec 1 with band 8 (depth 1, 2, 4) or band 16
+ * (depth 1, 2). */
 typedef struct EtlTune {
     int band_rows;     /* output rows of one wave's tile */
     int vec;           /* REAL elements per lane along y */
@@ -155,10 +156,16 @@ int etl_plan_tune(etl_plan_t plan, EtlTune* out);
 /* Advance steps [n, n + count); the tables must cover them.  On return *parity
  * tells which buffer of each pair holds the state (0: field[k][0]). */
 int etl_plan_advance(etl_plan_t plan, int64_t n, int64_t count, int* parity);
-/* Time `count` steps from step n on the library stream with HIP events: the
- * average duration of one fused stencil launch (ms) and of the whole call. */
-int etl_plan_time(etl_plan_t plan, int64_t n, int64_t count, double* kernel_ms,
-                  double* total_ms, int* parity);
+/* Event timing of `count` steps from step n (launched directly, not from the graph). */
+typedef struct EtlTiming {
+    double launch_ms;        /* average fused launch, fork to join (every tile kind) */
+    double interior_ms;      /* average duration of the interior-tile kernel alone */
+    double total_ms;         /* the whole call */
+    int64_t interior_cells;  /* Hz cells that kernel advances (per step of a launch) */
+    int launches;            /* fused launches timed */
+    int depth;               /* steps per launch (the plan's temporal depth) */
+} EtlTiming;
+int etl_plan_time(etl_plan_t plan, int64_t n, int64_t count, EtlTiming* out, int* parity);
 
 #ifdef __cplusplus
 }
