@@ -7,7 +7,7 @@
 
 namespace etl {
 
-constexpr int kMaxSources = 16;  // point-source nodes per plan (a bilinear source uses <= 4)
+constexpr int kMaxSources = 16;  // point-source nodes per simulation (bilinear: <= 4 each)
 
 template <typename R>
 struct FusedArgs {
@@ -19,10 +19,7 @@ struct FusedArgs {
     const R *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
     R db, inv_dx, inv_dy;
     int nx, ny;
-    int band;           // rows per tile
-    int wo;             // output columns per tile (64 * V - 2 * deepest T)
-    int n_bands, n_tj;  // tiles along x and along y
-    int tiles_per_xcd;
+    int wo;  // output columns per tile (64 * V - 2 * deepest T)
     // CPML strips: index k of an axis is in the strip iff k < lo or k >= hi.
     int xh_lo, xh_hi, xe_lo, xe_hi, yh_lo, yh_hi, ye_lo, ye_hi;
     // Point sources: n_src nodes (comp, i, j, 0) on the device; delta table rows of
@@ -43,11 +40,13 @@ struct FusedArgs {
 // Output columns of a tile for vector width v and temporal depth t.
 inline int tile_out_cols(int v, int t) { return 64 * v - 2 * t; }
 
-// Tile descriptors (first row, tile column, monitor cell, 0) split by kernel
-// variant: interior tiles carry no psi and are `band` rows high; strip tiles touch a
-// CPML strip and carry psi; extra tiles also inject sources or trace monitors.
-// Strip and extra tiles are kSubBand rows high: they are few, so splitting them
-// shortens the longest wave of a launch.
+// Tile descriptors (first row, tile column, extra-tile index, simulation) split by
+// kernel variant: interior tiles carry no psi and are `band` rows high; strip tiles
+// touch a CPML strip and carry psi; extra tiles also inject sources or trace
+// monitors.  Strip and extra tiles are kSubBand rows high: they are few, so
+// splitting them shortens the longest wave of a launch.  The simulation index
+// selects the FusedArgs of the tile in the plan's argument array: one launch
+// advances every simulation of a batch.
 constexpr int kSubBand = 4;
 struct TileList {
     const int4* interior;
@@ -77,15 +76,25 @@ inline bool tile_shape_supported(int precision, int vec, int depth, int band) {
 
 // Launch one fused step of depth T over all tiles: the interior variant on s_int, the
 // strip variant on s_strip and the extra variant on s_extra (the caller orders the
-// streams).
+// streams).  Interior and strip tiles run `waves` waves (tiles) per workgroup; extra
+// tiles one (they synchronise the workgroup).
 template <typename R>
-int launch_fused(const FusedArgs<R>& a, const TileList& tl, int vec, int depth, int band,
-                 const int* counter, int kstep0, int parity, hipStream_t s_int,
+int launch_fused(const FusedArgs<R>* args, const TileList& tl, int vec, int depth, int band,
+                 int waves, const int* counter, int kstep0, int parity, hipStream_t s_int,
                  hipStream_t s_strip, hipStream_t s_extra);
+
+// Monitor trace of one simulation, for the DFT fold.
 template <typename R>
-int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq, int* counter,
-                     const int4* pairs, int n_pairs, R* const* accs,
-                     hipStream_t stream);
+struct TraceSim {
+    const R* trace;  // (rows, n_slots)
+    const R* ph;     // (table rows, nq) complex
+    int n_slots, nq;
+};
+// pairs[p] = (slot, phasor column, accumulator, complex element), pair_sim[p] its
+// simulation.
+template <typename R>
+int launch_trace_dft(const TraceSim<R>* sims, int rows, int* counter, const int4* pairs,
+                     const int* pair_sim, int n_pairs, R* const* accs, hipStream_t stream);
 int launch_bump(int* counter, int by, hipStream_t stream);
 
 }  // namespace etl

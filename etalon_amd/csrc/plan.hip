@@ -1,11 +1,14 @@
-// Fused time-stepping plans: the C ABI around te_fused.hip.
+// Fused time-stepping plans: the C ABI around the register-tile kernels.
 //
 // A plan replaces the host loop of FDTDSolver.step()/run() (etalon/fdtd/solver.py:
-// 83-128,155-160).  The per-step host work of the reference — evaluating the source
-// pulse and the DFT phasors in float64 — is done by the caller ahead of time into
-// tables (same NumPy expressions, so the REAL values are identical), and the plan
-// replays the steps on the device: one fused kernel per T steps, one DFT kernel per
-// trace chunk and a counter bump, captured once into a hipGraph per buffer parity.
+// 83-128,155-160) for one simulation or for a batch of independent simulations of
+// any sizes that advance together.  The per-step host work of the reference —
+// evaluating the source pulse and the DFT phasors in float64 — is done by the
+// caller ahead of time into tables (same NumPy expressions, so the REAL values are
+// identical), and the plan replays the steps on the device: per T steps one launch
+// of each tile kernel over the tiles of every simulation (a work-list of tile
+// descriptors), per trace chunk one DFT fold over every monitor, captured once into
+// a hipGraph per buffer parity.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -65,49 +68,62 @@ using namespace etl;
 
 struct EtlPlan {
     int precision = 0;
-    int nx = 0, ny = 0;
     EtlTune tune{};
-    EtlTeDesc desc{};
     int parity = 0;
-    int strips[8] = {0};  // xh_lo, xh_hi, xe_lo, xe_hi, yh_lo, yh_hi, ye_lo, ye_hi
 
-    // sources
-    int n_src = 0;
-    std::vector<int4> src;  // comp, i, j, 0
-    int4* src_dev = nullptr;
-    void* delta = nullptr;
-    int64_t delta_n0 = 0;
-    int delta_rows = 0;
-    size_t delta_bytes = 0;
-    // Tables are staged on the host and uploaded in one contiguous copy each by the
-    // next advance (stale = staged but not yet on the device).
-    std::vector<unsigned char> delta_host, ph_host;  // ph_host: (ph_host_rows, nq) complex
-    int ph_host_rows = 0;
-    bool delta_stale = false, ph_stale = false;
-
-    // monitors
+    // One simulation of the plan.
     struct Mon {
         int comp, n_nodes, n_f;
         void* acc;
         std::vector<int64_t> flat;
-        int slot_base, q_base;
+        int slot_base = 0, q_base = 0;
         int64_t n0 = 0;
         int rows = -1;
     };
-    std::vector<Mon> mons;
-    bool dirty = true;  // monitor CSR / tile flags need a rebuild
-    int n_slots = 0, nq = 0, n_pairs = 0;
-    int* tile_mon[3] = {nullptr, nullptr, nullptr};   // per component: CSR over tiles
-    int4* tile_ent[3] = {nullptr, nullptr, nullptr};  // (row, column, slot, 0)
-    int4* tiles_dev = nullptr;  // interior, then strip, then extra tile descriptors
+    struct Sim {
+        int nx = 0, ny = 0;
+        EtlTeDesc desc{};
+        int strips[8] = {0};  // xh_lo, xh_hi, xe_lo, xe_hi, yh_lo, yh_hi, ye_lo, ye_hi
+        // sources
+        int n_src = 0;
+        std::vector<int4> src;  // comp, i, j, 0
+        int4* src_dev = nullptr;
+        void* delta = nullptr;
+        int64_t delta_n0 = 0;
+        int delta_rows = 0;
+        size_t delta_bytes = 0;
+        // Tables are staged on the host and uploaded in one contiguous copy each by
+        // the next advance (stale = staged but not yet on the device).
+        std::vector<unsigned char> delta_host, ph_host;  // ph_host: (ph_host_rows, nq)
+        int ph_host_rows = 0;
+        bool delta_stale = false, ph_stale = false;
+        // monitors
+        std::vector<Mon> mons;
+        int n_slots = 0, nq = 0;
+        int* tile_mon[3] = {nullptr, nullptr, nullptr};   // per component: CSR over extra tiles
+        int4* tile_ent[3] = {nullptr, nullptr, nullptr};  // (row, column, slot, 0)
+        void* trace = nullptr;                            // (trace_chunk, n_slots)
+        void* ph = nullptr;                               // (ph_rows_cap, nq) complex
+        int ph_rows_cap = 0;
+
+        int rows_of(int comp) const { return comp == 1 ? nx + 1 : nx; }
+        int pitch_of(int comp) const { return comp == 0 ? ny + 1 : ny; }
+    };
+    std::vector<Sim> sims;
+
+    bool dirty = true;        // tiles / monitor CSR / DFT pairs need a rebuild
+    bool args_stale = true;   // the device copies of the per-simulation arguments
+    void* args_dev = nullptr;        // FusedArgs<R>[n_sims]
+    void* trace_sims_dev = nullptr;  // TraceSim<R>[n_sims]
+    int4* tiles_dev = nullptr;       // interior, then strip, then extra tile descriptors
     int n_interior = 0, n_strip = 0, n_extra = 0;
+    int64_t interior_cells = 0;      // Hz cells the interior tiles own
+    int4* pairs = nullptr;           // DFT pairs of every simulation
+    int* pair_sim = nullptr;
+    void** accs = nullptr;
+    int n_pairs = 0;
     hipStream_t s_strip = nullptr, s_extra = nullptr;  // beside the interior tiles
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
-    void* trace = nullptr;
-    int4* pairs = nullptr;
-    void** accs = nullptr;
-    void* ph = nullptr;  // (ph_rows, nq) complex
-    int ph_rows_cap = 0;
 
     int* counter = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -115,10 +131,8 @@ struct EtlPlan {
 
     int elem() const { return precision == ETL_F32 ? 4 : 8; }
     int wo() const { return tile_out_cols(tune.vec, tune.temporal); }
-    int n_tj() const { return (ny + 1 + wo() - 1) / wo(); }
-    int n_bands() const { return (nx + tune.band_rows - 1) / tune.band_rows; }
-    int rows_of(int comp) const { return comp == 1 ? nx + 1 : nx; }
-    int pitch_of(int comp) const { return comp == 0 ? ny + 1 : ny; }
+    int n_tj(const Sim& s) const { return (s.ny + 1 + wo() - 1) / wo(); }
+    int n_bands(const Sim& s) const { return (s.nx + tune.band_rows - 1) / tune.band_rows; }
 };
 
 namespace {
@@ -133,19 +147,23 @@ void free_graphs(EtlPlan* p) {
 }
 
 void free_layout(EtlPlan* p) {
-    for (int c = 0; c < 3; ++c) {
-        etl_free(p->tile_mon[c]);
-        etl_free(p->tile_ent[c]);
-        p->tile_mon[c] = nullptr;
-        p->tile_ent[c] = nullptr;
+    for (auto& s : p->sims) {
+        for (int c = 0; c < 3; ++c) {
+            etl_free(s.tile_mon[c]);
+            etl_free(s.tile_ent[c]);
+            s.tile_mon[c] = nullptr;
+            s.tile_ent[c] = nullptr;
+        }
+        etl_free(s.trace);
+        s.trace = nullptr;
     }
-    etl_free(p->trace);
     etl_free(p->pairs);
+    etl_free(p->pair_sim);
     etl_free(p->accs);
     etl_free(p->tiles_dev);
     p->tiles_dev = nullptr;
-    p->trace = nullptr;
     p->pairs = nullptr;
+    p->pair_sim = nullptr;
     p->accs = nullptr;
 }
 
@@ -156,138 +174,158 @@ int upload(void** dst, const void* src, size_t bytes) {
 }
 
 // Tile descriptors of the three kernel variants, the monitor CSR over extra tiles,
-// DFT pairs and the trace buffer; rebuilt whenever sources, monitors or tiling change.
+// DFT pairs and the trace buffers; rebuilt whenever sources, monitors or tiling change.
 //
-// A full tile is `band` rows by WO columns.  Its window — rows [i0 - T, i0 + band + T]
-// and columns [o0 - T, o0 - T + W) — decides its variant: interior when it meets no
-// CPML strip, no monitored node it owns and no source; otherwise it is split into
-// kSubBand-row sub-tiles, each extra when it owns a monitored node or its own window
-// holds a source, else strip.
+// A full tile of a simulation is `band` rows by WO columns.  Its window — rows
+// [i0 - T, i0 + band + T] and columns [o0 - T, o0 - T + W) — decides its variant:
+// interior when it meets no CPML strip, no monitored node it owns and no source;
+// otherwise it is split into kSubBand-row sub-tiles, each extra when it owns a
+// monitored node or its own window holds a source, else strip.
 int build_layout(EtlPlan* p) {
     ETL_HIP(hipStreamSynchronize(ctx().stream));
     free_layout(p);
     free_graphs(p);
-    const int ntj = p->n_tj();
-    const int nb = p->n_bands();
     const int wo = p->wo();
     const int T = p->tune.temporal;
     const int W = 64 * p->tune.vec;
     const int band = p->tune.band_rows;
-    const int nx = p->nx;
 
-    // monitored nodes: slots, DFT pairs
-    struct Node {
-        int comp, row, col, slot;
-    };
-    std::vector<Node> nodes;
-    std::vector<int4> pairs;
+    std::vector<int4> interior, strip, extra, pairs;
+    std::vector<int> pair_sim;
     std::vector<void*> accs;
-    int slot = 0, q = 0;
-    for (size_t m = 0; m < p->mons.size(); ++m) {
-        auto& mon = p->mons[m];
-        mon.slot_base = slot;
-        mon.q_base = q;
-        const int pitch = p->pitch_of(mon.comp);
-        for (int k = 0; k < mon.n_nodes; ++k) {
-            const int64_t fl = mon.flat[k];
-            const int row = int(fl / pitch);
-            nodes.push_back({mon.comp, row, int(fl - int64_t(row) * pitch), slot + k});
-        }
-        for (int f = 0; f < mon.n_f; ++f)
-            for (int k = 0; k < mon.n_nodes; ++k)
-                pairs.push_back(make_int4(slot + k, q + f, int(m), f * mon.n_nodes + k));
-        accs.push_back(mon.acc);
-        slot += mon.n_nodes;
-        q += mon.n_f;
-    }
+    int64_t interior_cells = 0;
+    for (int si = 0; si < int(p->sims.size()); ++si) {
+        auto& sim = p->sims[si];
+        const int ntj = p->n_tj(sim);
+        const int nb = p->n_bands(sim);
+        const int nx = sim.nx;
 
-    const int* st = p->strips;
-    const int x_lo = std::max(st[0], st[2]), x_hi = std::min(st[1], st[3]);
-    const int y_lo = std::max(st[4], st[6]), y_hi = std::min(st[5], st[7]);
-    auto owns_node = [&](int r0, int r1, int tj) {
-        for (const Node& n : nodes)
-            if (n.row >= r0 && n.row < r1 && n.col / wo == tj) return true;
-        return false;
-    };
-    auto sees_source = [&](int r0, int r1, int tj) {
-        const int cs = tj * wo - T;
-        for (const int4& s : p->src)
-            if (s.y >= r0 - T && s.y <= r1 + T && s.z >= cs && s.z < cs + W) return true;
-        return false;
-    };
-    std::vector<int4> interior, strip, extra;
-    for (int tb = 0; tb < nb; ++tb) {
-        const int i0 = tb * band;
-        const int i1 = std::min(i0 + band, nx);
-        const bool rows_touch = (i0 - T < x_lo) || (i1 + T > x_hi);
-        for (int tj = 0; tj < ntj; ++tj) {
-            const int cs = tj * wo - T;
-            const bool cols_touch = (cs < y_lo) || (cs + W > y_hi);
-            if (!rows_touch && !cols_touch && !owns_node(i0, i1, tj) && !sees_source(i0, i1, tj)) {
-                interior.push_back(make_int4(i0, tj, 0, 0));
-                continue;
+        // monitored nodes: slots, DFT pairs
+        struct Node {
+            int comp, row, col, slot;
+        };
+        std::vector<Node> nodes;
+        int slot = 0, q = 0;
+        for (auto& mon : sim.mons) {
+            mon.slot_base = slot;
+            mon.q_base = q;
+            const int pitch = sim.pitch_of(mon.comp);
+            for (int k = 0; k < mon.n_nodes; ++k) {
+                const int64_t fl = mon.flat[k];
+                const int row = int(fl / pitch);
+                nodes.push_back({mon.comp, row, int(fl - int64_t(row) * pitch), slot + k});
             }
-            for (int r0 = i0; r0 < i1; r0 += kSubBand) {
-                const int r1 = std::min(r0 + kSubBand, nx);
-                if (owns_node(r0, r1, tj) || sees_source(r0, r1, tj))
-                    extra.push_back(make_int4(r0, tj, int(extra.size()), 0));
-                else
-                    strip.push_back(make_int4(r0, tj, 0, 0));
+            const int acc_index = int(accs.size());
+            for (int f = 0; f < mon.n_f; ++f)
+                for (int k = 0; k < mon.n_nodes; ++k) {
+                    pairs.push_back(make_int4(slot + k, q + f, acc_index, f * mon.n_nodes + k));
+                    pair_sim.push_back(si);
+                }
+            accs.push_back(mon.acc);
+            slot += mon.n_nodes;
+            q += mon.n_f;
+        }
+        sim.n_slots = slot;
+        sim.nq = q;
+
+        const int* st = sim.strips;
+        const int x_lo = std::max(st[0], st[2]), x_hi = std::min(st[1], st[3]);
+        const int y_lo = std::max(st[4], st[6]), y_hi = std::min(st[5], st[7]);
+        auto owns_node = [&](int r0, int r1, int tj) {
+            for (const Node& n : nodes)
+                if (n.row >= r0 && n.row < r1 && n.col / wo == tj) return true;
+            return false;
+        };
+        auto sees_source = [&](int r0, int r1, int tj) {
+            const int cs = tj * wo - T;
+            for (const int4& s : sim.src)
+                if (s.y >= r0 - T && s.y <= r1 + T && s.z >= cs && s.z < cs + W) return true;
+            return false;
+        };
+        const int extra0 = int(extra.size());
+        for (int tb = 0; tb < nb; ++tb) {
+            const int i0 = tb * band;
+            const int i1 = std::min(i0 + band, nx);
+            const bool rows_touch = (i0 - T < x_lo) || (i1 + T > x_hi);
+            for (int tj = 0; tj < ntj; ++tj) {
+                const int cs = tj * wo - T;
+                const bool cols_touch = (cs < y_lo) || (cs + W > y_hi);
+                if (!rows_touch && !cols_touch && !owns_node(i0, i1, tj) &&
+                    !sees_source(i0, i1, tj)) {
+                    interior.push_back(make_int4(i0, tj, 0, si));
+                    interior_cells += int64_t(band) * wo;
+                    continue;
+                }
+                for (int r0 = i0; r0 < i1; r0 += kSubBand) {
+                    const int r1 = std::min(r0 + kSubBand, nx);
+                    if (owns_node(r0, r1, tj) || sees_source(r0, r1, tj))
+                        extra.push_back(make_int4(r0, tj, int(extra.size()) - extra0, si));
+                    else
+                        strip.push_back(make_int4(r0, tj, 0, si));
+                }
             }
         }
-    }
-    // monitor entries per extra sub-tile (row, column, slot)
-    std::vector<std::vector<std::pair<int, int4>>> ent(3);
-    for (const Node& n : nodes) {
-        const int tj = n.col / wo;
-        for (const int4& e : extra)
-            if (e.y == tj && n.row >= e.x && n.row < e.x + kSubBand) {
-                ent[n.comp].push_back({e.z, make_int4(n.row, n.col, n.slot, 0)});
-                break;
+        // monitor entries per extra sub-tile of this simulation (row, column, slot)
+        const int n_extra_sim = int(extra.size()) - extra0;
+        std::vector<std::vector<std::pair<int, int4>>> ent(3);
+        for (const Node& n : nodes) {
+            const int tj = n.col / wo;
+            for (int e = extra0; e < int(extra.size()); ++e) {
+                const int4& x = extra[e];
+                if (x.y == tj && n.row >= x.x && n.row < x.x + kSubBand) {
+                    ent[n.comp].push_back({x.z, make_int4(n.row, n.col, n.slot, 0)});
+                    break;
+                }
             }
-    }
-    p->n_slots = slot;
-    p->nq = q;
-    p->n_pairs = int(pairs.size());
-    int rc;
-    for (int c = 0; c < 3; ++c) {
-        if (ent[c].empty()) continue;
-        std::stable_sort(ent[c].begin(), ent[c].end(),
-                         [](const auto& a, const auto& b) { return a.first < b.first; });
-        std::vector<int> ptr(extra.size() + 1, 0);
-        for (auto& e : ent[c]) ptr[e.first + 1]++;
-        for (size_t k = 0; k < extra.size(); ++k) ptr[k + 1] += ptr[k];
-        std::vector<int4> flat(ent[c].size());
-        for (size_t k = 0; k < ent[c].size(); ++k) flat[k] = ent[c][k].second;
-        if ((rc = upload((void**)&p->tile_mon[c], ptr.data(), ptr.size() * sizeof(int)))) return rc;
-        if ((rc = upload((void**)&p->tile_ent[c], flat.data(), flat.size() * sizeof(int4))))
+        }
+        int rc;
+        for (int c = 0; c < 3; ++c) {
+            if (ent[c].empty()) continue;
+            std::stable_sort(ent[c].begin(), ent[c].end(),
+                             [](const auto& a, const auto& b) { return a.first < b.first; });
+            std::vector<int> ptr(n_extra_sim + 1, 0);
+            for (auto& e : ent[c]) ptr[e.first + 1]++;
+            for (int k = 0; k < n_extra_sim; ++k) ptr[k + 1] += ptr[k];
+            std::vector<int4> flat(ent[c].size());
+            for (size_t k = 0; k < ent[c].size(); ++k) flat[k] = ent[c][k].second;
+            if ((rc = upload((void**)&sim.tile_mon[c], ptr.data(), ptr.size() * sizeof(int))))
+                return rc;
+            if ((rc = upload((void**)&sim.tile_ent[c], flat.data(), flat.size() * sizeof(int4))))
+                return rc;
+        }
+        if (sim.n_slots &&
+            (rc = etl_malloc(&sim.trace, size_t(p->tune.trace_chunk) * sim.n_slots * p->elem())))
             return rc;
     }
     std::vector<int4> tiles(interior);
     tiles.insert(tiles.end(), strip.begin(), strip.end());
     tiles.insert(tiles.end(), extra.begin(), extra.end());
-    if ((rc = upload((void**)&p->tiles_dev, tiles.data(), tiles.size() * sizeof(int4)))) return rc;
+    int rc;
+    if (!tiles.empty() &&
+        (rc = upload((void**)&p->tiles_dev, tiles.data(), tiles.size() * sizeof(int4))))
+        return rc;
     p->n_interior = int(interior.size());
     p->n_strip = int(strip.size());
     p->n_extra = int(extra.size());
+    p->interior_cells = interior_cells;
+    p->n_pairs = int(pairs.size());
     if (p->n_pairs) {
         if ((rc = upload((void**)&p->pairs, pairs.data(), pairs.size() * sizeof(int4)))) return rc;
-        if ((rc = upload((void**)&p->accs, accs.data(), accs.size() * sizeof(void*)))) return rc;
-        if ((rc = etl_malloc(&p->trace,
-                             size_t(p->tune.trace_chunk) * std::max(1, p->n_slots) * p->elem())))
+        if ((rc = upload((void**)&p->pair_sim, pair_sim.data(), pair_sim.size() * sizeof(int))))
             return rc;
+        if ((rc = upload((void**)&p->accs, accs.data(), accs.size() * sizeof(void*)))) return rc;
     }
     p->dirty = false;
+    p->args_stale = true;
     return ETL_OK;
 }
 
 template <typename R>
-FusedArgs<R> make_args(const EtlPlan* p) {
-    FusedArgs<R> a;
-    std::memset(&a, 0, sizeof a);
+FusedArgs<R> make_args(const EtlPlan* p, const EtlPlan::Sim& s) {
+    FusedArgs<R> a{};
     for (int k = 0; k < 7; ++k)
-        for (int b = 0; b < 2; ++b) a.f[k][b] = static_cast<const R*>(p->desc.field[k][b]);
-    const EtlTeDesc& d = p->desc;
+        for (int b = 0; b < 2; ++b) a.f[k][b] = (const R*)s.desc.field[k][b];
+    const EtlTeDesc& d = s.desc;
     a.cb_ex = (const R*)d.cb_ex;
     a.cb_ey = (const R*)d.cb_ey;
     a.bx_e = (const R*)d.bx_e;
@@ -305,42 +343,65 @@ FusedArgs<R> make_args(const EtlPlan* p) {
     a.db = R(d.db);
     a.inv_dx = R(d.inv_dx);
     a.inv_dy = R(d.inv_dy);
-    a.nx = p->nx;
-    a.ny = p->ny;
-    a.band = p->tune.band_rows;
+    a.nx = s.nx;
+    a.ny = s.ny;
     a.wo = p->wo();
-    a.n_bands = p->n_bands();
-    a.n_tj = p->n_tj();
-    a.tiles_per_xcd = (a.n_bands * a.n_tj + 7) / 8;
-    a.xh_lo = p->strips[0];
-    a.xh_hi = p->strips[1];
-    a.xe_lo = p->strips[2];
-    a.xe_hi = p->strips[3];
-    a.yh_lo = p->strips[4];
-    a.yh_hi = p->strips[5];
-    a.ye_lo = p->strips[6];
-    a.ye_hi = p->strips[7];
-    a.n_src = p->n_src;
-    a.src = p->src_dev;
-    a.delta = (const R*)p->delta;
-    a.has_mon = p->n_slots > 0;
+    a.xh_lo = s.strips[0];
+    a.xh_hi = s.strips[1];
+    a.xe_lo = s.strips[2];
+    a.xe_hi = s.strips[3];
+    a.yh_lo = s.strips[4];
+    a.yh_hi = s.strips[5];
+    a.ye_lo = s.strips[6];
+    a.ye_hi = s.strips[7];
+    a.n_src = s.n_src;
+    a.src = s.src_dev;
+    a.delta = (const R*)s.delta;
+    a.has_mon = s.n_slots > 0;
     for (int c = 0; c < 3; ++c) {
-        a.tile_mon[c] = p->tile_mon[c];
-        a.tile_ent[c] = p->tile_ent[c];
+        a.tile_mon[c] = s.tile_mon[c];
+        a.tile_ent[c] = s.tile_ent[c];
     }
-    a.trace = (R*)p->trace;
-    a.n_slots = p->n_slots;
+    a.trace = (R*)s.trace;
+    a.n_slots = s.n_slots;
     return a;
 }
 
+// Refresh the device copies of the per-simulation arguments (after the steps in
+// flight, which read them).  Their addresses never change, so captured graphs stay
+// valid.
+template <typename R>
+int upload_args_t(EtlPlan* p) {
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    std::vector<FusedArgs<R>> args;
+    std::vector<TraceSim<R>> traces;
+    for (const auto& s : p->sims) {
+        args.push_back(make_args<R>(p, s));
+        traces.push_back(TraceSim<R>{(const R*)s.trace, (const R*)s.ph, s.n_slots, s.nq});
+    }
+    int rc;
+    if ((rc = etl_memcpy_h2d(p->args_dev, args.data(), args.size() * sizeof(FusedArgs<R>))))
+        return rc;
+    if ((rc = etl_memcpy_h2d(p->trace_sims_dev, traces.data(),
+                             traces.size() * sizeof(TraceSim<R>))))
+        return rc;
+    p->args_stale = false;
+    return ETL_OK;
+}
+
+int upload_args(EtlPlan* p) {
+    return p->precision == ETL_F32 ? upload_args_t<float>(p) : upload_args_t<double>(p);
+}
+
 // Enqueue `count` (<= trace_chunk) steps starting at buffer parity `parity`: fused
-// launches of depth T, then depth-1 launches for the remainder, the DFT of the trace
-// rows and the counter bump.  Returns the number of fused launches in *launches.
-// `events` (optional) receives four events per fused launch: fork, interior kernel
-// start and end (on the main stream, where only that kernel runs), join.
+// launches of depth T, then depth-1 launches for the remainder, and the DFT of the
+// trace rows (which advances the step counter) or a counter bump.  Returns the
+// number of fused launches in *launches.  `events` (optional) receives four events
+// per fused launch: fork, interior kernel start and end (on the main stream, where
+// only that kernel runs), join.
 template <typename R>
 int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
-    const FusedArgs<R> a = make_args<R>(p);
+    const FusedArgs<R>* args = static_cast<const FusedArgs<R>*>(p->args_dev);
     hipStream_t s = ctx().stream;
     const TileList tl{p->tiles_dev,
                       p->n_interior,
@@ -358,8 +419,8 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
         ETL_HIP(hipStreamWaitEvent(p->s_strip, p->ev_fork, 0));
         ETL_HIP(hipStreamWaitEvent(p->s_extra, p->ev_fork, 0));
         if (events) ETL_HIP(hipEventRecord(events[4 * n + 1], s));
-        int rc = launch_fused<R>(a, tl, p->tune.vec, depth, p->tune.band_rows, p->counter, k,
-                                 parity ^ (n & 1), s, p->s_strip, p->s_extra);
+        int rc = launch_fused<R>(args, tl, p->tune.vec, depth, p->tune.band_rows, p->tune.waves,
+                                 p->counter, k, parity ^ (n & 1), s, p->s_strip, p->s_extra);
         if (rc) return rc;
         if (events) ETL_HIP(hipEventRecord(events[4 * n + 2], s));
         ETL_HIP(hipEventRecord(p->ev_join, p->s_strip));
@@ -371,10 +432,10 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
         ++n;
     }
     *launches = n;
-    // the DFT kernel advances the step counter itself
     if (p->n_pairs && count)
-        return launch_trace_dft<R>((const R*)p->trace, p->n_slots, count, (const R*)p->ph, p->nq,
-                                   p->counter, p->pairs, p->n_pairs, (R* const*)p->accs, s);
+        return launch_trace_dft<R>(static_cast<const TraceSim<R>*>(p->trace_sims_dev), count,
+                                   p->counter, p->pairs, p->pair_sim, p->n_pairs,
+                                   (R* const*)p->accs, s);
     return launch_bump(p->counter, count, s);
 }
 
@@ -391,38 +452,42 @@ int table_capacity(int rows) {
 }
 
 // Upload the staged tables (after the steps in flight, which may read the old ones).
-// A larger device table gets a new address, which invalidates the captured graphs.
+// A larger table gets a new address, which the argument arrays carry.
 int upload_tables(EtlPlan* p) {
-    if (!p->delta_stale && !p->ph_stale) return ETL_OK;
+    bool any = false;
+    for (const auto& s : p->sims) any = any || s.delta_stale || s.ph_stale;
+    if (!any) return ETL_OK;
     ETL_HIP(hipStreamSynchronize(ctx().stream));
     int rc;
-    if (p->delta_stale) {
-        const size_t bytes = p->delta_host.size();
-        if (bytes > p->delta_bytes) {
-            const size_t cap = size_t(table_capacity(p->delta_rows)) * p->n_src * p->elem();
-            etl_free(p->delta);
-            p->delta = nullptr;
-            if ((rc = etl_malloc(&p->delta, cap))) return rc;
-            p->delta_bytes = cap;
-            free_graphs(p);
+    for (auto& s : p->sims) {
+        if (s.delta_stale) {
+            const size_t bytes = s.delta_host.size();
+            if (bytes > s.delta_bytes) {
+                const size_t cap = size_t(table_capacity(s.delta_rows)) * s.n_src * p->elem();
+                etl_free(s.delta);
+                s.delta = nullptr;
+                if ((rc = etl_malloc(&s.delta, cap))) return rc;
+                s.delta_bytes = cap;
+                p->args_stale = true;
+            }
+            if (bytes && (rc = etl_memcpy_h2d(s.delta, s.delta_host.data(), bytes))) return rc;
+            s.delta_stale = false;
         }
-        if (bytes && (rc = etl_memcpy_h2d(p->delta, p->delta_host.data(), bytes))) return rc;
-        p->delta_stale = false;
-    }
-    if (p->ph_stale) {
-        const size_t row_bytes = size_t(p->nq) * 2 * p->elem();
-        if (p->ph_host_rows > p->ph_rows_cap) {
-            const int cap = table_capacity(p->ph_host_rows);
-            etl_free(p->ph);
-            p->ph = nullptr;
-            if ((rc = etl_malloc(&p->ph, size_t(cap) * row_bytes))) return rc;
-            p->ph_rows_cap = cap;
-            free_graphs(p);
+        if (s.ph_stale) {
+            const size_t row_bytes = size_t(s.nq) * 2 * p->elem();
+            if (s.ph_host_rows > s.ph_rows_cap) {
+                const int cap = table_capacity(s.ph_host_rows);
+                etl_free(s.ph);
+                s.ph = nullptr;
+                if ((rc = etl_malloc(&s.ph, size_t(cap) * row_bytes))) return rc;
+                s.ph_rows_cap = cap;
+                p->args_stale = true;
+            }
+            if (!s.ph_host.empty() &&
+                (rc = etl_memcpy_h2d(s.ph, s.ph_host.data(), s.ph_host.size())))
+                return rc;
+            s.ph_stale = false;
         }
-        if (!p->ph_host.empty() &&
-            (rc = etl_memcpy_h2d(p->ph, p->ph_host.data(), p->ph_host.size())))
-            return rc;
-        p->ph_stale = false;
     }
     return ETL_OK;
 }
@@ -445,32 +510,46 @@ int ensure_graph(EtlPlan* p, int parity) {
     return ETL_OK;
 }
 
-int check_tables(EtlPlan* p, int64_t n, int64_t count) {
-    if (p->n_src > 0) {
-        ETL_REQUIRE(p->delta != nullptr, "etl_plan_advance: no source table");
-        if (!(n >= p->delta_n0 && n + count <= p->delta_n0 + p->delta_rows)) {
-            set_error("etl_plan_advance: source table covers steps [%lld, %lld), asked [%lld, %lld)",
-                      (long long)p->delta_n0, (long long)(p->delta_n0 + p->delta_rows),
-                      (long long)n, (long long)(n + count));
-            return ETL_ESTATE;
+// Every table of every simulation must cover [n, n + count) and all must start at the
+// same step (one step counter indexes them all).  *n0 receives that step.
+int check_tables(EtlPlan* p, int64_t n, int64_t count, int64_t* n0) {
+    bool have = false;
+    *n0 = n;
+    auto same_start = [&](int64_t start) -> bool {
+        if (!have) {
+            *n0 = start;
+            have = true;
+            return true;
+        }
+        return start == *n0;
+    };
+    for (size_t si = 0; si < p->sims.size(); ++si) {
+        const auto& s = p->sims[si];
+        if (s.n_src > 0) {
+            if (s.delta_rows <= 0 || !(n >= s.delta_n0 && n + count <= s.delta_n0 + s.delta_rows)) {
+                set_error("etl_plan_advance: the source table of simulation %d covers steps "
+                          "[%lld, %lld), asked [%lld, %lld)", int(si), (long long)s.delta_n0,
+                          (long long)(s.delta_n0 + s.delta_rows), (long long)n,
+                          (long long)(n + count));
+                return ETL_ESTATE;
+            }
+            if (!same_start(s.delta_n0)) {
+                set_error("etl_plan_advance: tables start at different steps");
+                return ETL_ESTATE;
+            }
+        }
+        for (const auto& m : s.mons) {
+            if (m.rows < 0 || !(n >= m.n0 && n + count <= m.n0 + m.rows)) {
+                set_error("etl_plan_advance: a phasor table of simulation %d does not cover "
+                          "steps [%lld, %lld)", int(si), (long long)n, (long long)(n + count));
+                return ETL_ESTATE;
+            }
+            if (!same_start(m.n0)) {
+                set_error("etl_plan_advance: tables start at different steps");
+                return ETL_ESTATE;
+            }
         }
     }
-    for (auto& m : p->mons) {
-        if (m.rows < 0 || !(n >= m.n0 && n + count <= m.n0 + m.rows)) {
-            set_error("etl_plan_advance: a phasor table does not cover steps [%lld, %lld)",
-                      (long long)n, (long long)(n + count));
-            return ETL_ESTATE;
-        }
-        if (p->n_src > 0 && m.n0 != p->delta_n0) {
-            set_error("etl_plan_advance: source and phasor tables start at different steps");
-            return ETL_ESTATE;
-        }
-    }
-    for (size_t k = 1; k < p->mons.size(); ++k)
-        if (p->mons[k].n0 != p->mons[0].n0) {
-            set_error("etl_plan_advance: phasor tables start at different steps");
-            return ETL_ESTATE;
-        }
     return ETL_OK;
 }
 
@@ -480,8 +559,11 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_l
         if ((rc = build_layout(p))) return rc;
     }
     if ((rc = upload_tables(p))) return rc;
-    if ((rc = check_tables(p, n, count))) return rc;
-    const int64_t n0 = p->n_src > 0 ? p->delta_n0 : (p->mons.empty() ? n : p->mons[0].n0);
+    int64_t n0 = n;
+    if ((rc = check_tables(p, n, count, &n0))) return rc;
+    if (p->args_stale) {
+        if ((rc = upload_args(p))) return rc;
+    }
     const int64_t start = n - n0;
     ETL_REQUIRE(start >= 0 && start + count < (int64_t(1) << 31), "etl_plan_advance: step range");
     hipStream_t s = ctx().stream;
@@ -511,15 +593,11 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_l
     return ETL_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out) {
-    ETL_CHECK_INIT();
-    ETL_REQUIRE(desc && out, "etl_plan_create: null argument");
-    ETL_REQUIRE(desc->precision == ETL_F32 || desc->precision == ETL_F64,
-                "etl_plan_create: unknown precision %d", desc->precision);
+// Validate one simulation's descriptor and find its CPML strips; checks that psi is +0
+// wherever the fused kernel leaves it untouched.
+int init_sim(EtlPlan* p, const EtlTeDesc* desc, EtlPlan::Sim& sim) {
+    ETL_REQUIRE(desc->precision == p->precision,
+                "etl_plan_create: every simulation of a plan needs the same precision");
     ETL_REQUIRE(desc->nx >= 1 && desc->ny >= 1 && desc->nx < (1 << 24) && desc->ny < (1 << 24),
                 "etl_plan_create: bad grid %d x %d", desc->nx, desc->ny);
     for (int k = 0; k < 7; ++k)
@@ -529,30 +607,28 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
                     desc->bx_h && desc->cx_h && desc->ikx_h && desc->by_e && desc->cy_e &&
                     desc->iky_e && desc->by_h && desc->cy_h && desc->iky_h,
                 "etl_plan_create: null coefficient array");
-    EtlTune t{};
-    if (tune) t = *tune;
-    const bool f32 = desc->precision == ETL_F32;
-    if (t.vec == 0) t.vec = 1;
-    if (t.temporal == 0) t.temporal = 4;
-    if (t.band_rows == 0) t.band_rows = 8;
-    if (t.trace_chunk == 0) t.trace_chunk = 64;
-    if (tune == nullptr) t.use_graph = 1;
-    ETL_REQUIRE(tile_shape_supported(desc->precision, t.vec, t.temporal, t.band_rows) &&
-                    tile_shape_supported(desc->precision, t.vec, 1, t.band_rows),
-                "etl_plan_create: no compiled tile shape vec=%d temporal=%d band_rows=%d",
-                t.vec, t.temporal, t.band_rows);
-    ETL_REQUIRE(t.trace_chunk >= 2 && t.trace_chunk <= 4096 &&
-                    t.trace_chunk % (2 * t.temporal) == 0,
-                "etl_plan_create: trace_chunk must be a multiple of 2 * temporal, <= 4096");
-
-    EtlPlan* p = new EtlPlan();
-    p->precision = desc->precision;
-    p->nx = desc->nx;
-    p->ny = desc->ny;
-    p->tune = t;
-    p->desc = *desc;
-    const int nx = p->nx, ny = p->ny;
-
+    sim.nx = desc->nx;
+    sim.ny = desc->ny;
+    sim.desc = *desc;
+    const int nx = sim.nx, ny = sim.ny;
+    const bool f32 = p->precision == ETL_F32;
+    // The fused kernel loads whole vectors that start up to `temporal` elements before
+    // a row and may end past it: every array must carry etl_malloc's guard bands and be
+    // at least its shape's size.
+    {
+        const size_t el = p->elem();
+        const size_t sz_ex = size_t(nx) * (ny + 1) * el, sz_ey = size_t(nx + 1) * ny * el,
+                     sz_hz = size_t(nx) * ny * el;
+        const size_t need[7] = {sz_ex, sz_ey, sz_hz, sz_ex, sz_ey, sz_hz, sz_hz};
+        bool ok = guarded_bytes(desc->cb_ex) >= sz_ex && guarded_bytes(desc->cb_ey) >= sz_ey;
+        for (int k = 0; k < 7; ++k)
+            for (int b = 0; b < 2; ++b) ok = ok && guarded_bytes(desc->field[k][b]) >= need[k];
+        if (!ok) {
+            set_error("etl_plan_create: field and cb arrays must be etl_malloc allocations of "
+                      "their full shape (the fused kernel reads into their guard bands)");
+            return ETL_EARG;
+        }
+    }
     // CPML strips from the 1D profiles.
     auto fetch = [&](const void* src, int n, auto& v) -> int {
         v.resize(n);
@@ -575,26 +651,108 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
         }
         return ETL_OK;
     };
-    // The fused kernel loads whole vectors that overhang rows by up to 64*vec
-    // elements... clamped to the row end, so by less than vec elements, and starts up
-    // to `temporal` elements before a row: every array must carry etl_malloc's guard
-    // bands and be at least its shape's size.
-    {
-        const size_t el = p->elem();
-        const size_t sz_ex = size_t(nx) * (ny + 1) * el, sz_ey = size_t(nx + 1) * ny * el,
-                     sz_hz = size_t(nx) * ny * el;
-        const size_t need[7] = {sz_ex, sz_ey, sz_hz, sz_ex, sz_ey, sz_hz, sz_hz};
-        bool ok = guarded_bytes(desc->cb_ex) >= sz_ex && guarded_bytes(desc->cb_ey) >= sz_ey;
-        for (int k = 0; k < 7; ++k)
-            for (int b = 0; b < 2; ++b) ok = ok && guarded_bytes(desc->field[k][b]) >= need[k];
-        if (!ok) {
-            delete p;
-            set_error("etl_plan_create: field and cb arrays must be etl_malloc allocations of "
-                      "their full shape (the fused kernel reads into their guard bands)");
-            return ETL_EARG;
-        }
+    int rc = f32 ? bounds(float(0), sim.strips) : bounds(double(0), sim.strips);
+    if (rc) return rc;
+    // psi must be +0 wherever the fused kernel does not touch it.
+    int* flag = nullptr;
+    if ((rc = etl_malloc((void**)&flag, 256))) return rc;
+    rc = zero_async(flag, 4);
+    struct {
+        int field, rows, cols, lo_r, hi_r, lo_c, hi_c, er, ec;
+    } chk[4] = {
+        {3, nx, ny + 1, 0, nx, sim.strips[6], sim.strips[7], 0, 1},  // psi_ex_y
+        {4, nx + 1, ny, sim.strips[2], sim.strips[3], 0, ny, 1, 0},  // psi_ey_x
+        {5, nx, ny, sim.strips[0], sim.strips[1], 0, ny, 0, 0},      // psi_hz_x
+        {6, nx, ny, 0, nx, sim.strips[4], sim.strips[5], 0, 0},      // psi_hz_y
+    };
+    for (int k = 0; k < 4 && rc == ETL_OK; ++k) {
+        const auto& q = chk[k];
+        const int64_t n = int64_t(q.rows) * q.cols;
+        const dim3 grid(unsigned((n + 255) / 256));
+        if (f32)
+            hipLaunchKernelGGL(psi_check_kernel<float>, grid, dim3(256), 0, ctx().stream,
+                               (const float*)desc->field[q.field][0], q.rows, q.cols, q.lo_r,
+                               q.hi_r, q.lo_c, q.hi_c, q.er, q.ec, flag);
+        else
+            hipLaunchKernelGGL(psi_check_kernel<double>, grid, dim3(256), 0, ctx().stream,
+                               (const double*)desc->field[q.field][0], q.rows, q.cols, q.lo_r,
+                               q.hi_r, q.lo_c, q.hi_c, q.er, q.ec, flag);
+        if (hipGetLastError() != hipSuccess) rc = ETL_EHIP;
     }
-    int rc = f32 ? bounds(float(0), p->strips) : bounds(double(0), p->strips);
+    int host_flag = 0;
+    if (rc == ETL_OK) rc = etl_memcpy_d2h(&host_flag, flag, 4);
+    etl_free(flag);
+    if (rc == ETL_OK && host_flag) {
+        set_error("etl_plan_create: psi is non-zero outside the CPML strips; the fused "
+                  "path needs psi = +0 there (use the per-half-step primitives)");
+        rc = ETL_ESTATE;
+    }
+    return rc;
+}
+
+void destroy_plan(EtlPlan* p) {
+    free_graphs(p);
+    free_layout(p);
+    for (auto& s : p->sims) {
+        etl_free(s.src_dev);
+        etl_free(s.delta);
+        etl_free(s.ph);
+    }
+    etl_free(p->args_dev);
+    etl_free(p->trace_sims_dev);
+    etl_free(p->counter);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->ev_join2) (void)hipEventDestroy(p->ev_join2);
+    for (hipStream_t side : {p->s_strip, p->s_extra})
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+    delete p;
+}
+
+EtlPlan::Sim* sim_of(EtlPlan* p, int sim) {
+    if (p == nullptr || sim < 0 || sim >= int(p->sims.size())) return nullptr;
+    return &p->sims[sim];
+}
+
+}  // namespace
+
+extern "C" {
+
+int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tune,
+                          etl_plan_t* out) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(descs && out, "etl_plan_create: null argument");
+    ETL_REQUIRE(n_sims >= 1 && n_sims <= (1 << 20), "etl_plan_create: %d simulations", n_sims);
+    const int precision = descs[0].precision;
+    ETL_REQUIRE(precision == ETL_F32 || precision == ETL_F64,
+                "etl_plan_create: unknown precision %d", precision);
+    EtlTune t{};
+    if (tune) t = *tune;
+    if (t.vec == 0) t.vec = 1;
+    if (t.temporal == 0) t.temporal = 4;
+    if (t.band_rows == 0) t.band_rows = 8;
+    if (t.trace_chunk == 0) t.trace_chunk = 64;
+    if (t.waves == 0) t.waves = 1;
+    if (tune == nullptr) t.use_graph = 1;
+    ETL_REQUIRE(tile_shape_supported(precision, t.vec, t.temporal, t.band_rows) &&
+                    tile_shape_supported(precision, t.vec, 1, t.band_rows),
+                "etl_plan_create: no compiled tile shape vec=%d temporal=%d band_rows=%d",
+                t.vec, t.temporal, t.band_rows);
+    ETL_REQUIRE(t.trace_chunk >= 2 && t.trace_chunk <= 4096 &&
+                    t.trace_chunk % (2 * t.temporal) == 0,
+                "etl_plan_create: trace_chunk must be a multiple of 2 * temporal, <= 4096");
+    ETL_REQUIRE(t.waves == 1 || t.waves == 2 || t.waves == 4,
+                "etl_plan_create: waves per workgroup must be 1, 2 or 4 (got %d)", t.waves);
+
+    EtlPlan* p = new EtlPlan();
+    p->precision = precision;
+    p->tune = t;
+    p->sims.resize(n_sims);
+    int rc = ETL_OK;
+    for (int k = 0; k < n_sims && rc == ETL_OK; ++k) rc = init_sim(p, &descs[k], p->sims[k]);
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
@@ -605,192 +763,171 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
             rc = ETL_EHIP;
         }
     }
+    const size_t args_bytes =
+        size_t(n_sims) * (precision == ETL_F32 ? sizeof(FusedArgs<float>) : sizeof(FusedArgs<double>));
+    const size_t trace_bytes =
+        size_t(n_sims) * (precision == ETL_F32 ? sizeof(TraceSim<float>) : sizeof(TraceSim<double>));
     if (rc == ETL_OK) rc = etl_malloc((void**)&p->counter, 256);
-    // psi must be +0 wherever the fused kernel does not touch it.
-    if (rc == ETL_OK) {
-        int* flag = nullptr;
-        rc = etl_malloc((void**)&flag, 256);
-        if (rc == ETL_OK) rc = zero_async(flag, 4);
-        struct {
-            int field, rows, cols, lo_r, hi_r, lo_c, hi_c, er, ec;
-        } chk[4] = {
-            {3, nx, ny + 1, 0, nx, p->strips[6], p->strips[7], 0, 1},  // psi_ex_y
-            {4, nx + 1, ny, p->strips[2], p->strips[3], 0, ny, 1, 0},  // psi_ey_x
-            {5, nx, ny, p->strips[0], p->strips[1], 0, ny, 0, 0},      // psi_hz_x
-            {6, nx, ny, 0, nx, p->strips[4], p->strips[5], 0, 0},      // psi_hz_y
-        };
-        for (int k = 0; k < 4 && rc == ETL_OK; ++k) {
-            const auto& q = chk[k];
-            const int64_t n = int64_t(q.rows) * q.cols;
-            const dim3 grid(unsigned((n + 255) / 256));
-            if (f32)
-                hipLaunchKernelGGL(psi_check_kernel<float>, grid, dim3(256), 0, ctx().stream,
-                                   (const float*)desc->field[q.field][0], q.rows, q.cols, q.lo_r,
-                                   q.hi_r, q.lo_c, q.hi_c, q.er, q.ec, flag);
-            else
-                hipLaunchKernelGGL(psi_check_kernel<double>, grid, dim3(256), 0, ctx().stream,
-                                   (const double*)desc->field[q.field][0], q.rows, q.cols, q.lo_r,
-                                   q.hi_r, q.lo_c, q.hi_c, q.er, q.ec, flag);
-            if (hipGetLastError() != hipSuccess) rc = ETL_EHIP;
-        }
-        int host_flag = 0;
-        if (rc == ETL_OK) rc = etl_memcpy_d2h(&host_flag, flag, 4);
-        etl_free(flag);
-        if (rc == ETL_OK && host_flag) {
-            set_error("etl_plan_create: psi is non-zero outside the CPML strips; the fused "
-                      "path needs psi = +0 there (use the per-half-step primitives)");
-            rc = ETL_ESTATE;
-        }
-    }
+    if (rc == ETL_OK) rc = etl_malloc(&p->args_dev, args_bytes);
+    if (rc == ETL_OK) rc = etl_malloc(&p->trace_sims_dev, trace_bytes);
     if (rc != ETL_OK) {
-        etl_free(p->counter);
-        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-        if (p->ev_join) (void)hipEventDestroy(p->ev_join);
-        if (p->ev_join2) (void)hipEventDestroy(p->ev_join2);
-        if (p->s_strip) (void)hipStreamDestroy(p->s_strip);
-        if (p->s_extra) (void)hipStreamDestroy(p->s_extra);
-        delete p;
+        destroy_plan(p);
         return rc;
     }
     *out = p;
     return ETL_OK;
 }
 
+int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out) {
+    return etl_plan_create_batch(desc, 1, tune, out);
+}
+
 int etl_plan_destroy(etl_plan_t p) {
     if (p == nullptr) return ETL_OK;
     ETL_CHECK_INIT();
     ETL_HIP(hipStreamSynchronize(ctx().stream));
+    destroy_plan(p);
+    return ETL_OK;
+}
+
+int etl_plan_set_sources_sim(etl_plan_t p, int sim, int n_nodes, const int* comp, const int64_t* i,
+                             const int64_t* j) {
+    ETL_CHECK_INIT();
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr, "etl_plan_set_sources: no simulation %d", sim);
+    ETL_REQUIRE(n_nodes >= 0 && n_nodes <= kMaxSources,
+                "etl_plan_set_sources: %d source nodes (at most %d)", n_nodes, kMaxSources);
+    ETL_REQUIRE(n_nodes == 0 || (comp && i && j), "etl_plan_set_sources: null array");
+    std::vector<int4> src;
+    for (int k = 0; k < n_nodes; ++k) {
+        ETL_REQUIRE(comp[k] >= 0 && comp[k] <= 2, "etl_plan_set_sources: component %d", comp[k]);
+        ETL_REQUIRE(i[k] >= 0 && i[k] < s->rows_of(comp[k]) && j[k] >= 0 &&
+                        j[k] < s->pitch_of(comp[k]),
+                    "etl_plan_set_sources: node (%lld, %lld) outside component %d",
+                    (long long)i[k], (long long)j[k], comp[k]);
+        // Outer tangential E never changes in the update; a source there would be
+        // the only writer, which the fused step does not model.
+        const bool outer = (comp[k] == 0 && (j[k] == 0 || j[k] == s->ny)) ||
+                           (comp[k] == 1 && (i[k] == 0 || i[k] == s->nx));
+        ETL_REQUIRE(!outer, "etl_plan_set_sources: node on the outer tangential-E boundary");
+        src.push_back(make_int4(comp[k], int(i[k]), int(j[k]), 0));
+    }
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    etl_free(s->src_dev);
+    s->src_dev = nullptr;
+    if (n_nodes) {
+        int rc = upload((void**)&s->src_dev, src.data(), src.size() * sizeof(int4));
+        if (rc) return rc;
+    }
+    s->src = src;
+    s->n_src = n_nodes;
+    s->delta_rows = 0;  // a table of the old source set no longer applies
+    p->dirty = true;
+    p->args_stale = true;
     free_graphs(p);
-    free_layout(p);
-    etl_free(p->src_dev);
-    etl_free(p->delta);
-    etl_free(p->ph);
-    etl_free(p->counter);
-    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
-    if (p->ev_join2) (void)hipEventDestroy(p->ev_join2);
-    for (hipStream_t side : {p->s_strip, p->s_extra})
-        if (side) {
-            ETL_HIP(hipStreamSynchronize(side));
-            (void)hipStreamDestroy(side);
-        }
-    delete p;
     return ETL_OK;
 }
 
 int etl_plan_set_sources(etl_plan_t p, int n_nodes, const int* comp, const int64_t* i,
                          const int64_t* j) {
+    return etl_plan_set_sources_sim(p, 0, n_nodes, comp, i, j);
+}
+
+int etl_plan_source_table_sim(etl_plan_t p, int sim, int64_t n0, int rows, const void* delta) {
     ETL_CHECK_INIT();
-    ETL_REQUIRE(p != nullptr, "etl_plan_set_sources: null plan");
-    ETL_REQUIRE(n_nodes >= 0 && n_nodes <= kMaxSources,
-                "etl_plan_set_sources: %d source nodes (at most %d)", n_nodes, kMaxSources);
-    ETL_REQUIRE(n_nodes == 0 || (comp && i && j), "etl_plan_set_sources: null array");
-    std::vector<int4> src;
-    for (int s = 0; s < n_nodes; ++s) {
-        ETL_REQUIRE(comp[s] >= 0 && comp[s] <= 2, "etl_plan_set_sources: component %d", comp[s]);
-        ETL_REQUIRE(i[s] >= 0 && i[s] < p->rows_of(comp[s]) && j[s] >= 0 &&
-                        j[s] < p->pitch_of(comp[s]),
-                    "etl_plan_set_sources: node (%lld, %lld) outside component %d",
-                    (long long)i[s], (long long)j[s], comp[s]);
-        // Outer tangential E never changes in the update; a source there would be
-        // the only writer, which the fused step does not model.
-        const bool outer = (comp[s] == 0 && (j[s] == 0 || j[s] == p->ny)) ||
-                           (comp[s] == 1 && (i[s] == 0 || i[s] == p->nx));
-        ETL_REQUIRE(!outer, "etl_plan_set_sources: node on the outer tangential-E boundary");
-        src.push_back(make_int4(comp[s], int(i[s]), int(j[s]), 0));
-    }
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
-    etl_free(p->src_dev);
-    p->src_dev = nullptr;
-    if (n_nodes) {
-        int rc = upload((void**)&p->src_dev, src.data(), src.size() * sizeof(int4));
-        if (rc) return rc;
-    }
-    p->src = src;
-    p->n_src = n_nodes;
-    p->dirty = true;
-    free_graphs(p);
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr, "etl_plan_source_table: no simulation %d", sim);
+    ETL_REQUIRE(rows >= 0 && n0 >= 0, "etl_plan_source_table: bad rows");
+    if (s->n_src == 0) return ETL_OK;
+    ETL_REQUIRE(delta != nullptr, "etl_plan_source_table: null table");
+    const size_t bytes = size_t(rows) * s->n_src * p->elem();
+    const unsigned char* b = static_cast<const unsigned char*>(delta);
+    s->delta_host.assign(b, b + bytes);
+    s->delta_n0 = n0;
+    s->delta_rows = rows;
+    s->delta_stale = true;
     return ETL_OK;
 }
 
 int etl_plan_source_table(etl_plan_t p, int64_t n0, int rows, const void* delta) {
-    ETL_CHECK_INIT();
-    ETL_REQUIRE(p != nullptr, "etl_plan_source_table: null plan");
-    ETL_REQUIRE(rows >= 0 && n0 >= 0, "etl_plan_source_table: bad rows");
-    if (p->n_src == 0) return ETL_OK;
-    ETL_REQUIRE(delta != nullptr, "etl_plan_source_table: null table");
-    const size_t bytes = size_t(rows) * p->n_src * p->elem();
-    const unsigned char* b = static_cast<const unsigned char*>(delta);
-    p->delta_host.assign(b, b + bytes);
-    p->delta_n0 = n0;
-    p->delta_rows = rows;
-    p->delta_stale = true;
-    return ETL_OK;
+    return etl_plan_source_table_sim(p, 0, n0, rows, delta);
 }
 
-int etl_plan_add_monitor(etl_plan_t p, int comp, int n_nodes, const int64_t* flat_index, int n_f,
-                         void* acc, int* id) {
+int etl_plan_add_monitor_sim(etl_plan_t p, int sim, int comp, int n_nodes,
+                             const int64_t* flat_index, int n_f, void* acc, int* id) {
     ETL_CHECK_INIT();
-    ETL_REQUIRE(p && id, "etl_plan_add_monitor: null argument");
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr && id, "etl_plan_add_monitor: no simulation %d or null id", sim);
     ETL_REQUIRE(comp >= 0 && comp <= 2, "etl_plan_add_monitor: component %d", comp);
     ETL_REQUIRE(n_nodes >= 1 && n_f >= 1 && flat_index && acc,
                 "etl_plan_add_monitor: bad monitor (%d nodes, %d frequencies)", n_nodes, n_f);
-    const int64_t size = int64_t(p->rows_of(comp)) * p->pitch_of(comp);
+    const int64_t size = int64_t(s->rows_of(comp)) * s->pitch_of(comp);
     EtlPlan::Mon m;
     m.comp = comp;
     m.n_nodes = n_nodes;
     m.n_f = n_f;
     m.acc = acc;
     m.flat.assign(flat_index, flat_index + n_nodes);
-    const int pitch = p->pitch_of(comp);
+    const int pitch = s->pitch_of(comp);
     for (int64_t f : m.flat) {
         ETL_REQUIRE(f >= 0 && f < size, "etl_plan_add_monitor: index %lld outside component %d",
                     (long long)f, comp);
         const int64_t row = f / pitch;
-        ETL_REQUIRE(!(comp == 1 && row == p->nx),
+        ETL_REQUIRE(!(comp == 1 && row == s->nx),
                     "etl_plan_add_monitor: node on the outer tangential-E boundary");
     }
-    m.slot_base = m.q_base = 0;
     ETL_HIP(hipStreamSynchronize(ctx().stream));
-    p->mons.push_back(m);
+    s->mons.push_back(m);
     p->dirty = true;
-    *id = int(p->mons.size()) - 1;
-    // The combined phasor table changes shape: drop it.
-    etl_free(p->ph);
-    p->ph = nullptr;
-    p->ph_rows_cap = 0;
-    p->ph_host.clear();
-    p->ph_host_rows = 0;
-    p->ph_stale = false;
-    for (auto& mm : p->mons) mm.rows = -1;
+    *id = int(s->mons.size()) - 1;
+    // This simulation's combined phasor table changes shape: drop it.
+    etl_free(s->ph);
+    s->ph = nullptr;
+    s->ph_rows_cap = 0;
+    s->ph_host.clear();
+    s->ph_host_rows = 0;
+    s->ph_stale = false;
+    for (auto& mm : s->mons) mm.rows = -1;
+    p->args_stale = true;
     free_graphs(p);
     return ETL_OK;
 }
 
-int etl_plan_phasor_table(etl_plan_t p, int id, int64_t n0, int rows, const void* ph) {
+int etl_plan_add_monitor(etl_plan_t p, int comp, int n_nodes, const int64_t* flat_index, int n_f,
+                         void* acc, int* id) {
+    return etl_plan_add_monitor_sim(p, 0, comp, n_nodes, flat_index, n_f, acc, id);
+}
+
+int etl_plan_phasor_table_sim(etl_plan_t p, int sim, int id, int64_t n0, int rows,
+                              const void* ph) {
     ETL_CHECK_INIT();
-    ETL_REQUIRE(p != nullptr, "etl_plan_phasor_table: null plan");
-    ETL_REQUIRE(id >= 0 && id < int(p->mons.size()), "etl_plan_phasor_table: no monitor %d", id);
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr, "etl_plan_phasor_table: no simulation %d", sim);
+    ETL_REQUIRE(id >= 0 && id < int(s->mons.size()), "etl_plan_phasor_table: no monitor %d", id);
     ETL_REQUIRE(rows >= 0 && n0 >= 0 && (ph || rows == 0), "etl_plan_phasor_table: bad table");
     int rc;
     if (p->dirty) {
-        if ((rc = build_layout(p))) return rc;
+        if ((rc = build_layout(p))) return rc;  // assigns the monitors' phasor columns
     }
     const size_t cplx = size_t(2) * p->elem();
-    const size_t row_bytes = size_t(p->nq) * cplx;
-    if (rows > p->ph_host_rows) {  // rows are appended: the other monitors' columns stay
-        p->ph_host.resize(size_t(rows) * row_bytes);
-        p->ph_host_rows = rows;
+    const size_t row_bytes = size_t(s->nq) * cplx;
+    if (rows > s->ph_host_rows) {  // rows are appended: the other monitors' columns stay
+        s->ph_host.resize(size_t(rows) * row_bytes);
+        s->ph_host_rows = rows;
     }
-    auto& m = p->mons[id];
+    auto& m = s->mons[id];
     const unsigned char* src = static_cast<const unsigned char*>(ph);
     for (int r = 0; r < rows; ++r)
-        std::memcpy(p->ph_host.data() + size_t(r) * row_bytes + size_t(m.q_base) * cplx,
+        std::memcpy(s->ph_host.data() + size_t(r) * row_bytes + size_t(m.q_base) * cplx,
                     src + size_t(r) * m.n_f * cplx, size_t(m.n_f) * cplx);
     m.n0 = n0;
     m.rows = rows;
-    p->ph_stale = true;
+    s->ph_stale = true;
     return ETL_OK;
+}
+
+int etl_plan_phasor_table(etl_plan_t p, int id, int64_t n0, int rows, const void* ph) {
+    return etl_plan_phasor_table_sim(p, 0, id, n0, rows, ph);
 }
 
 int etl_plan_tune(etl_plan_t p, EtlTune* out) {
@@ -842,7 +979,7 @@ int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, EtlTiming* out, int* p
         out->launch_ms = all / L;
         out->interior_ms = p->n_interior ? interior / L : 0.0;
         out->total_ms = tot;
-        out->interior_cells = int64_t(p->n_interior) * p->tune.band_rows * p->wo();
+        out->interior_cells = p->interior_cells;
         out->launches = launches;
         out->depth = p->tune.temporal;
         if (parity) *parity = p->parity;

@@ -42,18 +42,34 @@ namespace etl {
 template <typename R, int V>
 using Vec = R __attribute__((ext_vector_type(V), aligned(sizeof(R))));
 
+// Pointers the kernel reads from the argument array are generic; casting them to
+// the global address space lets the compiler issue global (not flat) loads and
+// stores, which do not tie up the LDS counter.
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ global_t<T>* to_global(T* p) {
+    return (global_t<T>*)p;
+}
+// int4 descriptors read through global pointers (HIP's int4 class has no
+// address-space-qualified copy constructor; the builtin vector type needs none).
+using int4v = int __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ const global_t<const int4v>* to_global_v(const int4* p) {
+    return (const global_t<const int4v>*)p;
+}
+
 // dst[k] = p[row * pitch + min(c, limit - 1) + k]: always one full vector load.
 // Arrays come from etl_malloc, which keeps kGuard valid bytes before and after each
 // one, and c >= -T, so the overhang past a row (or before the first) stays in
 // memory; elements outside the row are garbage the caller never uses.
 template <typename R, int V>
-__device__ __forceinline__ void ldv(R (&dst)[V], const R* __restrict__ base, int row, int pitch,
-                                    int c, int limit) {
-    const R* p = base + int64_t(row) * pitch + min(c, limit - 1);
+__device__ __forceinline__ void ldv(R (&dst)[V], const global_t<R>* __restrict__ base, int row,
+                                    int pitch, int c, int limit) {
+    const global_t<R>* p = base + int64_t(row) * pitch + min(c, limit - 1);
     if constexpr (V == 1) {
         dst[0] = p[0];
     } else {
-        const Vec<R, V> x = *reinterpret_cast<const Vec<R, V>*>(p);
+        const Vec<R, V> x = *reinterpret_cast<const global_t<Vec<R, V>>*>(p);
 #pragma unroll
         for (int k = 0; k < V; ++k) dst[k] = x[k];
     }
@@ -61,8 +77,8 @@ __device__ __forceinline__ void ldv(R (&dst)[V], const R* __restrict__ base, int
 
 // p[c + k] = src[k] for the elements with lo <= c + k < hi.
 template <typename R, int V>
-__device__ __forceinline__ void store_row(R* __restrict__ p, const R (&src)[V], int c, int lo,
-                                          int hi) {
+__device__ __forceinline__ void store_row(global_t<R>* __restrict__ p, const R (&src)[V], int c,
+                                          int lo, int hi) {
     if (c >= lo && c + V <= hi) {
         if constexpr (V == 1) {
             p[c] = src[0];
@@ -70,7 +86,7 @@ __device__ __forceinline__ void store_row(R* __restrict__ p, const R (&src)[V], 
             Vec<R, V> x;
 #pragma unroll
             for (int k = 0; k < V; ++k) x[k] = src[k];
-            *reinterpret_cast<Vec<R, V>*>(p + c) = x;
+            *reinterpret_cast<global_t<Vec<R, V>>*>(p + c) = x;
         }
     } else {
 #pragma unroll
@@ -80,7 +96,7 @@ __device__ __forceinline__ void store_row(R* __restrict__ p, const R (&src)[V], 
 }
 
 template <typename R, int V>
-__device__ __forceinline__ void store_masked(R* __restrict__ p, const R (&src)[V], int c,
+__device__ __forceinline__ void store_masked(global_t<R>* __restrict__ p, const R (&src)[V], int c,
                                              const bool (&m)[V]) {
 #pragma unroll
     for (int k = 0; k < V; ++k)
@@ -154,57 +170,75 @@ __device__ __forceinline__ void inject_row(const TileSources<R, T>& src, int t, 
 // Monitored values of one component's owned rows into the trace row: the rows go
 // to LDS, then the lanes copy the tile's entries (row, column, slot).
 template <typename R, int V, int RW, int T, int BR>
-__device__ __forceinline__ void trace_tile(const int4* __restrict__ ent, int e0, int e1,
-                                           R* __restrict__ lds, const R (&f)[RW][V], int i0,
-                                           int cs, R* __restrict__ trow) {
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void trace_tile(const global_t<const int4v>* __restrict__ ent, int e0,
+                                           int e1, R* __restrict__ lds, const R (&f)[RW][V],
+                                           int i0, int cs, global_t<R>* __restrict__ trow) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int l = 0; l < BR; ++l)
 #pragma unroll
         for (int k = 0; k < V; ++k) lds[(l * 64 + lane) * V + k] = f[T + l][k];
     __syncthreads();
     for (int q = e0 + lane; q < e1; q += 64) {
-        const int4 en = ent[q];  // row, column, slot
+        const int4v en = ent[q];  // row, column, slot
         trow[en.z] = lds[(en.x - i0) * 64 * V + (en.y - cs)];
     }
     __syncthreads();
 }
 
+// One wave per tile, `blockDim.x / 64` waves per workgroup (always one for EXTRA,
+// which synchronises the workgroup).  Workgroups are dealt to the XCDs round-robin,
+// so consecutive workgroup indices are spread XCD-major: each XCD receives a
+// contiguous range of tiles, whose neighbours share its L2.
 template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
-__global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
-                                                  const int4* __restrict__ tiles, int n_tiles,
-                                                  const int* __restrict__ counter, int kstep0,
-                                                  int parity) {
+__global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restrict__ args,
+                                                   const int4* __restrict__ tiles, int n_tiles,
+                                                   const int* __restrict__ counter, int kstep0,
+                                                   int parity) {
     constexpr int W = 64 * V;
     constexpr int RW = BR + 2 * T;  // window rows (Ey: one more)
+    const int waves = blockDim.x >> 6;
+    const int n_groups = (n_tiles + waves - 1) / waves;
     const int b = blockIdx.x;
-    const int per_xcd = (n_tiles + 7) >> 3;
-    const int li = (b & 7) * per_xcd + (b >> 3);
+    const int per_xcd = (n_groups + 7) >> 3;
+    // the wave index is uniform across the wave: readfirstlane puts it (and all that
+    // derives from it — the descriptor, the simulation's arguments) in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int li = ((b & 7) * per_xcd + (b >> 3)) * waves + wave;
     if (li >= n_tiles) return;
-    const int4 td = tiles[li];  // first row, tile column, monitor cell, 0
+    const int4 td = tiles[li];  // first row, tile column, extra-tile index, simulation
+    // The simulation's arguments are read once, here, into registers: after the
+    // kernel's first store the compiler could no longer assume them unchanged.
+    const FusedArgs<R>& a = args[td.w];
+    const int xh_lo = a.xh_lo, xh_hi = a.xh_hi, xe_lo = a.xe_lo, xe_hi = a.xe_hi;
+    const int yh_lo = a.yh_lo, yh_hi = a.yh_hi, ye_lo = a.ye_lo, ye_hi = a.ye_hi;
+    const int wo = a.wo;
     const int tj = td.y;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int nx = a.nx, ny = a.ny, px = ny + 1;
-    const int o0 = tj * a.wo;  // first output column
+    const int o0 = tj * wo;  // first output column
     const int cs = o0 - T;     // first window column
     const int c = cs + lane * V;
     const int i0 = td.x;
     const int g0 = i0 - T;  // global row of window row 0
 
-    const R* __restrict__ exA = a.f[0][parity];
-    const R* __restrict__ eyA = a.f[1][parity];
-    const R* __restrict__ hzA = a.f[2][parity];
-    const R* __restrict__ pexA = a.f[3][parity];  // psi_ex_y
-    const R* __restrict__ peyA = a.f[4][parity];  // psi_ey_x
-    const R* __restrict__ phxA = a.f[5][parity];  // psi_hz_x
-    const R* __restrict__ phyA = a.f[6][parity];  // psi_hz_y
-    R* __restrict__ exB = const_cast<R*>(a.f[0][parity ^ 1]);
-    R* __restrict__ eyB = const_cast<R*>(a.f[1][parity ^ 1]);
-    R* __restrict__ hzB = const_cast<R*>(a.f[2][parity ^ 1]);
-    R* __restrict__ pexB = const_cast<R*>(a.f[3][parity ^ 1]);
-    R* __restrict__ peyB = const_cast<R*>(a.f[4][parity ^ 1]);
-    R* __restrict__ phxB = const_cast<R*>(a.f[5][parity ^ 1]);
-    R* __restrict__ phyB = const_cast<R*>(a.f[6][parity ^ 1]);
+    using G = global_t<R>;
+    const G* __restrict__ exA = to_global(a.f[0][parity]);
+    const G* __restrict__ eyA = to_global(a.f[1][parity]);
+    const G* __restrict__ hzA = to_global(a.f[2][parity]);
+    const G* __restrict__ pexA = to_global(a.f[3][parity]);  // psi_ex_y
+    const G* __restrict__ peyA = to_global(a.f[4][parity]);  // psi_ey_x
+    const G* __restrict__ phxA = to_global(a.f[5][parity]);  // psi_hz_x
+    const G* __restrict__ phyA = to_global(a.f[6][parity]);  // psi_hz_y
+    G* __restrict__ exB = to_global(const_cast<R*>(a.f[0][parity ^ 1]));
+    G* __restrict__ eyB = to_global(const_cast<R*>(a.f[1][parity ^ 1]));
+    G* __restrict__ hzB = to_global(const_cast<R*>(a.f[2][parity ^ 1]));
+    G* __restrict__ pexB = to_global(const_cast<R*>(a.f[3][parity ^ 1]));
+    G* __restrict__ peyB = to_global(const_cast<R*>(a.f[4][parity ^ 1]));
+    G* __restrict__ phxB = to_global(const_cast<R*>(a.f[5][parity ^ 1]));
+    G* __restrict__ phyB = to_global(const_cast<R*>(a.f[6][parity ^ 1]));
+    const G* __restrict__ cbx = to_global(a.cb_ex);
+    const G* __restrict__ cby = to_global(a.cb_ey);
 
     // ---- one burst of loads: the whole window ----
     constexpr int PR = STRIP ? RW : 1;  // psi rows held (none in interior tiles)
@@ -218,8 +252,8 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
         ldv<R, V>(ex[l], exA, gx, px, c, px);
         ldv<R, V>(hz[l], hzA, gx, ny, c, ny);
         ldv<R, V>(ey[l], eyA, ge, ny, c, ny);
-        ldv<R, V>(cx[l], a.cb_ex, gx, px, c, px);
-        ldv<R, V>(cy[l], a.cb_ey, ge, ny, c, ny);
+        ldv<R, V>(cx[l], cbx, gx, px, c, px);
+        ldv<R, V>(cy[l], cby, ge, ny, c, ny);
         if constexpr (STRIP) {
             ldv<R, V>(phy[l], phyA, gx, ny, c, ny);
             ldv<R, V>(phx[l], phxA, gx, ny, c, ny);
@@ -239,14 +273,14 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
     for (int k = 0; k < V; ++k) {
         const int j = c + k;
         mH[k] = (j >= 0 && j < ny);
-        ysh[k] = STRIP && mH[k] && (j < a.yh_lo || j >= a.yh_hi);
-        yse[k] = STRIP && j >= 1 && j <= ny - 1 && (j < a.ye_lo || j >= a.ye_hi);
-        byh[k] = ysh[k] ? a.by_h[j] : R(1);
-        cyh[k] = ysh[k] ? a.cy_h[j] : R(0);
-        ikyh[k] = ysh[k] ? a.iky_h[j] : R(1);
-        bye[k] = yse[k] ? a.by_e[j] : R(1);
-        cye[k] = yse[k] ? a.cy_e[j] : R(0);
-        ikye[k] = yse[k] ? a.iky_e[j] : R(1);
+        ysh[k] = STRIP && mH[k] && (j < yh_lo || j >= yh_hi);
+        yse[k] = STRIP && j >= 1 && j <= ny - 1 && (j < ye_lo || j >= ye_hi);
+        byh[k] = ysh[k] ? to_global(a.by_h)[j] : R(1);
+        cyh[k] = ysh[k] ? to_global(a.cy_h)[j] : R(0);
+        ikyh[k] = ysh[k] ? to_global(a.iky_h)[j] : R(1);
+        bye[k] = yse[k] ? to_global(a.by_e)[j] : R(1);
+        cye[k] = yse[k] ? to_global(a.cy_e)[j] : R(0);
+        ikye[k] = yse[k] ? to_global(a.iky_e)[j] : R(1);
     }
 
     // Row profiles of the x strips: lane l holds those of window row l (RW <= 64),
@@ -255,38 +289,46 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
     if constexpr (STRIP) {
         const int gl = g0 + min(lane, RW - 1);
         const int gh = min(max(gl, 0), nx - 1), ge = min(max(gl, 0), nx);
-        pbxh = a.bx_h[gh];
-        pcxh = a.cx_h[gh];
-        pikxh = a.ikx_h[gh];
-        pbxe = a.bx_e[ge];
-        pcxe = a.cx_e[ge];
-        pikxe = a.ikx_e[ge];
+        pbxh = to_global(a.bx_h)[gh];
+        pcxh = to_global(a.cx_h)[gh];
+        pikxh = to_global(a.ikx_h)[gh];
+        pbxe = to_global(a.bx_e)[ge];
+        pcxe = to_global(a.cx_e)[ge];
+        pikxe = to_global(a.ikx_e)[ge];
     }
 
     // Sources and monitor entries of an extra tile.
     TileSources<R, T> src;
     int mon_lo[3] = {0, 0, 0}, mon_hi[3] = {0, 0, 0};
+    G* trace_base = nullptr;
+    int n_slots = 0;
+    const global_t<const int4v>* ent[3] = {nullptr, nullptr, nullptr};
     if constexpr (EXTRA) {
         const int nbase = *counter + kstep0;  // table row of this launch's first step
         src.n = a.n_src;
         if (lane < a.n_src) {
-            const int4 sd = a.src[lane];
+            const int4v sd = to_global_v(a.src)[lane];
             src.comp = sd.x;
             src.row = sd.y;
             src.col = sd.z;
 #pragma unroll
-            for (int t = 0; t < T; ++t) src.delta[t] = a.delta[int64_t(nbase + t) * a.n_src + lane];
+            for (int t = 0; t < T; ++t)
+                src.delta[t] = to_global(a.delta)[int64_t(nbase + t) * a.n_src + lane];
         }
         for (int s = 0; s < src.n; ++s) {
             const int l = __builtin_amdgcn_readlane(src.row, s) - g0;
             if (l >= 0 && l < RW) src.rows[__builtin_amdgcn_readlane(src.comp, s)] |= 1ull << l;
         }
         if (a.has_mon) {
+            trace_base = to_global(a.trace);
+            n_slots = a.n_slots;
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) ent[comp] = to_global_v(a.tile_ent[comp]);
 #pragma unroll
             for (int comp = 0; comp < 3; ++comp) {
                 if (a.tile_mon[comp] == nullptr) continue;
-                mon_lo[comp] = a.tile_mon[comp][td.z];
-                mon_hi[comp] = a.tile_mon[comp][td.z + 1];
+                mon_lo[comp] = to_global(a.tile_mon[comp])[td.z];
+                mon_hi[comp] = to_global(a.tile_mon[comp])[td.z + 1];
             }
         }
     }
@@ -301,7 +343,7 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
             bool xh = false;
             R bx = R(1), cxp = R(0), ikx = R(1);
             if constexpr (STRIP) {
-                xh = g < a.xh_lo || g >= a.xh_hi;
+                xh = g < xh_lo || g >= xh_hi;
                 if (xh) {
                     bx = lane_value(pbxh, l);
                     cxp = lane_value(pcxh, l);
@@ -346,7 +388,7 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
             bool xe = false;
             R bxe = R(1), cxe = R(0), ikxe = R(1);
             if constexpr (STRIP) {
-                xe = ey_row && (g < a.xe_lo || g >= a.xe_hi);
+                xe = ey_row && (g < xe_lo || g >= xe_hi);
                 if (xe) {
                     bxe = lane_value(pbxe, l);
                     cxe = lane_value(pcxe, l);
@@ -391,21 +433,19 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
             }
         }
         if constexpr (EXTRA) {
-            R* trow = a.trace + int64_t(kstep0 + t) * a.n_slots;
+            G* trow = trace_base + int64_t(kstep0 + t) * n_slots;
             if (mon_lo[2] < mon_hi[2])
-                trace_tile<R, V, RW, T, BR>(a.tile_ent[2], mon_lo[2], mon_hi[2], lds, hz, i0, cs,
-                                            trow);
+                trace_tile<R, V, RW, T, BR>(ent[2], mon_lo[2], mon_hi[2], lds, hz, i0, cs, trow);
             if (mon_lo[0] < mon_hi[0])
-                trace_tile<R, V, RW, T, BR>(a.tile_ent[0], mon_lo[0], mon_hi[0], lds, ex, i0, cs,
-                                            trow);
+                trace_tile<R, V, RW, T, BR>(ent[0], mon_lo[0], mon_hi[0], lds, ex, i0, cs, trow);
             if (mon_lo[1] < mon_hi[1])
-                trace_tile<R, V, RW + 1, T, BR>(a.tile_ent[1], mon_lo[1], mon_hi[1], lds, ey, i0,
-                                                cs, trow);
+                trace_tile<R, V, RW + 1, T, BR>(ent[1], mon_lo[1], mon_hi[1], lds, ey, i0, cs,
+                                                trow);
         }
     }
 
     // ---- store the owned rows [i0, i0 + BR) and columns [o0, o0 + WO) ----
-    const int lo = max(o0, 0), hi = o0 + a.wo;
+    const int lo = max(o0, 0), hi = o0 + wo;
 #pragma unroll
     for (int l = T; l < T + BR; ++l) {
         const int g = g0 + l;
@@ -424,9 +464,9 @@ __global__ __launch_bounds__(64) void tile_kernel(const FusedArgs<R> a,
             }
             store_masked<R, V>(phyB + int64_t(g) * ny, phy[STRIP ? l : 0], c, moy);
             store_masked<R, V>(pexB + int64_t(g) * px, pey[STRIP ? l : 0], c, moe);
-            if (g < a.xh_lo || g >= a.xh_hi)
+            if (g < xh_lo || g >= xh_hi)
                 store_masked<R, V>(phxB + int64_t(g) * ny, phx[STRIP ? l : 0], c, mo);
-            if (g >= 1 && (g < a.xe_lo || g >= a.xe_hi))
+            if (g >= 1 && (g < xe_lo || g >= xe_hi))
                 store_masked<R, V>(peyB + int64_t(g) * ny, pex[STRIP ? l : 0], c, mo);
         }
     }

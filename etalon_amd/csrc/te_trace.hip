@@ -17,20 +17,22 @@ namespace {
 // first row; the last block to finish advances it by `rows` (every block has read it
 // by then) and clears the block tally counter[1] for the next chunk.
 template <typename R>
-__global__ __launch_bounds__(256) void trace_dft_kernel(const R* __restrict__ trace, int n_slots,
-                                                        int rows, const R* __restrict__ ph, int nq,
-                                                        int* __restrict__ counter,
+__global__ __launch_bounds__(256) void trace_dft_kernel(const TraceSim<R>* __restrict__ sims,
+                                                        int rows, int* __restrict__ counter,
                                                         const int4* __restrict__ pairs,
+                                                        const int* __restrict__ pair_sim,
                                                         int n_pairs, R* const* __restrict__ accs) {
     constexpr int B = sizeof(R) == 4 ? 32 : 16;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int base = __atomic_load_n(counter, __ATOMIC_RELAXED);
     if (p < n_pairs) {
-        const int4 pr = pairs[p];  // slot, q, monitor, complex element index
+        const int4 pr = pairs[p];  // slot, q, accumulator, complex element index
+        const TraceSim<R> sim = sims[pair_sim[p]];
+        const int n_slots = sim.n_slots, nq = sim.nq;
         R* acc = accs[pr.z] + 2 * int64_t(pr.w);
         R re = acc[0], im = acc[1];
-        const R* __restrict__ php = ph + 2 * (int64_t(base) * nq + pr.y);
-        const R* __restrict__ tp = trace + pr.x;
+        const R* __restrict__ php = sim.ph + 2 * (int64_t(base) * nq + pr.y);
+        const R* __restrict__ tp = sim.trace + pr.x;
         int r = 0;
         for (; r + B <= rows; r += B) {
             R v[B], pre[B], pim[B];
@@ -71,12 +73,11 @@ __global__ void bump_counter_kernel(int* counter, int by) {
 }  // namespace
 
 template <typename R>
-int launch_trace_dft(const R* trace, int n_slots, int rows, const R* ph, int nq, int* counter,
-                     const int4* pairs, int n_pairs, R* const* accs,
-                     hipStream_t stream) {
+int launch_trace_dft(const TraceSim<R>* sims, int rows, int* counter, const int4* pairs,
+                     const int* pair_sim, int n_pairs, R* const* accs, hipStream_t stream) {
     if (n_pairs == 0 || rows == 0) return ETL_OK;
     hipLaunchKernelGGL(trace_dft_kernel<R>, dim3((n_pairs + 255) / 256), dim3(256), 0, stream,
-                       trace, n_slots, rows, ph, nq, counter, pairs, n_pairs, accs);
+                       sims, rows, counter, pairs, pair_sim, n_pairs, accs);
     ETL_LAUNCHED();
     return ETL_OK;
 }
@@ -87,9 +88,9 @@ int launch_bump(int* counter, int by, hipStream_t stream) {
     return ETL_OK;
 }
 
-template int launch_trace_dft<float>(const float*, int, int, const float*, int, int*,
-                                     const int4*, int, float* const*, hipStream_t);
-template int launch_trace_dft<double>(const double*, int, int, const double*, int, int*,
-                                      const int4*, int, double* const*, hipStream_t);
+template int launch_trace_dft<float>(const TraceSim<float>*, int, int*, const int4*, const int*,
+                                     int, float* const*, hipStream_t);
+template int launch_trace_dft<double>(const TraceSim<double>*, int, int*, const int4*,
+                                      const int*, int, double* const*, hipStream_t);
 
 }  // namespace etl

@@ -47,6 +47,7 @@ class EtlTune(C.Structure):
         ("use_graph", C.c_int),
         ("trace_chunk", C.c_int),
         ("temporal", C.c_int),
+        ("waves", C.c_int),
     ]
 
 
@@ -87,6 +88,12 @@ _SIGNATURES = {
     "etl_plan_source_table": [vp, C.c_int64, C.c_int, vp],
     "etl_plan_add_monitor": [vp, C.c_int, C.c_int, i64p, C.c_int, vp, C.POINTER(C.c_int)],
     "etl_plan_phasor_table": [vp, C.c_int, C.c_int64, C.c_int, vp],
+    "etl_plan_create_batch": [C.POINTER(EtlTeDesc), C.c_int, C.POINTER(EtlTune), C.POINTER(vp)],
+    "etl_plan_set_sources_sim": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), i64p, i64p],
+    "etl_plan_source_table_sim": [vp, C.c_int, C.c_int64, C.c_int, vp],
+    "etl_plan_add_monitor_sim": [vp, C.c_int, C.c_int, C.c_int, i64p, C.c_int, vp,
+                                 C.POINTER(C.c_int)],
+    "etl_plan_phasor_table_sim": [vp, C.c_int, C.c_int, C.c_int64, C.c_int, vp],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],
     "etl_plan_advance": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)],
     "etl_plan_time": [vp, C.c_int64, C.c_int64, C.POINTER(EtlTiming), C.POINTER(C.c_int)],

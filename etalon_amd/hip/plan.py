@@ -1,15 +1,16 @@
 """Fused time-stepping plan: Python side of etl_plan_* (include/etalon_hip.h).
 
-A plan owns nothing but device-side bookkeeping; the ping-pong buffer pairs are
-DeviceArrays held here.  After ``advance`` the state lives in buffer
-``parity`` of each pair.
+A plan advances one simulation, or a batch of independent simulations of any
+sizes, with the fused kernels.  It owns nothing but device-side bookkeeping; the
+ping-pong buffer pairs are DeviceArrays held here.  After ``advance`` the state
+lives in buffer ``parity`` of each pair (the same parity for every simulation).
 """
 
 from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
-from typing import Optional
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -18,54 +19,64 @@ from etalon_amd.hip.backend import DeviceArray, _precision_of, zeros
 from etalon_amd.state import FIELDS, PROFILES
 
 COMPONENT_CODE = {"Ex": 0, "Ey": 1, "Hz": 2}
-MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h
+MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h, per simulation
 
 
 @dataclass
 class Tune:
     """Fused-kernel shape; 0 = the library default (vec 1, 8-row tiles, temporal
-    depth 4, 64-step trace chunks)."""
+    depth 4, 64-step trace chunks, one wave per workgroup)."""
 
     band_rows: int = 0
     vec: int = 0
     use_graph: int = 1
     trace_chunk: int = 0
     temporal: int = 0
+    waves: int = 0
 
     def as_struct(self) -> _lib.EtlTune:
         return _lib.EtlTune(self.band_rows, self.vec, self.use_graph, self.trace_chunk,
-                            self.temporal)
+                            self.temporal, self.waves)
+
+
+def _describe(state, pair: dict) -> _lib.EtlTeDesc:
+    d = _lib.EtlTeDesc()
+    d.precision = _precision_of(state.hz.dtype)
+    d.nx, d.ny = int(state.nx), int(state.ny)
+    d.db, d.inv_dx, d.inv_dy = float(state.db), float(state.inv_dx), float(state.inv_dy)
+    for k, name in enumerate(FIELDS):
+        d.field[k][0] = pair[name][0].ptr
+        d.field[k][1] = pair[name][1].ptr
+    d.cb_ex = state.cb_ex.ptr
+    d.cb_ey = state.cb_ey.ptr
+    for name in PROFILES:
+        setattr(d, name, getattr(state, name).ptr)
+    return d
 
 
 class FusedPlan:
-    """Fused kernel plan over a TEState whose arrays are DeviceArrays."""
+    """Fused kernel plan over one TEState or a sequence of them (DeviceArrays)."""
 
-    def __init__(self, state, tune: Optional[Tune] = None):
-        self.precision = _precision_of(state.hz.dtype)
-        self.state = state
-        self.pairs = {name: [getattr(state, name), zeros(getattr(state, name).shape,
-                                                         getattr(state, name).dtype)]
-                      for name in FIELDS}
-        d = _lib.EtlTeDesc()
-        d.precision = self.precision
-        d.nx, d.ny = int(state.nx), int(state.ny)
-        d.db, d.inv_dx, d.inv_dy = float(state.db), float(state.inv_dx), float(state.inv_dy)
-        for k, name in enumerate(FIELDS):
-            d.field[k][0] = self.pairs[name][0].ptr
-            d.field[k][1] = self.pairs[name][1].ptr
-        d.cb_ex = state.cb_ex.ptr
-        d.cb_ey = state.cb_ey.ptr
-        for name in PROFILES:
-            setattr(d, name, getattr(state, name).ptr)
-        self._desc = d
+    def __init__(self, states, tune: Optional[Tune] = None):
+        self.batch = isinstance(states, (list, tuple))
+        self.states: Sequence = list(states) if self.batch else [states]
+        self.state = self.states[0]
+        self.precision = _precision_of(self.state.hz.dtype)
+        self.pair_sets = [{name: [getattr(s, name), zeros(getattr(s, name).shape,
+                                                          getattr(s, name).dtype)]
+                           for name in FIELDS} for s in self.states]
+        self.pairs = self.pair_sets[0]
+        descs = (_lib.EtlTeDesc * len(self.states))()
+        for k, (s, pair) in enumerate(zip(self.states, self.pair_sets)):
+            descs[k] = _describe(s, pair)
+        self._descs = descs
         handle = C.c_void_p()
         t = tune.as_struct() if tune is not None else None
-        _lib.call("etl_plan_create", C.byref(d), C.byref(t) if t is not None else None,
-                  C.byref(handle))
+        _lib.call("etl_plan_create_batch", descs, len(self.states),
+                  C.byref(t) if t is not None else None, C.byref(handle))
         self.handle = handle
         self.parity = 0
-        self.n_src = 0
-        self._keep: list = []
+        self.n_src = [0] * len(self.states)
 
     def close(self) -> None:
         if self.handle:
@@ -78,32 +89,34 @@ class FusedPlan:
         except Exception:
             pass
 
-    def set_sources(self, comp: np.ndarray, i: np.ndarray, j: np.ndarray) -> None:
+    def set_sources(self, comp: np.ndarray, i: np.ndarray, j: np.ndarray, sim: int = 0) -> None:
         comp = np.ascontiguousarray(comp, dtype=np.int32)
         i = np.ascontiguousarray(i, dtype=np.int64)
         j = np.ascontiguousarray(j, dtype=np.int64)
-        _lib.call("etl_plan_set_sources", self.handle, int(comp.size),
+        _lib.call("etl_plan_set_sources_sim", self.handle, int(sim), int(comp.size),
                   comp.ctypes.data_as(C.POINTER(C.c_int)), i.ctypes.data_as(_lib.i64p),
                   j.ctypes.data_as(_lib.i64p))
-        self.n_src = int(comp.size)
+        self.n_src[sim] = int(comp.size)
 
-    def source_table(self, n0: int, delta: np.ndarray) -> None:
-        dtype = self.state.hz.dtype
+    def source_table(self, n0: int, delta: np.ndarray, sim: int = 0) -> None:
+        dtype = self.states[sim].hz.dtype
         delta = np.ascontiguousarray(delta, dtype=dtype)
-        _lib.call("etl_plan_source_table", self.handle, int(n0), int(delta.shape[0]),
+        _lib.call("etl_plan_source_table_sim", self.handle, int(sim), int(n0), int(delta.shape[0]),
                   C.c_void_p(delta.ctypes.data))
 
-    def add_monitor(self, component: str, flat_index: np.ndarray, acc: DeviceArray) -> int:
+    def add_monitor(self, component: str, flat_index: np.ndarray, acc: DeviceArray,
+                    sim: int = 0) -> int:
         idx = np.ascontiguousarray(flat_index, dtype=np.int64)
         n_f = acc.shape[0]
         out = C.c_int()
-        _lib.call("etl_plan_add_monitor", self.handle, COMPONENT_CODE[component], int(idx.size),
-                  idx.ctypes.data_as(_lib.i64p), int(n_f), C.c_void_p(acc.ptr), C.byref(out))
+        _lib.call("etl_plan_add_monitor_sim", self.handle, int(sim), COMPONENT_CODE[component],
+                  int(idx.size), idx.ctypes.data_as(_lib.i64p), int(n_f), C.c_void_p(acc.ptr),
+                  C.byref(out))
         return out.value
 
-    def phasor_table(self, mon_id: int, n0: int, table: np.ndarray) -> None:
+    def phasor_table(self, mon_id: int, n0: int, table: np.ndarray, sim: int = 0) -> None:
         table = np.ascontiguousarray(table)
-        _lib.call("etl_plan_phasor_table", self.handle, int(mon_id), int(n0),
+        _lib.call("etl_plan_phasor_table_sim", self.handle, int(sim), int(mon_id), int(n0),
                   int(table.shape[0]), C.c_void_p(table.ctypes.data))
 
     def advance(self, n0: int, count: int) -> int:
@@ -116,7 +129,7 @@ class FusedPlan:
         """The tuning in effect (library defaults filled in)."""
         t = _lib.EtlTune()
         _lib.call("etl_plan_tune", self.handle, C.byref(t))
-        return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal)
+        return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal, t.waves)
 
     def time(self, n0: int, count: int) -> dict:
         """HIP-event timing of ``count`` steps launched directly: ``launch_ms`` (average
@@ -129,6 +142,7 @@ class FusedPlan:
         return {name: getattr(t, name) for name, _ in _lib.EtlTiming._fields_}
 
     def bind(self) -> None:
-        """Point the state's field attributes at the buffers holding the current state."""
-        for name in FIELDS:
-            setattr(self.state, name, self.pairs[name][self.parity])
+        """Point each state's field attributes at the buffers holding its current state."""
+        for s, pair in zip(self.states, self.pair_sets):
+            for name in FIELDS:
+                setattr(s, name, pair[name][self.parity])

@@ -122,20 +122,29 @@ typedef struct EtlTeDesc {
 } EtlTeDesc;
 
 /* Shape of the fused kernel (0 = library default: vec 1, band 8, temporal 4,
- * graph on, 64-step trace chunks).  (vec, temporal, band_rows) must be a compiled
- * register-tile shape: float32 vec 1 with band 8 or 16, or vec 2 with band 8, at
- * temporal depth 1, 2 or 4; float64 vec 1 with band 8 (depth 1, 2, 4) or band 16
- * (depth 1, 2). */
+ * graph on, 64-step trace chunks, one wave per workgroup).  (vec, temporal,
+ * band_rows) must be a compiled register-tile shape: float32 vec 1 with band 8 or
+ * 16, or vec 2 with band 8, at temporal depth 1, 2 or 4; float64 vec 1 with band 8
+ * (depth 1, 2, 4) or band 16 (depth 1, 2). */
 typedef struct EtlTune {
     int band_rows;     /* output rows of one wave's tile */
     int vec;           /* REAL elements per lane along y */
     int use_graph;     /* replay steps from a hipGraph (1) or launch directly (0) */
     int trace_chunk;   /* steps per DFT flush: a multiple of 2 * temporal */
     int temporal;      /* steps fused per launch: 1 (none), 2 or 4 */
+    int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
 int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out);
+/* A plan over n_sims independent simulations of any grid sizes (one precision),
+ * advanced together: every fused launch covers the tiles of all of them (one tile
+ * work-list), and one DFT fold covers all their monitors.  The functions without
+ * _sim address simulation 0; etl_plan_create is the batch of one.  All the
+ * simulations' tables must start at the same step.  No reference counterpart: the
+ * reference runs one simulation per FDTDSolver (etalon/fdtd/solver.py:49-160). */
+int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tune,
+                          etl_plan_t* out);
 int etl_plan_destroy(etl_plan_t plan);
 /* Point sources: node (comp, i, j) receives field -= delta[n][k] in step n; a node
  * listed more than once is updated once per listing, in list order. */
@@ -151,6 +160,14 @@ int etl_plan_add_monitor(etl_plan_t plan, int comp, int n_nodes, const int64_t* 
  * ph[n][f] is the factor of the H sample of step n when comp = Hz and of the E
  * sample otherwise (etalon/fdtd/solver.py:91-110). */
 int etl_plan_phasor_table(etl_plan_t plan, int id, int64_t n0, int rows, const void* ph);
+/* The same, for simulation `sim` of a batch plan. */
+int etl_plan_set_sources_sim(etl_plan_t plan, int sim, int n_nodes, const int* comp,
+                             const int64_t* i, const int64_t* j);
+int etl_plan_source_table_sim(etl_plan_t plan, int sim, int64_t n0, int rows, const void* delta);
+int etl_plan_add_monitor_sim(etl_plan_t plan, int sim, int comp, int n_nodes,
+                             const int64_t* flat_index, int n_f, void* acc, int* id);
+int etl_plan_phasor_table_sim(etl_plan_t plan, int sim, int id, int64_t n0, int rows,
+                              const void* ph);
 /* The tuning the plan resolved (library defaults filled in). */
 int etl_plan_tune(etl_plan_t plan, EtlTune* out);
 /* Advance steps [n, n + count); the tables must cover them.  On return *parity
@@ -161,7 +178,7 @@ typedef struct EtlTiming {
     double launch_ms;        /* average fused launch, fork to join (every tile kind) */
     double interior_ms;      /* average duration of the interior-tile kernel alone */
     double total_ms;         /* the whole call */
-    int64_t interior_cells;  /* Hz cells that kernel advances (per step of a launch) */
+    int64_t interior_cells;  /* Hz cells that kernel advances per step (all simulations) */
     int launches;            /* fused launches timed */
     int depth;               /* steps per launch (the plan's temporal depth) */
 } EtlTiming;
