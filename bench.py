@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision single|double]
                     [--no-cpu-baseline]
+    python bench.py --workload batched [--batch N] [--batch-steps S] [--seed K]
 
 Workload (the throughput grid): 2000 × 2000 interior cells at dl = 0.05 µm plus a
 12-layer CPML on every face (2024 × 2024 cells), a silicon strip waveguide
@@ -19,6 +20,15 @@ Ey, Hz, read cb_ex, cb_ey — times the cells it advances times the steps fused 
 launch, over its average duration from HIP events on the stream it runs on,
 against 8 TB/s; "effective", since temporal blocking moves fewer bytes than that);
 and the NumPy oracle on the host (one core) on a bounded sample of the same grid.
+
+``--workload batched``: N (64) simulations of uneven sizes (interior 200–1500
+cells per side, seeded bends and rings, 1–8 monitors each, 4000 steps) advanced
+together by one autotuned fused plan.  value = aggregate cell-updates/s; also
+sims/s, the same simulations run one after another (fused, and with the per-step
+primitives on a bounded sample of steps), a bitwise check of every simulation's
+fields against its sequential run, and the float32 error of every monitor against
+the float64 run of the same batch (float64 on this path is bit-identical to the
+NumPy oracle).
 """
 
 from __future__ import annotations
@@ -48,26 +58,7 @@ PMC_KIB = {("single", 1, 4, 8): (40183.4, 44627.4), ("double", 1, 4, 8): (86718.
 FETCH_SCALE, WRITE_SCALE = 1.9998, 1.0
 
 
-def throughput_sim(precision: str):
-    import etalon_amd as et
-
-    oxide = et.Medium(n=float(np.sqrt(2.085)))
-    silicon = et.Medium(n=float(np.sqrt(12.11)))
-    return et.Simulation(
-        size=(100.0, 100.0, 0.0),
-        grid=et.UniformGrid(dl=0.05),
-        medium=oxide,
-        structures=[et.Box(center=(0.0, 0.0, 0.0), size=(200.0, 0.5, 0.0), medium=silicon)],
-        boundary=et.PML(layers=12),
-        sources=[et.PointSource(center=(-48.0, 0.0, 0.0), component="Hz",
-                                pulse=et.GaussianPulse(wavelength=1.55, bandwidth=0.1))],
-        monitors=[et.DFTMonitor(name="line", center=(45.0, 0.0, 0.0), size=(0.0, 4.0, 0.0),
-                                wavelengths=np.linspace(1.50, 1.60, 21), components=("Hz",))],
-        run_time=1.0,
-        courant=0.99,
-        precision=precision,
-        backend="hip",
-    )
+from etalon_amd.workloads import throughput_sim, uneven_batch  # noqa: E402
 
 
 def cpu_baseline(precision: str, sample_steps: int) -> dict:
@@ -123,7 +114,8 @@ def run(precision: str, steps: int, warmup: int) -> dict:
     if delta is not None:
         plan.source_table(solver.n, delta)
     for mon_id, table in zip(solver._plan_ids, phasors):
-        plan.phasor_table(mon_id, solver.n, table)
+        if table is not None:
+            plan.phasor_table(mon_id, solver.n, table)
     tm = plan.time(solver.n, n_time)
     plan.bind()
     solver.n += n_time
@@ -144,6 +136,126 @@ def run(precision: str, steps: int, warmup: int) -> dict:
     }
 
 
+def _fingerprint(solver) -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in ("ex", "ey", "hz", "psi_ex_y", "psi_ey_x", "psi_hz_x", "psi_hz_y"):
+        h.update(np.ascontiguousarray(getattr(solver.state, name).get()).tobytes())
+    for mon in solver.monitors:
+        h.update(np.ascontiguousarray(mon.acc.get()).tobytes())
+    return h.hexdigest()
+
+
+def _monitor_values(solver) -> list:
+    return [np.asarray(mon.acc.get()).astype(np.complex128) for mon in solver.monitors]
+
+
+def run_batched(n: int, steps: int, seed: int, plain_steps: int) -> dict:
+    from etalon_amd.batch import BatchSolver
+    from etalon_amd.solver import FDTDSolver
+
+    sims = uneven_batch(n, seed=seed, steps=steps)
+    t0 = time.perf_counter()
+    batch = BatchSolver(sims, autotune=True)
+    setup = time.perf_counter() - t0
+    cells = batch.cells
+    batch.backend.synchronize()
+    t0 = time.perf_counter()
+    batch.advance(steps)
+    batch.backend.synchronize()
+    wall = time.perf_counter() - t0
+    prints = [_fingerprint(s) for s in batch.solvers]
+    f32_values = [_monitor_values(s) for s in batch.solvers]
+    tune = batch._plan.resolved_tune().__dict__
+    report = batch.tune_report
+    del batch
+
+    # the same simulations one after another on the fused path
+    seq_wall, same = 0.0, 0
+    for sim, fp in zip(sims, prints):
+        solver = FDTDSolver(sim)
+        solver.backend.synchronize()
+        t0 = time.perf_counter()
+        solver.advance(steps)
+        solver.backend.synchronize()
+        seq_wall += time.perf_counter() - t0
+        same += _fingerprint(solver) == fp
+        del solver
+
+    # ... and with the per-step primitives (one launch per half-step, the reference's
+    # loop), on the first `plain_steps` steps of each, scaled to the full step count
+    plain_wall = 0.0
+    for sim in sims:
+        solver = FDTDSolver(sim)
+        for _ in range(3):
+            solver.step()
+        solver.backend.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            solver.step()
+        solver.backend.synchronize()
+        plain_wall += (time.perf_counter() - t0) * steps / plain_steps
+        del solver
+
+    # float32 error of every monitor against the float64 run of the same batch
+    sims64 = uneven_batch(n, seed=seed, steps=steps, precision="double")
+    batch64 = BatchSolver(sims64, autotune=True)
+    batch64.advance(steps)
+    batch64.backend.synchronize()
+    errors = []
+    for s32, s64 in zip(f32_values, (_monitor_values(s) for s in batch64.solvers)):
+        for a, b in zip(s32, s64):
+            scale = float(np.max(np.abs(b)))
+            errors.append(float(np.max(np.abs(a - b)) / scale) if scale > 0 else 0.0)
+    del batch64
+    return {"n": n, "steps": steps, "cells": cells, "setup_s": setup, "wall": wall,
+            "seq_wall": seq_wall, "plain_wall": plain_wall, "bitwise_same": same,
+            "errors": errors, "tune": tune, "tune_report": report}
+
+
+def main_batched(args) -> int:
+    r = run_batched(args.batch, args.batch_steps, args.seed, args.plain_steps)
+    updates = r["cells"] * r["steps"]
+    errors = np.array(r["errors"])
+    line = {
+        "metric": "batched sims/s (aggregate Gcell-updates/s)",
+        "value": round(updates / r["wall"] / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": 0,
+        "ms_per_step": round(r["wall"] / r["steps"] * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: {r['n']} seeded simulations (seed {args.seed})",
+        "config": {"workload": f"batched uneven runs: {r['n']} simulations, interior 200-1500 "
+                               f"cells per side at dl 0.05 um + 12 CPML, bends/rings eps 12.11 in "
+                               f"2.085, Hz source, 1-8 monitors at 11 wavelengths, "
+                               f"{r['steps']} steps each", "cells_total": r["cells"]},
+        "sims_per_s": round(r["n"] / r["wall"], 2),
+        "wall_s": round(r["wall"], 4),
+        "sequential_fused": {"wall_s": round(r["seq_wall"], 4),
+                             "gcells": round(updates / r["seq_wall"] / 1e9, 3),
+                             "speedup": round(r["seq_wall"] / r["wall"], 2)},
+        "sequential_plain": {"wall_s": round(r["plain_wall"], 3),
+                             "gcells": round(updates / r["plain_wall"] / 1e9, 3),
+                             "speedup": round(r["plain_wall"] / r["wall"], 2),
+                             "sample": f"{args.plain_steps} step() calls per simulation, "
+                                       "scaled to the full step count"},
+        "bitwise_equal_to_sequential": f"{r['bitwise_same']}/{r['n']}",
+        "f32_vs_f64": {"monitors": int(errors.size), "max": float(f"{errors.max():.3g}"),
+                       "median": float(f"{np.median(errors):.3g}"),
+                       "per_monitor": [float(f"{e:.3g}") for e in errors]},
+        "tune": r["tune"],
+        "setup_s": round(r["setup_s"], 2),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,12 +264,19 @@ def main() -> int:
     ap.add_argument("--precision", choices=("single", "double"), default="single")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--workload", choices=("throughput", "batched"), default="throughput")
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=4000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the FDTD "
               "path runs on a single MI355X; multi-GPU sharding is a design note only)",
               file=sys.stderr)
         return 2
+    if args.workload == "batched":
+        return main_batched(args)
 
     main_run = run(args.precision, args.steps, args.warmup)
     nx, ny = main_run["cells"]

@@ -79,6 +79,7 @@ struct EtlPlan {
         int slot_base = 0, q_base = 0;
         int64_t n0 = 0;
         int rows = -1;
+        int owner = -1;  // >= 0: reads the phasor columns of that monitor
     };
     struct Sim {
         int nx = 0, ny = 0;
@@ -215,15 +216,17 @@ int build_layout(EtlPlan* p) {
                 const int row = int(fl / pitch);
                 nodes.push_back({mon.comp, row, int(fl - int64_t(row) * pitch), slot + k});
             }
+            if (mon.owner >= 0) mon.q_base = sim.mons[mon.owner].q_base;
             const int acc_index = int(accs.size());
             for (int f = 0; f < mon.n_f; ++f)
                 for (int k = 0; k < mon.n_nodes; ++k) {
-                    pairs.push_back(make_int4(slot + k, q + f, acc_index, f * mon.n_nodes + k));
+                    pairs.push_back(
+                        make_int4(slot + k, mon.q_base + f, acc_index, f * mon.n_nodes + k));
                     pair_sim.push_back(si);
                 }
             accs.push_back(mon.acc);
             slot += mon.n_nodes;
-            q += mon.n_f;
+            if (mon.owner < 0) q += mon.n_f;
         }
         sim.n_slots = slot;
         sim.nq = q;
@@ -538,7 +541,8 @@ int check_tables(EtlPlan* p, int64_t n, int64_t count, int64_t* n0) {
                 return ETL_ESTATE;
             }
         }
-        for (const auto& m : s.mons) {
+        for (const auto& mm : s.mons) {
+            const auto& m = mm.owner >= 0 ? s.mons[mm.owner] : mm;
             if (m.rows < 0 || !(n >= m.n0 && n + count <= m.n0 + m.rows)) {
                 set_error("etl_plan_advance: a phasor table of simulation %d does not cover "
                           "steps [%lld, %lld)", int(si), (long long)n, (long long)(n + count));
@@ -916,6 +920,8 @@ int etl_plan_phasor_table_sim(etl_plan_t p, int sim, int id, int64_t n0, int row
         s->ph_host_rows = rows;
     }
     auto& m = s->mons[id];
+    ETL_REQUIRE(m.owner < 0, "etl_plan_phasor_table: monitor %d reads the table of monitor %d",
+                id, m.owner);
     const unsigned char* src = static_cast<const unsigned char*>(ph);
     for (int r = 0; r < rows; ++r)
         std::memcpy(s->ph_host.data() + size_t(r) * row_bytes + size_t(m.q_base) * cplx,
@@ -923,6 +929,32 @@ int etl_plan_phasor_table_sim(etl_plan_t p, int sim, int id, int64_t n0, int row
     m.n0 = n0;
     m.rows = rows;
     s->ph_stale = true;
+    return ETL_OK;
+}
+
+int etl_plan_share_phasors(etl_plan_t p, int sim, int id, int owner) {
+    ETL_CHECK_INIT();
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr, "etl_plan_share_phasors: no simulation %d", sim);
+    ETL_REQUIRE(owner >= 0 && owner < id && id < int(s->mons.size()),
+                "etl_plan_share_phasors: monitor %d cannot share the table of monitor %d", id,
+                owner);
+    ETL_REQUIRE(s->mons[owner].owner < 0 && s->mons[owner].n_f == s->mons[id].n_f,
+                "etl_plan_share_phasors: monitor %d must own a table of %d frequencies", owner,
+                s->mons[id].n_f);
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    s->mons[id].owner = owner;
+    // the combined table loses monitor id's columns: drop it
+    etl_free(s->ph);
+    s->ph = nullptr;
+    s->ph_rows_cap = 0;
+    s->ph_host.clear();
+    s->ph_host_rows = 0;
+    s->ph_stale = false;
+    for (auto& mm : s->mons) mm.rows = -1;
+    p->dirty = true;
+    p->args_stale = true;
+    free_graphs(p);
     return ETL_OK;
 }
 

@@ -94,6 +94,7 @@ _SIGNATURES = {
     "etl_plan_add_monitor_sim": [vp, C.c_int, C.c_int, C.c_int, i64p, C.c_int, vp,
                                  C.POINTER(C.c_int)],
     "etl_plan_phasor_table_sim": [vp, C.c_int, C.c_int, C.c_int64, C.c_int, vp],
+    "etl_plan_share_phasors": [vp, C.c_int, C.c_int, C.c_int],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],
     "etl_plan_advance": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)],
     "etl_plan_time": [vp, C.c_int64, C.c_int64, C.POINTER(EtlTiming), C.POINTER(C.c_int)],

@@ -114,6 +114,10 @@ class FusedPlan:
                   C.byref(out))
         return out.value
 
+    def share_phasors(self, mon_id: int, owner: int, sim: int = 0) -> None:
+        """Monitor ``mon_id`` reads the phasor columns of monitor ``owner`` (equal tables)."""
+        _lib.call("etl_plan_share_phasors", self.handle, int(sim), int(mon_id), int(owner))
+
     def phasor_table(self, mon_id: int, n0: int, table: np.ndarray, sim: int = 0) -> None:
         table = np.ascontiguousarray(table)
         _lib.call("etl_plan_phasor_table_sim", self.handle, int(sim), int(mon_id), int(n0),

@@ -133,6 +133,13 @@ class FDTDSolver:
                                     for name in FIELDS):
             return plan
         plan = FusedPlan(self.state, self.tune)
+        self._plan_ids = self.register(plan, 0)
+        self._plan = plan
+        return plan
+
+    def register(self, plan, sim: int) -> list[int]:
+        """Give simulation ``sim`` of a fused plan this solver's sources and monitors;
+        returns the plan's monitor ids, in the order of ``self.monitors``."""
         comp, ii, jj = [], [], []
         code = {"Ex": 0, "Ey": 1, "Hz": 2}
         for inj in self.injections:
@@ -140,11 +147,34 @@ class FDTDSolver:
             ii += list(inj.i)
             jj += list(inj.j)
         plan.set_sources(np.array(comp, np.int32), np.array(ii, np.int64),
-                         np.array(jj, np.int64))
-        self._plan_ids = [plan.add_monitor(m.sampling.component, m.sampling.flat_index, m.acc)
-                          for m in self.monitors]
-        self._plan = plan
-        return plan
+                         np.array(jj, np.int64), sim=sim)
+        ids = [plan.add_monitor(m.sampling.component, m.sampling.flat_index, m.acc, sim=sim)
+               for m in self.monitors]
+        for k, owner in enumerate(self._phasor_owner()):
+            if owner is not None:
+                plan.share_phasors(ids[k], ids[owner], sim=sim)
+        return ids
+
+    def _phasor_owner(self) -> list:
+        """Per monitor: None if it owns its phasor table, else the index of the first
+        monitor with the same table (same frequencies, and both H or both E samples:
+        dft_phasors of the same times — equal bits)."""
+        first: dict = {}
+        owners = []
+        for k, mon in enumerate(self.monitors):
+            key = (mon.sampling.is_magnetic, mon.sampling.frequencies.tobytes())
+            owners.append(first.get(key))
+            first.setdefault(key, k)
+        return owners
+
+    def reset(self) -> None:
+        """Back to step 0: fields, ψ and monitor accumulators zeroed (stream-ordered)."""
+        for name in FIELDS:
+            getattr(self.state, name).fill_zero()
+        for mon in self.monitors:
+            mon.acc.fill_zero()
+            mon.table = None
+        self.n = 0
 
     def _tables(self, n0: int, rows: int):
         dt = self.grid.dt
@@ -154,7 +184,10 @@ class FDTDSolver:
                                    axis=1).astype(self.real)
         steps = np.arange(n0, n0 + rows, dtype=np.float64)
         phasors = []
-        for mon in self.monitors:
+        for mon, owner in zip(self.monitors, self._phasor_owner()):
+            if owner is not None:  # the plan reads the owner's table
+                phasors.append(None)
+                continue
             t = (steps + 0.5) * dt if mon.sampling.is_magnetic else (steps + 1.0) * dt
             phasors.append(dft_phasors(mon.sampling.frequencies, t, dt).astype(self.complex))
         return delta, phasors
@@ -195,7 +228,8 @@ class FDTDSolver:
             if delta is not None:
                 plan.source_table(self.n, delta)
             for mon_id, table in zip(self._plan_ids, phasors):
-                plan.phasor_table(mon_id, self.n, table)
+                if table is not None:
+                    plan.phasor_table(mon_id, self.n, table)
             plan.advance(self.n, rows)
             plan.bind()
             self.n += rows

@@ -168,6 +168,11 @@ int etl_plan_add_monitor_sim(etl_plan_t plan, int sim, int comp, int n_nodes,
                              const int64_t* flat_index, int n_f, void* acc, int* id);
 int etl_plan_phasor_table_sim(etl_plan_t plan, int sim, int id, int64_t n0, int rows,
                               const void* ph);
+/* Monitor `id` of simulation `sim` reads the phasor columns of monitor `owner` (< id,
+ * same n_f) instead of its own table: for monitors whose tables are equal (same
+ * frequencies, both H or both E), which then is computed and uploaded once.  Tables
+ * of the owner only; the shared monitor takes none. */
+int etl_plan_share_phasors(etl_plan_t plan, int sim, int id, int owner);
 /* The tuning the plan resolved (library defaults filled in). */
 int etl_plan_tune(etl_plan_t plan, EtlTune* out);
 /* Advance steps [n, n + count); the tables must cover them.  On return *parity
