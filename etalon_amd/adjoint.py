@@ -1,0 +1,323 @@
+"""Permittivity gradients by the discrete adjoint, on the HIP path.
+
+For a simulation whose last structure is a ``PixelGrid`` design region and an
+objective J = Σ over named monitors of Σ |values|² (every component they record,
+values as the result reports them), ``AdjointProblem`` computes J and ∂J/∂ε for
+every design pixel:
+
+1. forward: the fused plan advances the simulation in segments of K ≈ √N steps
+   and saves the full state at every segment start (checkpoints), then J comes
+   from the monitors;
+2. backward: for each segment from the last, the state is restored from its
+   checkpoint and the segment is recomputed forward with a probe on Hz around
+   the design box (time reversal through the CPML is unstable, so the forward
+   fields are recomputed, never run backwards), then the adjoint kernels run the
+   segment's reverse steps, accumulating ∂J/∂cb on the box;
+3. ∂J/∂εr = −(cb/εr) ∂J/∂cb, summed over the Ex/Ey nodes each pixel paints.
+
+``mode="store"`` keeps the probe history of the whole forward run instead (no
+recompute) when its memory is affordable.  ``PermittivityGradient`` exposes it
+as a ``torch.autograd.Function``.  The reference has no adjoint; the float64
+NumPy adjoint in the tests and central differences pin this one.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from etalon_amd import models as et
+from etalon_amd.grid import build_yee_grid, pixel_node_map
+from etalon_amd.hip import _lib
+from etalon_amd.hip.backend import zeros
+from etalon_amd.sampling import dft_phasors
+from etalon_amd.solver import FDTDSolver
+from etalon_amd.state import FIELDS, PROFILES
+
+_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
+
+
+def checkpoint_interval(n_steps: int) -> int:
+    """K ≈ √N, a multiple of 64 (the plan's graph chunk) and at least 64."""
+    return max(64, 64 * round(math.sqrt(n_steps) / 64))
+
+
+def _weights(sampling) -> np.ndarray:
+    """Interpolation matrix W (n_values, n_nodes) of a monitor component's labelled
+    values (monitor_data: size-0 axes interpolated in float64)."""
+    nxs, nys = sampling.x.index.size, sampling.y.index.size
+    wx = np.eye(nxs) if sampling.x.weights is None else np.asarray(sampling.x.weights)[None, :]
+    wy = np.eye(nys) if sampling.y.weights is None else np.asarray(sampling.y.weights)[None, :]
+    return np.kron(wx, wy)
+
+
+@dataclass
+class Tape:
+    """What the backward pass needs from the forward pass."""
+
+    solver: FDTDSolver
+    J: float
+    checkpoints: dict = field(default_factory=dict)  # field name -> (n_ckpt, size) DeviceArray
+    starts: list = field(default_factory=list)       # segment starts
+    history: Optional[object] = None                 # store mode: (N, box) probe output
+    forward_s: float = 0.0
+
+
+class AdjointProblem:
+    def __init__(self, sim, objective: Sequence[str], checkpoint_every: Optional[int] = None,
+                 mode: str = "checkpoint", tune=None):
+        if not sim.structures or sim.structures[-1].type != "PixelGrid":
+            raise ValueError("the design region must be the simulation's last structure, a "
+                             "PixelGrid")
+        names = {m.name for m in sim.monitors}
+        missing = set(objective) - names
+        if missing:
+            raise ValueError(f"objective monitors not in the simulation: {sorted(missing)}")
+        if mode not in ("checkpoint", "store"):
+            raise ValueError("mode is 'checkpoint' or 'store'")
+        self.sim = sim
+        self.objective = tuple(objective)
+        self.mode = mode
+        self.tune = tune
+        grid = build_yee_grid(sim)
+        self.grid = grid
+        self.n_steps = grid.n_steps(sim.run_time)
+        self.K = checkpoint_every or checkpoint_interval(self.n_steps)
+        design = sim.structures[-1]
+        self.shape = np.asarray(design.permittivity).shape
+        self.pixmap = pixel_node_map(grid, design.center, design.size, self.shape)
+        ii = np.concatenate([self.pixmap[c][0] for c in ("Ex", "Ey")])
+        jj = np.concatenate([self.pixmap[c][1] for c in ("Ex", "Ey")])
+        if ii.size == 0:
+            raise ValueError("the design region covers no E node")
+        self.box = (int(ii.min()), int(jj.min()), int(ii.max() - ii.min() + 1),
+                    int(jj.max() - jj.min() + 1))
+        for src in sim.sources:
+            if src.component != "Hz":
+                raise ValueError("adjoint problems take Hz sources (an E source's strength "
+                                 "depends on the permittivity at its nodes)")
+
+    # ------------------------------------------------------------------ helpers
+    def with_permittivity(self, eps: np.ndarray):
+        eps = np.asarray(eps, dtype=np.float64)
+        if eps.shape != self.shape:
+            raise ValueError(f"design permittivity must be {self.shape}, got {eps.shape}")
+        design = self.sim.structures[-1].model_copy(update={"permittivity": eps})
+        return self.sim.model_copy(update={"structures": (*self.sim.structures[:-1], design)})
+
+    def _check_box_off_cpml(self, solver) -> None:
+        i0, j0, ni, nj = self.box
+        s = solver.state
+        for name, lo, hi in (("bx_e", i0 - 1, i0 + ni + 1), ("bx_h", i0 - 1, i0 + ni + 1),
+                             ("by_e", j0 - 1, j0 + nj + 1), ("by_h", j0 - 1, j0 + nj + 1)):
+            b = getattr(s, name).get()[lo:hi]
+            c = getattr(s, name.replace("b", "c", 1)).get()[lo:hi]
+            ik = getattr(s, name.replace("b", "ik", 1)).get()[lo:hi]
+            if np.any(b != 1) or np.any(c != 0) or np.any(ik != 1):
+                raise ValueError("the design region must lie outside the CPML")
+
+    def _box_hz_nodes(self) -> np.ndarray:
+        i0, j0, ni, nj = self.box
+        ny = self.grid.ny
+        ii, jj = np.meshgrid(np.arange(i0 - 1, i0 + ni), np.arange(j0 - 1, j0 + nj), indexing="ij")
+        return (ii * ny + jj).ravel().astype(np.int64)
+
+    # ------------------------------------------------------------------ forward
+    def forward(self, eps: np.ndarray) -> Tape:
+        """Run the simulation with design permittivity ``eps``; J and the tape."""
+        sim = self.with_permittivity(eps)
+        solver = FDTDSolver(sim, tune=self.tune)
+        self._check_box_off_cpml(solver)
+        backend = solver.backend
+        N, K = self.n_steps, self.K
+        t0 = time.perf_counter()
+        plan = solver._ensure_plan()
+        delta, phasors = solver._tables(0, N)
+        if delta is not None:
+            plan.source_table(0, delta)
+        for mon_id, table in zip(solver._plan_ids, phasors):
+            if table is not None:
+                plan.phasor_table(mon_id, 0, table)
+        tape = Tape(solver=solver, J=0.0)
+        if self.mode == "store":
+            tape.history = zeros((N, self._box_hz_nodes().size), solver.real)
+            plan.add_probe("Hz", self._box_hz_nodes(), tape.history)
+            plan.advance(0, N)
+            plan.bind()
+        else:
+            tape.starts = list(range(0, N, K))
+            n_ck = len(tape.starts)
+            for name in FIELDS:
+                a = getattr(solver.state, name)
+                tape.checkpoints[name] = zeros((n_ck, a.size), a.dtype)
+            for k, c in enumerate(tape.starts):
+                for name in FIELDS:  # stream-ordered copies between the plan's launches
+                    a = plan.pairs[name][plan.parity]
+                    ck = tape.checkpoints[name]
+                    _lib.call("etl_memcpy_d2d", C.c_void_p(ck.ptr + k * a.nbytes),
+                              C.c_void_p(a.ptr), a.nbytes)
+                plan.advance(c, min(K, N - c))
+            plan.bind()
+        solver.n = N
+        backend.synchronize()
+        tape.forward_s = time.perf_counter() - t0
+        tape.J = self._objective(solver)
+        return tape
+
+    def _objective(self, solver) -> float:
+        J = 0.0
+        for mon in solver.monitors:
+            if mon.sampling.monitor in self.objective:
+                acc = mon.acc.get().astype(np.complex128)
+                G = acc @ _weights(mon.sampling).T
+                J += float(np.sum(np.abs(G) ** 2))
+        return J
+
+    def _sources(self, solver):
+        """Distinct adjoint-source nodes (comp, flat) and the table (N, n) of dJ/dv."""
+        N, dt = self.n_steps, self.grid.dt
+        steps = np.arange(N, dtype=np.float64)
+        cols: dict = {}
+        for mon in solver.monitors:
+            smp = mon.sampling
+            if smp.monitor not in self.objective:
+                continue
+            W = _weights(smp)
+            acc = mon.acc.get().astype(np.complex128)        # (n_f, n_nodes)
+            A = (acc @ W.T) @ W                              # (n_f, n_nodes)
+            t = (steps + 0.5) * dt if smp.is_magnetic else (steps + 1.0) * dt
+            P = dft_phasors(smp.frequencies, t, dt)          # (N, n_f), what the forward used
+            P = P.astype(solver.complex).astype(np.complex128)
+            S = 2.0 * np.real(P @ np.conj(A))                # (N, n_nodes)
+            for k, flat in enumerate(smp.flat_index):
+                key = (_COMP[smp.component], int(flat))
+                cols[key] = cols.get(key, 0.0) + S[:, k]
+        keys = sorted(cols)
+        comp = np.array([k[0] for k in keys], np.int32)
+        flat = np.array([k[1] for k in keys], np.int64)
+        table = np.ascontiguousarray(np.stack([cols[k] for k in keys], axis=1), dtype=solver.real) \
+            if keys else np.zeros((N, 0), solver.real)
+        return comp, flat, table
+
+    # ------------------------------------------------------------------ backward
+    def backward(self, tape: Tape) -> np.ndarray:
+        """∂J/∂ε per design pixel (float64), from the forward tape."""
+        solver = tape.solver
+        state = solver.state
+        real = solver.real
+        N, K = self.n_steps, self.K
+        i0, j0, ni, nj = self.box
+        lam = {name: zeros(getattr(state, name).shape, real) for name in FIELDS}
+        g_ex = zeros((ni, nj), np.float64)
+        g_ey = zeros((ni, nj), np.float64)
+        box_nodes = self._box_hz_nodes()
+        if self.mode == "store":
+            hist = tape.history
+        else:
+            hist = zeros((K, box_nodes.size), real)
+        d = _lib.EtlAdjointDesc()
+        d.precision = 0 if real == np.float32 else 1
+        d.nx, d.ny = int(state.nx), int(state.ny)
+        d.db, d.inv_dx, d.inv_dy = float(state.db), float(state.inv_dx), float(state.inv_dy)
+        for k, name in enumerate(FIELDS):
+            d.lam[k] = lam[name].ptr
+        d.cb_ex, d.cb_ey = state.cb_ex.ptr, state.cb_ey.ptr
+        for name in PROFILES:
+            setattr(d, name, getattr(state, name).ptr)
+        d.box_i0, d.box_j0, d.box_ni, d.box_nj = i0, j0, ni, nj
+        d.grad_ex, d.grad_ey = g_ex.ptr, g_ey.ptr
+        d.hist, d.hist_rows = hist.ptr, hist.shape[0]
+        handle = C.c_void_p()
+        _lib.call("etl_adjoint_create", C.byref(d), C.byref(handle))
+        try:
+            comp, flat, table = self._sources(solver)
+            _lib.call("etl_adjoint_sources", handle, int(comp.size),
+                      comp.ctypes.data_as(C.POINTER(C.c_int)), flat.ctypes.data_as(_lib.i64p),
+                      int(table.shape[0]), C.c_void_p(table.ctypes.data))
+            if self.mode == "store":
+                _lib.call("etl_adjoint_run", handle, 0, N)
+            else:
+                self._reverse_segments(tape, handle, hist)
+            solver.backend.synchronize()
+        finally:
+            _lib.call("etl_adjoint_destroy", handle)
+        # ∂J/∂εr = −(cb/εr) ∂J/∂cb at the nodes, summed per pixel
+        eps = solver.eps_r
+        cb = {"Ex": state.cb_ex.get(), "Ey": state.cb_ey.get()}
+        gcb = {"Ex": g_ex.get(), "Ey": g_ey.get()}
+        out = np.zeros(self.shape[0] * self.shape[1])
+        for c in ("Ex", "Ey"):
+            ii, jj, p = self.pixmap[c]
+            g = -(cb[c][ii, jj].astype(np.float64) / eps[c][ii, jj]) * gcb[c][ii - i0, jj - j0]
+            np.add.at(out, p, g)
+        return out.reshape(self.shape)
+
+    def _reverse_segments(self, tape: Tape, handle, hist) -> None:
+        """Recompute each segment from its checkpoint (with the Hz probe on the box),
+        then run its reverse steps, from the last segment to the first."""
+        from etalon_amd.hip.plan import FusedPlan
+
+        solver = tape.solver
+        N = self.n_steps
+        plan = FusedPlan(solver.state, self.tune)
+        try:
+            comp, ii, jj = [], [], []
+            for inj in solver.injections:
+                comp += [_COMP[inj.component]] * inj.i.size
+                ii += list(inj.i)
+                jj += list(inj.j)
+            plan.set_sources(np.array(comp, np.int32), np.array(ii, np.int64),
+                             np.array(jj, np.int64))
+            delta, _ = solver._tables(0, N)
+            if delta is not None:
+                plan.source_table(0, delta)
+            plan.add_probe("Hz", self._box_hz_nodes(), hist)
+            for k in range(len(tape.starts) - 1, -1, -1):
+                c = tape.starts[k]
+                count = min(self.K, N - c)
+                for name in FIELDS:
+                    a = plan.pairs[name][plan.parity]
+                    ck = tape.checkpoints[name]
+                    _lib.call("etl_memcpy_d2d", C.c_void_p(a.ptr),
+                              C.c_void_p(ck.ptr + k * a.nbytes), a.nbytes)
+                plan.advance(c, count)
+                _lib.call("etl_adjoint_run", handle, c, count)
+        finally:
+            plan.close()
+
+    def run(self, eps: np.ndarray) -> tuple[float, np.ndarray]:
+        """(J, ∂J/∂ε per pixel)."""
+        tape = self.forward(eps)
+        return tape.J, self.backward(tape)
+
+
+def _function():
+    import torch
+
+    class PermittivityGradient(torch.autograd.Function):
+        """J(ε) with ∂J/∂ε by the adjoint; ε a (npx, npy) tensor of pixel permittivities."""
+
+        @staticmethod
+        def forward(ctx, eps, problem):
+            tape = problem.forward(eps.detach().to("cpu", torch.float64).numpy())
+            ctx.problem, ctx.tape = problem, tape
+            return torch.tensor(tape.J, dtype=eps.dtype, device=eps.device)
+
+        @staticmethod
+        def backward(ctx, grad_output):
+            g = ctx.problem.backward(ctx.tape)
+            ctx.tape = None
+            return grad_output * torch.as_tensor(g, dtype=grad_output.dtype,
+                                                 device=grad_output.device), None
+
+    return PermittivityGradient
+
+
+def objective(eps, problem: AdjointProblem):
+    """J(ε) as a differentiable torch scalar (``torch.autograd.Function``)."""
+    return _function().apply(eps, problem)

@@ -96,5 +96,9 @@ template <typename R>
 int launch_trace_dft(const TraceSim<R>* sims, int rows, int* counter, const int4* pairs,
                      const int* pair_sim, int n_pairs, R* const* accs, hipStream_t stream);
 int launch_bump(int* counter, int by, hipStream_t stream);
+// out[(counter[0] - counter[2] + r) * n + k] = trace[r * n_slots + slot_base + k], r < rows.
+template <typename R>
+int launch_probe_copy(const R* trace, int n_slots, int slot_base, int n, R* out, int rows,
+                      const int* counter, hipStream_t stream);
 
 }  // namespace etl

@@ -22,7 +22,8 @@ namespace {
 __global__ void set_counter_kernel(int* counter, int value) {
     if (threadIdx.x == 0) {
         counter[0] = value;
-        counter[1] = 0;  // block tally of the DFT kernel
+        counter[1] = 0;      // block tally of the DFT kernel
+        counter[2] = value;  // first step of this advance: probe rows count from it
     }
 }
 
@@ -81,6 +82,15 @@ struct EtlPlan {
         int rows = -1;
         int owner = -1;  // >= 0: reads the phasor columns of that monitor
     };
+    // A time-domain probe: the values of its nodes at every step of an advance go to
+    // rows of `out` (n_nodes REALs per row, row 0 = the first step of the advance).
+    struct Probe {
+        int comp;
+        std::vector<int64_t> flat;
+        void* out;
+        int out_rows;
+        int slot_base = 0;
+    };
     struct Sim {
         int nx = 0, ny = 0;
         EtlTeDesc desc{};
@@ -98,8 +108,9 @@ struct EtlPlan {
         std::vector<unsigned char> delta_host, ph_host;  // ph_host: (ph_host_rows, nq)
         int ph_host_rows = 0;
         bool delta_stale = false, ph_stale = false;
-        // monitors
+        // monitors and probes
         std::vector<Mon> mons;
+        std::vector<Probe> probes;
         int n_slots = 0, nq = 0;
         int* tile_mon[3] = {nullptr, nullptr, nullptr};   // per component: CSR over extra tiles
         int4* tile_ent[3] = {nullptr, nullptr, nullptr};  // (row, column, slot, 0)
@@ -227,6 +238,16 @@ int build_layout(EtlPlan* p) {
             accs.push_back(mon.acc);
             slot += mon.n_nodes;
             if (mon.owner < 0) q += mon.n_f;
+        }
+        for (auto& pr : sim.probes) {
+            pr.slot_base = slot;
+            const int pitch = sim.pitch_of(pr.comp);
+            for (size_t k = 0; k < pr.flat.size(); ++k) {
+                const int row = int(pr.flat[k] / pitch);
+                nodes.push_back({pr.comp, row, int(pr.flat[k] - int64_t(row) * pitch),
+                                 slot + int(k)});
+            }
+            slot += int(pr.flat.size());
         }
         sim.n_slots = slot;
         sim.nq = q;
@@ -435,6 +456,13 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
         ++n;
     }
     *launches = n;
+    for (const auto& sim : p->sims)
+        for (const auto& pr : sim.probes) {
+            const int rc = launch_probe_copy<R>((const R*)sim.trace, sim.n_slots, pr.slot_base,
+                                                int(pr.flat.size()), (R*)pr.out, count,
+                                                p->counter, s);
+            if (rc) return rc;
+        }
     if (p->n_pairs && count)
         return launch_trace_dft<R>(static_cast<const TraceSim<R>*>(p->trace_sims_dev), count,
                                    p->counter, p->pairs, p->pair_sim, p->n_pairs,
@@ -568,6 +596,11 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_l
     if (p->args_stale) {
         if ((rc = upload_args(p))) return rc;
     }
+    for (const auto& sim : p->sims)
+        for (const auto& pr : sim.probes)
+            ETL_REQUIRE(count <= pr.out_rows,
+                        "etl_plan_advance: %lld steps, a probe holds %d rows", (long long)count,
+                        pr.out_rows);
     const int64_t start = n - n0;
     ETL_REQUIRE(start >= 0 && start + count < (int64_t(1) << 31), "etl_plan_advance: step range");
     hipStream_t s = ctx().stream;
@@ -892,6 +925,37 @@ int etl_plan_add_monitor_sim(etl_plan_t p, int sim, int comp, int n_nodes,
     s->ph_host_rows = 0;
     s->ph_stale = false;
     for (auto& mm : s->mons) mm.rows = -1;
+    p->args_stale = true;
+    free_graphs(p);
+    return ETL_OK;
+}
+
+int etl_plan_add_probe_sim(etl_plan_t p, int sim, int comp, int n_nodes,
+                           const int64_t* flat_index, void* out, int out_rows) {
+    ETL_CHECK_INIT();
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr, "etl_plan_add_probe: no simulation %d", sim);
+    ETL_REQUIRE(comp >= 0 && comp <= 2, "etl_plan_add_probe: component %d", comp);
+    ETL_REQUIRE(n_nodes >= 1 && flat_index && out && out_rows >= 1,
+                "etl_plan_add_probe: bad probe (%d nodes, %d rows)", n_nodes, out_rows);
+    const int64_t size = int64_t(s->rows_of(comp)) * s->pitch_of(comp);
+    const int pitch = s->pitch_of(comp);
+    EtlPlan::Probe pr;
+    pr.comp = comp;
+    pr.flat.assign(flat_index, flat_index + n_nodes);
+    for (int64_t f : pr.flat) {
+        ETL_REQUIRE(f >= 0 && f < size, "etl_plan_add_probe: index %lld outside component %d",
+                    (long long)f, comp);
+        ETL_REQUIRE(!(comp == 1 && f / pitch == s->nx),
+                    "etl_plan_add_probe: node on the outer tangential-E boundary");
+    }
+    ETL_REQUIRE(guarded_bytes(out) >= size_t(out_rows) * n_nodes * p->elem(),
+                "etl_plan_add_probe: out must be an etl_malloc allocation of out_rows x n_nodes");
+    pr.out = out;
+    pr.out_rows = out_rows;
+    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    s->probes.push_back(std::move(pr));
+    p->dirty = true;
     p->args_stale = true;
     free_graphs(p);
     return ETL_OK;

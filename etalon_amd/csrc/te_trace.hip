@@ -66,6 +66,17 @@ __global__ __launch_bounds__(256) void trace_dft_kernel(const TraceSim<R>* __res
     }
 }
 
+template <typename R>
+__global__ void probe_copy_kernel(const R* __restrict__ trace, int n_slots, int slot_base, int n,
+                                  R* __restrict__ out, int rows, const int* __restrict__ counter) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(rows) * n) return;
+    const int r = int(t / n);
+    const int k = int(t - int64_t(r) * n);
+    const int64_t base = int64_t(counter[0]) - counter[2];
+    out[(base + r) * n + k] = trace[int64_t(r) * n_slots + slot_base + k];
+}
+
 __global__ void bump_counter_kernel(int* counter, int by) {
     if (threadIdx.x == 0) counter[0] = counter[0] + by;
 }
@@ -82,12 +93,27 @@ int launch_trace_dft(const TraceSim<R>* sims, int rows, int* counter, const int4
     return ETL_OK;
 }
 
+template <typename R>
+int launch_probe_copy(const R* trace, int n_slots, int slot_base, int n, R* out, int rows,
+                      const int* counter, hipStream_t stream) {
+    const int64_t total = int64_t(rows) * n;
+    if (total == 0) return ETL_OK;
+    hipLaunchKernelGGL(probe_copy_kernel<R>, dim3(unsigned((total + 255) / 256)), dim3(256), 0,
+                       stream, trace, n_slots, slot_base, n, out, rows, counter);
+    ETL_LAUNCHED();
+    return ETL_OK;
+}
+
 int launch_bump(int* counter, int by, hipStream_t stream) {
     hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(64), 0, stream, counter, by);
     ETL_LAUNCHED();
     return ETL_OK;
 }
 
+template int launch_probe_copy<float>(const float*, int, int, int, float*, int, const int*,
+                                      hipStream_t);
+template int launch_probe_copy<double>(const double*, int, int, int, double*, int, const int*,
+                                       hipStream_t);
 template int launch_trace_dft<float>(const TraceSim<float>*, int, int*, const int4*, const int*,
                                      int, float* const*, hipStream_t);
 template int launch_trace_dft<double>(const TraceSim<double>*, int, int*, const int4*,

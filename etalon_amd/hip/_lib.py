@@ -51,6 +51,29 @@ class EtlTune(C.Structure):
     ]
 
 
+class EtlAdjointDesc(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("db", C.c_double),
+        ("inv_dx", C.c_double),
+        ("inv_dy", C.c_double),
+        ("lam", vp * 7),
+        ("cb_ex", vp),
+        ("cb_ey", vp),
+        ("bx_e", vp), ("cx_e", vp), ("ikx_e", vp),
+        ("bx_h", vp), ("cx_h", vp), ("ikx_h", vp),
+        ("by_e", vp), ("cy_e", vp), ("iky_e", vp),
+        ("by_h", vp), ("cy_h", vp), ("iky_h", vp),
+        ("box_i0", C.c_int), ("box_j0", C.c_int), ("box_ni", C.c_int), ("box_nj", C.c_int),
+        ("grad_ex", vp),
+        ("grad_ey", vp),
+        ("hist", vp),
+        ("hist_rows", C.c_int),
+    ]
+
+
 class EtlTiming(C.Structure):
     _fields_ = [
         ("launch_ms", C.c_double),
@@ -95,6 +118,11 @@ _SIGNATURES = {
                                  C.POINTER(C.c_int)],
     "etl_plan_phasor_table_sim": [vp, C.c_int, C.c_int, C.c_int64, C.c_int, vp],
     "etl_plan_share_phasors": [vp, C.c_int, C.c_int, C.c_int],
+    "etl_plan_add_probe_sim": [vp, C.c_int, C.c_int, C.c_int, i64p, vp, C.c_int],
+    "etl_adjoint_create": [C.POINTER(EtlAdjointDesc), C.POINTER(vp)],
+    "etl_adjoint_destroy": [vp],
+    "etl_adjoint_sources": [vp, C.c_int, C.POINTER(C.c_int), i64p, C.c_int, vp],
+    "etl_adjoint_run": [vp, C.c_int64, C.c_int],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],
     "etl_plan_advance": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)],
     "etl_plan_time": [vp, C.c_int64, C.c_int64, C.POINTER(EtlTiming), C.POINTER(C.c_int)],

@@ -114,6 +114,17 @@ class FusedPlan:
                   C.byref(out))
         return out.value
 
+    def add_probe(self, component: str, flat_index: np.ndarray, out: DeviceArray,
+                  sim: int = 0) -> None:
+        """Time-domain probe: row r of ``out`` (rows, n_nodes) receives the nodes' values
+        of step n + r during ``advance(n, count)`` (count <= rows)."""
+        idx = np.ascontiguousarray(flat_index, dtype=np.int64)
+        if out.ndim != 2 or out.shape[1] != idx.size:
+            raise ValueError("probe output must be (rows, n_nodes)")
+        _lib.call("etl_plan_add_probe_sim", self.handle, int(sim), COMPONENT_CODE[component],
+                  int(idx.size), idx.ctypes.data_as(_lib.i64p), C.c_void_p(out.ptr),
+                  int(out.shape[0]))
+
     def share_phasors(self, mon_id: int, owner: int, sim: int = 0) -> None:
         """Monitor ``mon_id`` reads the phasor columns of monitor ``owner`` (equal tables)."""
         _lib.call("etl_plan_share_phasors", self.handle, int(sim), int(mon_id), int(owner))

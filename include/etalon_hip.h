@@ -168,6 +168,14 @@ int etl_plan_add_monitor_sim(etl_plan_t plan, int sim, int comp, int n_nodes,
                              const int64_t* flat_index, int n_f, void* acc, int* id);
 int etl_plan_phasor_table_sim(etl_plan_t plan, int sim, int id, int64_t n0, int rows,
                               const void* ph);
+/* A time-domain probe on simulation `sim`: during etl_plan_advance(n, count), row r
+ * of `out` (an etl_malloc buffer of out_rows x n_nodes REALs, count <= out_rows)
+ * receives the values of the nodes of step n + r, sampled where monitors are (Hz
+ * after the H update and M injection, E after the E update and J injection).  No
+ * reference counterpart; the reference records such traces with a Python loop of
+ * step() and single-node reads (etalon/validation/m0.py:153-158). */
+int etl_plan_add_probe_sim(etl_plan_t plan, int sim, int comp, int n_nodes,
+                           const int64_t* flat_index, void* out, int out_rows);
 /* Monitor `id` of simulation `sim` reads the phasor columns of monitor `owner` (< id,
  * same n_f) instead of its own table: for monitors whose tables are equal (same
  * frequencies, both H or both E), which then is computed and uploaded once.  Tables
@@ -188,6 +196,44 @@ typedef struct EtlTiming {
     int depth;               /* steps per launch (the plan's temporal depth) */
 } EtlTiming;
 int etl_plan_time(etl_plan_t plan, int64_t n, int64_t count, EtlTiming* out, int* parity);
+
+/* ------------------------------------------------------------------ adjoint
+ * The discrete adjoint of the step for permittivity gradients of DFT-monitor
+ * objectives (no reference counterpart: the reference has no adjoint).  The adjoint
+ * state lambda has the forward state's seven arrays and shapes; a reverse step n
+ * adds dJ/d(monitored values of step n) to lambda, then applies the transposed E
+ * and H half-steps (CPML recurrences transposed), and accumulates dJ/dcb_ex and
+ * dJ/dcb_ey on a design box from the forward Hz of that step (a history the
+ * caller provides, e.g. from a probe of a forward recompute). */
+typedef struct EtlAdjoint* etl_adjoint_t;
+typedef struct EtlAdjointDesc {
+    int precision;
+    int nx, ny;
+    double db, inv_dx, inv_dy;
+    void* lam[7];   /* lambda: ex, ey, hz, psi_ex_y, psi_ey_x, psi_hz_x, psi_hz_y (zeroed) */
+    const void* cb_ex;
+    const void* cb_ey;
+    const void *bx_e, *cx_e, *ikx_e, *bx_h, *cx_h, *ikx_h;
+    const void *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
+    /* Design box: Ex and Ey nodes (i, j), box_i0 <= i < box_i0 + box_ni, box_j0 <= j <
+     * box_j0 + box_nj, off the CPML (its profiles must be (1, 0, 1) there). */
+    int box_i0, box_j0, box_ni, box_nj;
+    double* grad_ex;    /* (box_ni, box_nj) dJ/dcb_ex, accumulated */
+    double* grad_ey;    /* (box_ni, box_nj) dJ/dcb_ey, accumulated */
+    /* (hist_rows, (box_ni + 1) * (box_nj + 1)) REAL: Hz after the H update of each step
+     * of a run, on rows box_i0 - 1 .. box_i0 + box_ni - 1 and columns box_j0 - 1 ..
+     * box_j0 + box_nj - 1; row r is step n_first + r of etl_adjoint_run. */
+    const void* hist;
+    int hist_rows;
+} EtlAdjointDesc;
+int etl_adjoint_create(const EtlAdjointDesc* desc, etl_adjoint_t* out);
+int etl_adjoint_destroy(etl_adjoint_t adj);
+/* Adjoint sources: distinct nodes (comp, flat index) and a host table (rows, n) REAL
+ * of dJ/dv for steps 0 .. rows - 1. */
+int etl_adjoint_sources(etl_adjoint_t adj, int n, const int* comp, const int64_t* flat, int rows,
+                        const void* table);
+/* Reverse steps n_first + count - 1 down to n_first (stream-ordered). */
+int etl_adjoint_run(etl_adjoint_t adj, int64_t n_first, int count);
 
 #ifdef __cplusplus
 }
