@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision single|double]
                     [--no-cpu-baseline]
     python bench.py --workload batched [--batch N] [--batch-steps S] [--seed K]
+    python bench.py --workload adjoint [--seed K] [--fd-pixels P]
 
 Workload (the throughput grid): 2000 × 2000 interior cells at dl = 0.05 µm plus a
 12-layer CPML on every face (2024 × 2024 cells), a silicon strip waveguide
@@ -29,6 +30,14 @@ primitives on a bounded sample of steps), a bitwise check of every simulation's
 fields against its sequential run, and the float32 error of every monitor against
 the float64 run of the same batch (float64 on this path is bit-identical to the
 NumPy oracle).
+
+``--workload adjoint``: a 424 x 424 grid (400 x 400 interior + 12 CPML) with a
+2 x 2 µm design region of 40 x 40 pixels between an input and an output strip,
+J = Σ over 5 wavelengths of |DFT Hz|² on the output line monitor, 4000 steps,
+float32, checkpoints every 64 steps.  value = wall time of J and ∂J/∂ε over the
+wall time of one plain forward run; the gradient is checked against the float64
+gradient and float64 central differences on random pixels; the NumPy float64
+adjoint is timed on a bounded sample of steps of the same grid.
 """
 
 from __future__ import annotations
@@ -58,7 +67,7 @@ PMC_KIB = {("single", 1, 4, 8): (40183.4, 44627.4), ("double", 1, 4, 8): (86718.
 FETCH_SCALE, WRITE_SCALE = 1.9998, 1.0
 
 
-from etalon_amd.workloads import throughput_sim, uneven_batch  # noqa: E402
+from etalon_amd.workloads import adjoint_design_sim, throughput_sim, uneven_batch  # noqa: E402
 
 
 def cpu_baseline(precision: str, sample_steps: int) -> dict:
@@ -256,6 +265,101 @@ def main_batched(args) -> int:
     return 0
 
 
+def run_adjoint(seed: int, fd_pixels: int, cpu_steps: int) -> dict:
+    from etalon_amd.adjoint import AdjointProblem
+    from etalon_amd.solver import FDTDSolver
+
+    sim, design, names = adjoint_design_sim("single", seed=seed)
+    eps = np.asarray(design.permittivity)
+    prob = AdjointProblem(sim, names)
+    FDTDSolver(sim).run()  # warm: first launches of the kernels
+    prob.run(eps)
+    solver = FDTDSolver(sim)
+    t_fwd = solver.run().metadata["wall_time_s"]
+    tape = prob.forward(eps)
+    t0 = time.perf_counter()
+    grad = prob.backward(tape)
+    t_grad = tape.forward_s + time.perf_counter() - t0
+    store = AdjointProblem(sim, names, mode="store")
+    store.run(eps)
+    tape_s = store.forward(eps)
+    t0 = time.perf_counter()
+    grad_s = store.backward(tape_s)
+    t_store = tape_s.forward_s + time.perf_counter() - t0
+
+    sim64, _, _ = adjoint_design_sim("double", seed=seed)
+    prob64 = AdjointProblem(sim64, names)
+    J64, grad64 = prob64.run(eps)
+    scale = float(np.abs(grad64).max())
+    rng = np.random.default_rng(seed + 1)
+    fd_rel = []
+    for flat in rng.choice(eps.size, size=fd_pixels, replace=False):
+        p, q = divmod(int(flat), eps.shape[1])
+        h = 1e-4 * eps[p, q]
+        plus, minus = eps.copy(), eps.copy()
+        plus[p, q] += h
+        minus[p, q] -= h
+        fd = (prob64.forward(plus).J - prob64.forward(minus).J) / (2 * h)
+        fd_rel.append(abs(grad[p, q] - fd) / scale)
+
+    # NumPy float64 adjoint (the oracle) on the first `cpu_steps` steps of the grid
+    from oracle import adjoint_numpy as adj
+    from etalon_amd.workloads import adjoint_design_sim as make
+
+    short, short_design, _ = make("double", seed=seed, steps=cpu_steps)
+    t0 = time.perf_counter()
+    adj.gradient(short, set(names), short_design)
+    cpu_wall = time.perf_counter() - t0
+    return {"t_fwd": t_fwd, "t_grad": t_grad, "t_store": t_store, "J": tape.J, "J64": J64,
+            "grad_rel_f64": float(np.abs(grad - grad64).max() / scale),
+            "store_equal": bool(np.array_equal(grad, grad_s)), "fd_rel": fd_rel,
+            "cpu_wall": cpu_wall, "cpu_steps": cpu_steps, "K": prob.K,
+            "cells": prob.grid.nx * prob.grid.ny, "steps": prob.n_steps, "pixels": eps.shape}
+
+
+def main_adjoint(args) -> int:
+    r = run_adjoint(args.seed, args.fd_pixels, args.cpu_adjoint_steps)
+    fd = np.array(r["fd_rel"])
+    cpu_grad = r["cpu_wall"] * r["steps"] / r["cpu_steps"]
+    line = {
+        "metric": "adjoint grad time vs 1 fwd",
+        "value": round(r["t_grad"] / r["t_fwd"], 3),
+        "unit": "x (wall of J and dJ/deps / wall of one plain forward run)",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": 1,
+        "ms_per_step": round(r["t_grad"] / r["steps"] * 1e3, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: seeded design permittivity (seed {args.seed})",
+        "config": {"workload": "adjoint gradient: 400x400 interior + 12 CPML at dl 0.05 um, "
+                               "2x2 um design region of 40x40 pixels (eps 2.085-12.11) between "
+                               "input/output strips, J = sum over 5 wavelengths of |DFT Hz|^2 on "
+                               "the output line monitor, 4000 steps, checkpoints every "
+                               f"{r['K']} steps", "cells": r["cells"],
+                   "pixels": list(r["pixels"])},
+        "t_fwd_s": round(r["t_fwd"], 4),
+        "t_grad_s": round(r["t_grad"], 4),
+        "store_history": {"t_grad_s": round(r["t_store"], 4),
+                          "ratio": round(r["t_store"] / r["t_fwd"], 3),
+                          "gradient_bitwise_equal": r["store_equal"]},
+        "gradient_check": {"vs_f64_adjoint_max_rel": float(f"{r['grad_rel_f64']:.3g}"),
+                           "fd_pixels": int(fd.size),
+                           "vs_f64_central_fd_max_rel": float(f"{fd.max():.3g}"),
+                           "J_f32": r["J"], "J_f64": r["J64"]},
+        "cpu_baseline": {"value": round(cpu_grad, 2), "unit": "s per gradient (extrapolated)",
+                         "cores": 1, "kind": "port",
+                         "sample": f"NumPy float64 adjoint of the same grid over {r['cpu_steps']} "
+                                   f"steps ({r['cpu_wall']:.1f} s), scaled to {r['steps']}; "
+                                   f"host {os.cpu_count()} CPUs, {_cpu_model()}",
+                         "vs_gpu_grad": round(cpu_grad / r["t_grad"], 1)},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,7 +368,10 @@ def main() -> int:
     ap.add_argument("--precision", choices=("single", "double"), default="single")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=400)
-    ap.add_argument("--workload", choices=("throughput", "batched"), default="throughput")
+    ap.add_argument("--workload", choices=("throughput", "batched", "adjoint"),
+                    default="throughput")
+    ap.add_argument("--fd-pixels", type=int, default=16)
+    ap.add_argument("--cpu-adjoint-steps", type=int, default=300)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=4000)
     ap.add_argument("--seed", type=int, default=0)
@@ -277,6 +384,8 @@ def main() -> int:
         return 2
     if args.workload == "batched":
         return main_batched(args)
+    if args.workload == "adjoint":
+        return main_adjoint(args)
 
     main_run = run(args.precision, args.steps, args.warmup)
     nx, ny = main_run["cells"]

@@ -111,3 +111,37 @@ def uneven_batch(n: int = 64, seed: int = 0, lo: int = 200, hi: int = 1500, step
                                   sources=[source], monitors=monitors, run_time=run_time,
                                   courant=0.99, precision=precision, backend="hip"))
     return sims
+
+
+def adjoint_design_sim(precision: str = "single", seed: int = 0, steps: int = 4000,
+                       interior: int = 400, region: float = 2.0, pixels: int = 40):
+    """``interior`` x ``interior`` cells at dl = 0.05 µm + 12-layer CPML, an input
+    silicon strip (0.5 µm) from the left face and an output strip to the right face,
+    a ``region`` µm square design region of ``pixels`` x ``pixels`` pixels between
+    them with seeded permittivity in [2.085, 12.11], an Hz source in the input strip
+    and a 2 µm Hz line monitor across the output strip at 5 wavelengths 1.50–1.60
+    µm; ``steps`` steps.  Returns (simulation, design PixelGrid, objective names)."""
+    rng = np.random.default_rng(seed)
+    oxide = et.Medium(n=float(np.sqrt(OXIDE)))
+    silicon = et.Medium(n=float(np.sqrt(SILICON)))
+    size = interior * DL
+    half = size / 2.0
+    eps = rng.uniform(OXIDE, SILICON, size=(pixels, pixels))
+    design = et.PixelGrid(center=(0.0, 0.0, 0.0), size=(region, region, 0.0), permittivity=eps)
+    strip_len = half - region / 2.0
+    sim = et.Simulation(
+        size=(size, size, 0.0), grid=et.UniformGrid(dl=DL), medium=oxide,
+        structures=[
+            et.Box(center=(-half + strip_len / 2.0, 0.0, 0.0), size=(strip_len + 1.0, WIDTH, 0.0),
+                   medium=silicon),
+            et.Box(center=(half - strip_len / 2.0, 0.0, 0.0), size=(strip_len + 1.0, WIDTH, 0.0),
+                   medium=silicon),
+            design,
+        ],
+        boundary=et.PML(layers=12),
+        sources=[et.PointSource(center=(-half + 1.0, 0.0, 0.0), component="Hz",
+                                pulse=et.GaussianPulse(wavelength=1.55, bandwidth=0.1))],
+        monitors=[et.DFTMonitor(name="out", center=(half - 2.0, 0.0, 0.0), size=(0.0, 2.0, 0.0),
+                                wavelengths=np.linspace(1.50, 1.60, 5), components=("Hz",))],
+        run_time=_run_time_for(steps), courant=0.99, precision=precision, backend="hip")
+    return sim, design, ("out",)
