@@ -70,7 +70,7 @@ class Tape:
 
 class AdjointProblem:
     def __init__(self, sim, objective: Sequence[str], checkpoint_every: Optional[int] = None,
-                 mode: str = "checkpoint", tune=None):
+                 mode: str = "checkpoint", tune=None, kernel: str = "tile"):
         if not sim.structures or sim.structures[-1].type != "PixelGrid":
             raise ValueError("the design region must be the simulation's last structure, a "
                              "PixelGrid")
@@ -80,6 +80,10 @@ class AdjointProblem:
             raise ValueError(f"objective monitors not in the simulation: {sorted(missing)}")
         if mode not in ("checkpoint", "store"):
             raise ValueError("mode is 'checkpoint' or 'store'")
+        if kernel not in ("tile", "plain"):
+            raise ValueError("kernel is 'tile' (temporally blocked) or 'plain' (one launch per "
+                             "reverse half-step)")
+        self.kernel = kernel
         self.sim = sim
         self.objective = tuple(objective)
         self.mode = mode
@@ -232,20 +236,17 @@ class AdjointProblem:
         d.box_i0, d.box_j0, d.box_ni, d.box_nj = i0, j0, ni, nj
         d.grad_ex, d.grad_ey = g_ex.ptr, g_ey.ptr
         d.hist, d.hist_rows = hist.ptr, hist.shape[0]
-        handle = C.c_void_p()
-        _lib.call("etl_adjoint_create", C.byref(d), C.byref(handle))
+        rev = _Reverse(self.kernel, d, self.tune)
         try:
             comp, flat, table = self._sources(solver)
-            _lib.call("etl_adjoint_sources", handle, int(comp.size),
-                      comp.ctypes.data_as(C.POINTER(C.c_int)), flat.ctypes.data_as(_lib.i64p),
-                      int(table.shape[0]), C.c_void_p(table.ctypes.data))
+            rev.sources(comp, flat, table)
             if self.mode == "store":
-                _lib.call("etl_adjoint_run", handle, 0, N)
+                rev.run(0, N)
             else:
-                self._reverse_segments(tape, handle, hist)
+                self._reverse_segments(tape, rev, hist)
             solver.backend.synchronize()
         finally:
-            _lib.call("etl_adjoint_destroy", handle)
+            rev.close()
         # ∂J/∂εr = −(cb/εr) ∂J/∂cb at the nodes, summed per pixel
         eps = solver.eps_r
         cb = {"Ex": state.cb_ex.get(), "Ey": state.cb_ey.get()}
@@ -257,7 +258,7 @@ class AdjointProblem:
             np.add.at(out, p, g)
         return out.reshape(self.shape)
 
-    def _reverse_segments(self, tape: Tape, handle, hist) -> None:
+    def _reverse_segments(self, tape: Tape, rev, hist) -> None:
         """Recompute each segment from its checkpoint (with the Hz probe on the box),
         then run its reverse steps, from the last segment to the first."""
         from etalon_amd.hip.plan import FusedPlan
@@ -286,7 +287,7 @@ class AdjointProblem:
                     _lib.call("etl_memcpy_d2d", C.c_void_p(a.ptr),
                               C.c_void_p(ck.ptr + k * a.nbytes), a.nbytes)
                 plan.advance(c, count)
-                _lib.call("etl_adjoint_run", handle, c, count)
+                rev.run(c, count)
         finally:
             plan.close()
 
@@ -294,6 +295,41 @@ class AdjointProblem:
         """(J, ∂J/∂ε per pixel)."""
         tape = self.forward(eps)
         return tape.J, self.backward(tape)
+
+
+class _Reverse:
+    """The reverse steps: the temporally blocked adjoint plan ("tile") or the
+    one-launch-per-half-step kernels ("plain"); the same arithmetic."""
+
+    def __init__(self, kernel: str, desc, tune) -> None:
+        self.kernel = kernel
+        self.handle = C.c_void_p()
+        if kernel == "tile":
+            t = None
+            if tune is not None:
+                t = _lib.EtlTune(0, 1, 1, 0, tune.temporal, tune.waves)
+            _lib.call("etl_adjplan_create", C.byref(desc), 1, C.byref(t) if t else None,
+                      C.byref(self.handle))
+        else:
+            _lib.call("etl_adjoint_create", C.byref(desc), C.byref(self.handle))
+
+    def sources(self, comp, flat, table) -> None:
+        args = (int(comp.size), comp.ctypes.data_as(C.POINTER(C.c_int)),
+                flat.ctypes.data_as(_lib.i64p), int(table.shape[0]), C.c_void_p(table.ctypes.data))
+        if self.kernel == "tile":
+            _lib.call("etl_adjplan_sources", self.handle, 0, *args)
+        else:
+            _lib.call("etl_adjoint_sources", self.handle, *args)
+
+    def run(self, n_first: int, count: int) -> None:
+        name = "etl_adjplan_run" if self.kernel == "tile" else "etl_adjoint_run"
+        _lib.call(name, self.handle, int(n_first), int(count))
+
+    def close(self) -> None:
+        if self.handle:
+            name = "etl_adjplan_destroy" if self.kernel == "tile" else "etl_adjoint_destroy"
+            _lib.call(name, self.handle)
+            self.handle = C.c_void_p()
 
 
 def _function():

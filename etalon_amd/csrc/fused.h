@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace etl {
 
@@ -37,6 +38,25 @@ struct FusedArgs {
     int n_slots;
 };
 
+// Strip of an axis: every index whose profile differs from (1, 0, 1) lies in
+// [0, lo) or [hi, n).
+template <typename R>
+void strip_bounds(const std::vector<R>& b, const std::vector<R>& c, const std::vector<R>& ik,
+                  int& lo, int& hi) {
+    const int n = int(b.size());
+    const int half = (n + 1) / 2;
+    lo = 0;
+    hi = n;
+    for (int k = 0; k < n; ++k) {
+        const bool nontrivial = (b[k] != R(1)) || (c[k] != R(0)) || (ik[k] != R(1));
+        if (!nontrivial) continue;
+        if (k < half)
+            lo = k + 1;
+        else if (hi == n)
+            hi = k;
+    }
+}
+
 // Output columns of a tile for vector width v and temporal depth t.
 inline int tile_out_cols(int v, int t) { return 64 * v - 2 * t; }
 
@@ -53,8 +73,10 @@ struct TileList {
     int n_interior;
     const int4* strip;
     int n_strip;
-    const int4* extra;
+    const int4* extra;  // extra tiles off the CPML strips (no psi)
     int n_extra;
+    const int4* extra_strip;  // extra tiles that meet a strip (rare)
+    int n_extra_strip;
 };
 
 // Register-tile shapes compiled into the library: (precision, vec, depth, band rows).

@@ -43,25 +43,6 @@ __global__ void psi_check_kernel(const R* __restrict__ psi, int rows, int cols, 
     if ((!in_strip || on_edge) && psi[k] != R(0)) flag[0] = 1;
 }
 
-// Strip of an axis: every index whose profile differs from (1, 0, 1) lies in
-// [0, lo) or [hi, n).
-template <typename R>
-void strip_bounds(const std::vector<R>& b, const std::vector<R>& c, const std::vector<R>& ik,
-                  int& lo, int& hi) {
-    const int n = int(b.size());
-    const int half = (n + 1) / 2;
-    lo = 0;
-    hi = n;
-    for (int k = 0; k < n; ++k) {
-        const bool nontrivial = (b[k] != R(1)) || (c[k] != R(0)) || (ik[k] != R(1));
-        if (!nontrivial) continue;
-        if (k < half)
-            lo = k + 1;
-        else if (hi == n)
-            hi = k;
-    }
-}
-
 }  // namespace
 }  // namespace etl
 
@@ -128,7 +109,7 @@ struct EtlPlan {
     void* args_dev = nullptr;        // FusedArgs<R>[n_sims]
     void* trace_sims_dev = nullptr;  // TraceSim<R>[n_sims]
     int4* tiles_dev = nullptr;       // interior, then strip, then extra tile descriptors
-    int n_interior = 0, n_strip = 0, n_extra = 0;
+    int n_interior = 0, n_strip = 0, n_extra = 0, n_extra_strip = 0;
     int64_t interior_cells = 0;      // Hz cells the interior tiles own
     int4* pairs = nullptr;           // DFT pairs of every simulation
     int* pair_sim = nullptr;
@@ -202,7 +183,7 @@ int build_layout(EtlPlan* p) {
     const int W = 64 * p->tune.vec;
     const int band = p->tune.band_rows;
 
-    std::vector<int4> interior, strip, extra, pairs;
+    std::vector<int4> interior, strip, extra, extra_strip, pairs;
     std::vector<int> pair_sim;
     std::vector<void*> accs;
     int64_t interior_cells = 0;
@@ -266,7 +247,11 @@ int build_layout(EtlPlan* p) {
                 if (s.y >= r0 - T && s.y <= r1 + T && s.z >= cs && s.z < cs + W) return true;
             return false;
         };
-        const int extra0 = int(extra.size());
+        std::vector<int4> sim_extra;  // this simulation's extra sub-tiles, numbered in order
+        auto sub_touches = [&](int r0, int r1, int tj) {
+            const int cs = tj * wo - T;
+            return (r0 - T < x_lo) || (r1 + T > x_hi) || (cs < y_lo) || (cs + W > y_hi);
+        };
         for (int tb = 0; tb < nb; ++tb) {
             const int i0 = tb * band;
             const int i1 = std::min(i0 + band, nx);
@@ -282,20 +267,23 @@ int build_layout(EtlPlan* p) {
                 }
                 for (int r0 = i0; r0 < i1; r0 += kSubBand) {
                     const int r1 = std::min(r0 + kSubBand, nx);
-                    if (owns_node(r0, r1, tj) || sees_source(r0, r1, tj))
-                        extra.push_back(make_int4(r0, tj, int(extra.size()) - extra0, si));
-                    else
+                    if (owns_node(r0, r1, tj) || sees_source(r0, r1, tj)) {
+                        const int4 d = make_int4(r0, tj, int(sim_extra.size()), si);
+                        sim_extra.push_back(d);
+                        (sub_touches(r0, r1, tj) ? extra_strip : extra).push_back(d);
+                    } else {
                         strip.push_back(make_int4(r0, tj, 0, si));
+                    }
                 }
             }
         }
         // monitor entries per extra sub-tile of this simulation (row, column, slot)
-        const int n_extra_sim = int(extra.size()) - extra0;
+        const int n_extra_sim = int(sim_extra.size());
         std::vector<std::vector<std::pair<int, int4>>> ent(3);
         for (const Node& n : nodes) {
             const int tj = n.col / wo;
-            for (int e = extra0; e < int(extra.size()); ++e) {
-                const int4& x = extra[e];
+            for (int e = 0; e < n_extra_sim; ++e) {
+                const int4& x = sim_extra[e];
                 if (x.y == tj && n.row >= x.x && n.row < x.x + kSubBand) {
                     ent[n.comp].push_back({x.z, make_int4(n.row, n.col, n.slot, 0)});
                     break;
@@ -324,6 +312,7 @@ int build_layout(EtlPlan* p) {
     std::vector<int4> tiles(interior);
     tiles.insert(tiles.end(), strip.begin(), strip.end());
     tiles.insert(tiles.end(), extra.begin(), extra.end());
+    tiles.insert(tiles.end(), extra_strip.begin(), extra_strip.end());
     int rc;
     if (!tiles.empty() &&
         (rc = upload((void**)&p->tiles_dev, tiles.data(), tiles.size() * sizeof(int4))))
@@ -331,6 +320,7 @@ int build_layout(EtlPlan* p) {
     p->n_interior = int(interior.size());
     p->n_strip = int(strip.size());
     p->n_extra = int(extra.size());
+    p->n_extra_strip = int(extra_strip.size());
     p->interior_cells = interior_cells;
     p->n_pairs = int(pairs.size());
     if (p->n_pairs) {
@@ -427,12 +417,10 @@ template <typename R>
 int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
     const FusedArgs<R>* args = static_cast<const FusedArgs<R>*>(p->args_dev);
     hipStream_t s = ctx().stream;
-    const TileList tl{p->tiles_dev,
-                      p->n_interior,
-                      p->tiles_dev + p->n_interior,
-                      p->n_strip,
-                      p->tiles_dev + p->n_interior + p->n_strip,
-                      p->n_extra};
+    const int4* t = p->tiles_dev;
+    const TileList tl{t, p->n_interior, t + p->n_interior, p->n_strip,
+                      t + p->n_interior + p->n_strip, p->n_extra,
+                      t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
     const int T = p->tune.temporal;
     int k = 0, n = 0;
     while (k < count) {

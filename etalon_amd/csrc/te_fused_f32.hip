@@ -21,8 +21,14 @@ int launch_shape(const FusedArgs<R>* args, const TileList& tl, int waves, const 
                            kstep0, parity);
         ETL_LAUNCHED();
     }
+    if (tl.n_extra_strip) {  // after the strip tiles, on their stream
+        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, true, true>), grid(tl.n_extra_strip, 1),
+                           dim3(64), 0, s_strip, args, tl.extra_strip, tl.n_extra_strip, counter,
+                           kstep0, parity);
+        ETL_LAUNCHED();
+    }
     if (tl.n_extra) {
-        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, true, true>), grid(tl.n_extra, 1),
+        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, false, true>), grid(tl.n_extra, 1),
                            dim3(64), 0, s_extra, args, tl.extra, tl.n_extra, counter, kstep0,
                            parity);
         ETL_LAUNCHED();

@@ -123,6 +123,11 @@ _SIGNATURES = {
     "etl_adjoint_destroy": [vp],
     "etl_adjoint_sources": [vp, C.c_int, C.POINTER(C.c_int), i64p, C.c_int, vp],
     "etl_adjoint_run": [vp, C.c_int64, C.c_int],
+    "etl_adjplan_create": [C.POINTER(EtlAdjointDesc), C.c_int, C.POINTER(EtlTune), C.POINTER(vp)],
+    "etl_adjplan_destroy": [vp],
+    "etl_adjplan_sources": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), i64p, C.c_int, vp],
+    "etl_adjplan_run": [vp, C.c_int64, C.c_int],
+    "etl_adjplan_lambda": [vp, C.c_int, C.c_int, C.POINTER(vp)],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],
     "etl_plan_advance": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)],
     "etl_plan_time": [vp, C.c_int64, C.c_int64, C.POINTER(EtlTiming), C.POINTER(C.c_int)],

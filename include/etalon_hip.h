@@ -235,6 +235,22 @@ int etl_adjoint_sources(etl_adjoint_t adj, int n, const int* comp, const int64_t
 /* Reverse steps n_first + count - 1 down to n_first (stream-ordered). */
 int etl_adjoint_run(etl_adjoint_t adj, int64_t n_first, int count);
 
+/* The same reverse steps with the temporally blocked register-tile kernels, for one
+ * or many simulations at once (a work-list over all their tiles, like
+ * etl_plan_create_batch).  desc.lam are the A buffers of lambda (etl_malloc, zeroed);
+ * the plan allocates the B buffers and etl_adjplan_lambda names the one holding
+ * lambda after a run.  tune: temporal 1, 2 or 4 and waves (vec 1, 8-row tiles). */
+typedef struct EtlAdjPlan* etl_adjplan_t;
+int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* tune,
+                       etl_adjplan_t* out);
+int etl_adjplan_destroy(etl_adjplan_t plan);
+int etl_adjplan_sources(etl_adjplan_t plan, int sim, int n, const int* comp, const int64_t* flat,
+                        int rows, const void* table);
+/* Reverse steps n_first + count - 1 .. n_first of every simulation (history row r =
+ * step n_first + r). */
+int etl_adjplan_run(etl_adjplan_t plan, int64_t n_first, int count);
+int etl_adjplan_lambda(etl_adjplan_t plan, int sim, int field, void** ptr);
+
 #ifdef __cplusplus
 }
 #endif

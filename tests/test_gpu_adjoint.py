@@ -69,15 +69,16 @@ def oracle_gradient():
     return adj.gradient(sim, {"out"}, design)
 
 
+@pytest.mark.parametrize("kernel", ["tile", "plain"])
 @pytest.mark.parametrize("mode", ["checkpoint", "store"])
-def test_double_gradient_equals_the_numpy_adjoint(hip, oracle_gradient, mode):
+def test_double_gradient_equals_the_numpy_adjoint(hip, oracle_gradient, mode, kernel):
     sim, design = design_problem("double", seed=4, steps=200)
     J_ref, g_ref = oracle_gradient
-    prob = AdjointProblem(sim, ["out"], checkpoint_every=64, mode=mode)
+    prob = AdjointProblem(sim, ["out"], checkpoint_every=64, mode=mode, kernel=kernel)
     J, g = prob.run(np.asarray(design.permittivity))
     assert J == pytest.approx(J_ref, rel=1e-13)
     err = np.abs(g - g_ref).max() / np.abs(g_ref).max()
-    print(f"{mode}: max |g - g_numpy| / max |g_numpy| = {err:.3e}")
+    print(f"{mode}/{kernel}: max |g - g_numpy| / max |g_numpy| = {err:.3e}")
     assert err < 1e-10
 
 
@@ -88,6 +89,21 @@ def test_single_gradient_against_float64(hip, oracle_gradient):
     err = np.abs(g - g_ref).max() / np.abs(g_ref).max()
     print(f"f32 J rel err {abs(J - J_ref) / J_ref:.3e}, gradient max rel err {err:.3e}")
     assert abs(J - J_ref) / J_ref < 1e-4 and err < 1e-3
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4])
+def test_tile_and_plain_adjoint_agree(hip, depth):
+    """The temporally blocked reverse steps equal the per-half-step kernels (signed
+    zeros aside), with sources, design box and CPML, in float32 and any depth."""
+    from etalon_amd.hip.plan import Tune
+
+    sim, design = design_problem("single", seed=21, steps=250, pixels=(7, 5))
+    eps = np.asarray(design.permittivity)
+    _, g_plain = AdjointProblem(sim, ["out"], kernel="plain").run(eps)
+    _, g_tile = AdjointProblem(sim, ["out"], kernel="tile", tune=Tune(temporal=depth)).run(eps)
+    err = np.abs(g_tile - g_plain).max() / np.abs(g_plain).max()
+    print(f"T={depth}: max |tile - plain| / max |plain| = {err:.3e}")
+    assert err < 1e-6
 
 
 def test_checkpoint_and_store_modes_agree_bitwise(hip):
