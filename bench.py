@@ -280,6 +280,12 @@ def run_adjoint(seed: int, fd_pixels: int, cpu_steps: int) -> dict:
     t0 = time.perf_counter()
     grad = prob.backward(tape)
     t_grad = tape.forward_s + time.perf_counter() - t0
+    seq = AdjointProblem(sim, names, schedule="sequential")
+    seq.run(eps)
+    tape_q = seq.forward(eps)
+    t0 = time.perf_counter()
+    grad_q = seq.backward(tape_q)
+    t_seq = tape_q.forward_s + time.perf_counter() - t0
     store = AdjointProblem(sim, names, mode="store")
     store.run(eps)
     tape_s = store.forward(eps)
@@ -310,7 +316,8 @@ def run_adjoint(seed: int, fd_pixels: int, cpu_steps: int) -> dict:
     t0 = time.perf_counter()
     adj.gradient(short, set(names), short_design)
     cpu_wall = time.perf_counter() - t0
-    return {"t_fwd": t_fwd, "t_grad": t_grad, "t_store": t_store, "J": tape.J, "J64": J64,
+    return {"t_fwd": t_fwd, "t_grad": t_grad, "t_store": t_store, "t_seq": t_seq,
+            "seq_equal": bool(np.array_equal(grad, grad_q)), "J": tape.J, "J64": J64,
             "grad_rel_f64": float(np.abs(grad - grad64).max() / scale),
             "store_equal": bool(np.array_equal(grad, grad_s)), "fd_rel": fd_rel,
             "cpu_wall": cpu_wall, "cpu_steps": cpu_steps, "K": prob.K,
@@ -342,6 +349,9 @@ def main_adjoint(args) -> int:
                    "pixels": list(r["pixels"])},
         "t_fwd_s": round(r["t_fwd"], 4),
         "t_grad_s": round(r["t_grad"], 4),
+        "sequential_segments": {"t_grad_s": round(r["t_seq"], 4),
+                                "ratio": round(r["t_seq"] / r["t_fwd"], 3),
+                                "gradient_bitwise_equal": r["seq_equal"]},
         "store_history": {"t_grad_s": round(r["t_store"], 4),
                           "ratio": round(r["t_store"] / r["t_fwd"], 3),
                           "gradient_bitwise_equal": r["store_equal"]},

@@ -12,7 +12,10 @@ every design pixel:
    checkpoint and the segment is recomputed forward with a probe on Hz around
    the design box (time reversal through the CPML is unstable, so the forward
    fields are recomputed, never run backwards), then the adjoint kernels run the
-   segment's reverse steps, accumulating ∂J/∂cb on the box;
+   segment's reverse steps, accumulating ∂J/∂cb on the box; with the tile
+   kernels the recompute of segment k − 1 runs beside the reverse steps of
+   segment k, in the same launches (``schedule="joint"``, a co-plan) or on a
+   second stream (``"lanes"``), the two alternating between two history halves;
 3. ∂J/∂εr = −(cb/εr) ∂J/∂cb, summed over the Ex/Ey nodes each pixel paints.
 
 ``mode="store"`` keeps the probe history of the whole forward run instead (no
@@ -70,7 +73,8 @@ class Tape:
 
 class AdjointProblem:
     def __init__(self, sim, objective: Sequence[str], checkpoint_every: Optional[int] = None,
-                 mode: str = "checkpoint", tune=None, kernel: str = "tile"):
+                 mode: str = "checkpoint", tune=None, kernel: str = "tile",
+                 schedule: str = "joint"):
         if not sim.structures or sim.structures[-1].type != "PixelGrid":
             raise ValueError("the design region must be the simulation's last structure, a "
                              "PixelGrid")
@@ -83,7 +87,12 @@ class AdjointProblem:
         if kernel not in ("tile", "plain"):
             raise ValueError("kernel is 'tile' (temporally blocked) or 'plain' (one launch per "
                              "reverse half-step)")
+        if schedule not in ("joint", "lanes", "sequential"):
+            raise ValueError("schedule is 'joint', 'lanes' or 'sequential'")
         self.kernel = kernel
+        # checkpoint mode: how the recompute of segment k - 1 and the reverse steps of
+        # segment k share the GPU (the plain kernels run them one after the other)
+        self.schedule = schedule if kernel == "tile" else "sequential"
         self.sim = sim
         self.objective = tuple(objective)
         self.mode = mode
@@ -159,13 +168,8 @@ class AdjointProblem:
             for name in FIELDS:
                 a = getattr(solver.state, name)
                 tape.checkpoints[name] = zeros((n_ck, a.size), a.dtype)
-            for k, c in enumerate(tape.starts):
-                for name in FIELDS:  # stream-ordered copies between the plan's launches
-                    a = plan.pairs[name][plan.parity]
-                    ck = tape.checkpoints[name]
-                    _lib.call("etl_memcpy_d2d", C.c_void_p(ck.ptr + k * a.nbytes),
-                              C.c_void_p(a.ptr), a.nbytes)
-                plan.advance(c, min(K, N - c))
+            plan.set_checkpoints([tape.checkpoints[name] for name in FIELDS], n_ck)
+            plan.advance_checkpointed(0, N, K)  # slot k = the state at step k K
             plan.bind()
         solver.n = N
         backend.synchronize()
@@ -223,7 +227,9 @@ class AdjointProblem:
         if self.mode == "store":
             hist = tape.history
         else:
-            hist = zeros((K, box_nodes.size), real)
+            # two halves when recompute and reverse steps overlap
+            rows = 2 * K if self.schedule != "sequential" else K
+            hist = zeros((rows, box_nodes.size), real)
         d = _lib.EtlAdjointDesc()
         d.precision = 0 if real == np.float32 else 1
         d.nx, d.ny = int(state.nx), int(state.ny)
@@ -278,16 +284,55 @@ class AdjointProblem:
             if delta is not None:
                 plan.source_table(0, delta)
             plan.add_probe("Hz", self._box_hz_nodes(), hist)
-            for k in range(len(tape.starts) - 1, -1, -1):
-                c = tape.starts[k]
-                count = min(self.K, N - c)
+            # Schedules: "joint" — the recompute of segment k - 1 and the reverse steps
+            # of segment k in one launch per step block (a co-plan); "lanes" — the
+            # same two on two streams; "sequential" — one after the other.  The first
+            # two alternate between two history halves.
+            schedule = self.schedule if hist.shape[0] >= 2 * self.K else "sequential"
+            lane = 1 if schedule == "lanes" else 0
+            if lane:
+                plan.set_lane(1)
+
+            def half(k: int) -> int:
+                return (k & 1) * self.K if schedule != "sequential" else 0
+
+            def count(k: int) -> int:
+                return min(self.K, N - tape.starts[k])
+
+            def restore(k: int) -> None:
                 for name in FIELDS:
                     a = plan.pairs[name][plan.parity]
                     ck = tape.checkpoints[name]
-                    _lib.call("etl_memcpy_d2d", C.c_void_p(a.ptr),
-                              C.c_void_p(ck.ptr + k * a.nbytes), a.nbytes)
-                plan.advance(c, count)
-                rev.run(c, count)
+                    _lib.call("etl_memcpy_d2d_lane", C.c_void_p(a.ptr),
+                              C.c_void_p(ck.ptr + k * a.nbytes), a.nbytes, lane)
+
+            def recompute(k: int) -> None:
+                restore(k)
+                plan.advance(tape.starts[k], count(k), half(k))
+
+            if schedule == "joint":
+                plan.set_checkpoints([tape.checkpoints[name] for name in FIELDS],
+                                     len(tape.starts))
+                co = C.c_void_p()
+                _lib.call("etl_coplan_create", plan.handle, rev.handle, C.byref(co))
+                try:
+                    _lib.call("etl_coplan_sweep", co, N, self.K)
+                finally:
+                    _lib.call("etl_coplan_destroy", co)
+                return
+            last = len(tape.starts) - 1
+            recompute(last)
+            for k in range(last, -1, -1):
+                if schedule == "lanes":
+                    _lib.call("etl_lane_wait", 0, 1)      # reverse k after recompute k
+                    if k > 0:
+                        _lib.call("etl_lane_wait", 1, 0)  # recompute k-1 after reverse k+1
+                        recompute(k - 1)
+                    rev.run(tape.starts[k], count(k), half(k))
+                else:
+                    rev.run(tape.starts[k], count(k))
+                    if k > 0:
+                        recompute(k - 1)
         finally:
             plan.close()
 
@@ -307,7 +352,7 @@ class _Reverse:
         if kernel == "tile":
             t = None
             if tune is not None:
-                t = _lib.EtlTune(0, 1, 1, 0, tune.temporal, tune.waves)
+                t = _lib.EtlTune(0, 1, 1, 0, tune.temporal, tune.waves, tune.launch)
             _lib.call("etl_adjplan_create", C.byref(desc), 1, C.byref(t) if t else None,
                       C.byref(self.handle))
         else:
@@ -321,9 +366,14 @@ class _Reverse:
         else:
             _lib.call("etl_adjoint_sources", self.handle, *args)
 
-    def run(self, n_first: int, count: int) -> None:
-        name = "etl_adjplan_run" if self.kernel == "tile" else "etl_adjoint_run"
-        _lib.call(name, self.handle, int(n_first), int(count))
+    def run(self, n_first: int, count: int, hist_row0: int = 0) -> None:
+        if self.kernel == "tile":
+            _lib.call("etl_adjplan_run_rows", self.handle, int(n_first), int(count),
+                      int(hist_row0))
+        else:
+            if hist_row0:
+                raise ValueError("the plain reverse kernels read history from row 0")
+            _lib.call("etl_adjoint_run", self.handle, int(n_first), int(count))
 
     def close(self) -> None:
         if self.handle:

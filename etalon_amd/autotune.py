@@ -1,5 +1,6 @@
-"""Autotuner of the fused kernels' shape: tile rows, vector width, temporal depth and
-waves per workgroup, measured on the simulations it will run.
+"""Autotuner of the fused kernels' shape: tile rows, vector width, temporal depth,
+waves per workgroup and launch mode (one kernel per step block over every tile
+kind, or one per kind on three streams), measured on the simulations it will run.
 
 The candidates are the register-tile shapes compiled into libetalon_hip.so (see
 ``CANDIDATES``).  Each one advances the real simulations by ``steps`` steps from
@@ -20,12 +21,15 @@ from typing import Optional, Sequence
 
 from etalon_amd.hip.plan import FusedPlan, Tune
 
-# (vec, temporal, band_rows, waves) per precision, all compiled shapes
+# (vec, temporal, band_rows, waves, launch) per precision, all compiled shapes
 CANDIDATES = {
-    "single": [(1, 4, 8, 1), (1, 4, 8, 2), (1, 4, 8, 4), (1, 4, 16, 1), (1, 4, 16, 2),
-               (2, 4, 8, 1), (1, 2, 8, 1)],
-    "double": [(1, 4, 8, 1), (1, 4, 8, 2), (1, 4, 8, 4), (1, 2, 16, 1), (1, 2, 8, 1)],
+    "single": [(1, 4, 8, 1, 3), (1, 4, 8, 2, 3), (1, 4, 8, 4, 3), (1, 4, 16, 1, 3),
+               (1, 4, 16, 2, 3), (2, 4, 8, 1, 3), (1, 2, 8, 1, 3), (1, 4, 8, 1, 1),
+               (1, 2, 8, 1, 1)],
+    "double": [(1, 4, 8, 1, 3), (1, 4, 8, 2, 3), (1, 4, 8, 4, 3), (1, 2, 16, 1, 3),
+               (1, 2, 8, 1, 3), (1, 4, 8, 1, 1), (1, 2, 8, 1, 1)],
 }
+_KEYS = ("vec", "temporal", "band_rows", "waves", "launch")
 
 
 def cache_path() -> Path:
@@ -93,12 +97,13 @@ def autotune(solvers: Sequence, steps: int = 128, candidates: Optional[list] = N
     if use_cache and key in cache:
         v = cache[key]
         return Tune(band_rows=v["band_rows"], vec=v["vec"], temporal=v["temporal"],
-                    waves=v["waves"]), {"key": key, "cached": True, "choice": v}
+                    waves=v["waves"], launch=v.get("launch", 0)), \
+            {"key": key, "cached": True, "choice": v}
     cands = candidates or CANDIDATES[precision]
     cells = sum(s.grid.nx * s.grid.ny for s in solvers)
     timings = []
-    for vec, depth, band, waves in cands:
-        tune = Tune(band_rows=band, vec=vec, temporal=depth, waves=waves)
+    for vec, depth, band, waves, launch in cands:
+        tune = Tune(band_rows=band, vec=vec, temporal=depth, waves=waves, launch=launch)
         try:
             wall = _time_candidate(solvers, tune, steps)
         finally:
@@ -106,12 +111,13 @@ def autotune(solvers: Sequence, steps: int = 128, candidates: Optional[list] = N
                 s.reset()
             backend.synchronize()
         timings.append({"vec": vec, "temporal": depth, "band_rows": band, "waves": waves,
-                        "gcells": cells * steps / wall / 1e9})
+                        "launch": launch, "gcells": cells * steps / wall / 1e9})
     best = max(timings, key=lambda t: t["gcells"])
-    choice = {k: best[k] for k in ("vec", "temporal", "band_rows", "waves")}
+    choice = {k: best[k] for k in _KEYS}
     if use_cache:
         cache[key] = {**choice, "gcells": round(best["gcells"], 2)}
         _save_cache(cache)
     return (Tune(band_rows=choice["band_rows"], vec=choice["vec"], temporal=choice["temporal"],
-                 waves=choice["waves"]), {"key": key, "cached": False, "timings": timings,
+                 waves=choice["waves"], launch=choice["launch"]),
+            {"key": key, "cached": False, "timings": timings,
                                           "choice": choice})

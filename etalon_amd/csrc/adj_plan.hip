@@ -8,13 +8,18 @@
 #include <vector>
 
 #include "adj_tile.h"
+#include "coplan.h"
+#include "joint.h"
 #include "common.h"
 
 namespace etl {
 namespace {
 
-__global__ void set_adj_step_kernel(int* step, int value) {
-    if (threadIdx.x == 0) step[0] = value;
+__global__ void set_adj_step_kernel(int* step, int value, int hist_row0) {
+    if (threadIdx.x == 0) {
+        step[0] = value;
+        step[1] = hist_row0;
+    }
 }
 
 }  // namespace
@@ -26,6 +31,8 @@ struct EtlAdjPlan {
     int precision = 0;
     EtlTune tune{};
     int parity = 0;
+    unsigned layout_gen = 0;  // bumped by every layout rebuild (graphs hold its pointers)
+    int lane = 0;  // stream lane of its work
     struct Sim {
         EtlAdjointDesc desc{};
         int nx = 0, ny = 0;
@@ -91,7 +98,8 @@ int upload_bytes(void** dst, const void* src, size_t bytes) {
 // source or it owns box nodes, else strip.  Extra sub-tiles get the runs of sources
 // in their window: one run per (component, row) and consecutive columns.
 int build_adj_layout(EtlAdjPlan* p) {
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
+    ++p->layout_gen;
     free_adj_layout(p);
     free_adj_graph(p);
     const int T = p->tune.temporal, wo = p->wo(), W = 64;
@@ -266,11 +274,18 @@ int enqueue_adj(EtlAdjPlan* p, int count, int parity, int* launches) {
     const TileList tl{t, p->n_interior, t + p->n_interior, p->n_strip,
                       t + p->n_interior + p->n_strip, p->n_extra,
                       t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
-    hipStream_t s = ctx().stream;
+    hipStream_t s = lane_stream(p->lane);
     const int T = p->tune.temporal;
     int rtop = count - 1, n = 0;
     while (rtop >= 0) {
         const int depth = (rtop + 1 >= T) ? T : 1;
+        if (p->tune.launch == 1) {  // every tile kind in one kernel
+            const AdjSet<R> as = adj_set<R>(args, tl, p->step, rtop, parity ^ (n & 1));
+            if (int rc = launch_joint<R>(nullptr, 0, &as, depth, s)) return rc;
+            rtop -= depth;
+            ++n;
+            continue;
+        }
         ETL_HIP(hipEventRecord(p->ev_fork, s));
         ETL_HIP(hipStreamWaitEvent(p->s_strip, p->ev_fork, 0));
         ETL_HIP(hipStreamWaitEvent(p->s_extra, p->ev_fork, 0));
@@ -337,6 +352,61 @@ int adj_strips(const EtlAdjointDesc& d, int* out) {
 
 }  // namespace
 
+namespace {
+
+int adj_prepare_run(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0) {
+    ETL_REQUIRE(count >= 0 && n_first >= 0 && n_first < (int64_t(1) << 31),
+                "etl_adjplan_run: bad step range");
+    for (size_t k = 0; k < p->sims.size(); ++k) {
+        const auto& s = p->sims[k];
+        const bool box = s.desc.box_ni > 0 && s.desc.box_nj > 0;
+        ETL_REQUIRE(!box || (hist_row0 >= 0 && hist_row0 + count <= s.desc.hist_rows),
+                    "etl_adjplan_run: %d steps from history row %d, simulation %d's history "
+                    "holds %d", count, hist_row0, int(k), s.desc.hist_rows);
+        ETL_REQUIRE(s.comp.empty() || n_first + count <= s.src_rows,
+                    "etl_adjplan_run: simulation %d's source table covers steps [0, %d)", int(k),
+                    s.src_rows);
+    }
+    int rc;
+    if (p->dirty) {
+        if ((rc = build_adj_layout(p))) return rc;
+        if (int rc_ = sync_lanes()) return rc_;
+        if ((rc = p->precision == ETL_F32 ? upload_adj_args_t<float>(p) : upload_adj_args_t<double>(p)))
+            return rc;
+    }
+    hipStream_t s = lane_stream(p->lane);
+    hipLaunchKernelGGL(set_adj_step_kernel, dim3(1), dim3(64), 0, s, p->step, int(n_first),
+                       hist_row0);
+    ETL_LAUNCHED();
+    return ETL_OK;
+}
+
+}  // namespace
+
+namespace etl {
+
+int adj_prepare(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0) {
+    return adj_prepare_run(p, n_first, count, hist_row0);
+}
+
+void adj_info(const EtlAdjPlan* p, AdjInfo* out) {
+    const int4* t = p->tiles_dev;
+    out->precision = p->precision;
+    out->temporal = p->tune.temporal;
+    out->lane = p->lane;
+    out->args = p->args_dev;
+    out->tl = TileList{t, p->n_interior, t + p->n_interior, p->n_strip,
+                       t + p->n_interior + p->n_strip, p->n_extra,
+                       t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
+    out->step = p->step;
+    out->parity = p->parity;
+    out->gen = p->layout_gen;
+}
+
+void adj_set_parity(EtlAdjPlan* p, int parity) { p->parity = parity; }
+
+}  // namespace etl
+
 extern "C" {
 
 int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* tune,
@@ -356,6 +426,8 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
                 "etl_adjplan_create: temporal depth %d (1, 2 or 4)", t.temporal);
     ETL_REQUIRE(t.waves == 1 || t.waves == 2 || t.waves == 4,
                 "etl_adjplan_create: waves per workgroup must be 1, 2 or 4");
+    ETL_REQUIRE(t.launch == 0 || t.launch == 1 || t.launch == 3,
+                "etl_adjplan_create: launch must be 0, 1 or 3 (got %d)", t.launch);
     EtlAdjPlan* p = new EtlAdjPlan();
     p->precision = precision;
     p->tune = t;
@@ -397,6 +469,12 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
         if (rc == ETL_OK)
             rc = precision == ETL_F32 ? adj_strips<float>(d, s.strips) : adj_strips<double>(d, s.strips);
     }
+    if (rc == ETL_OK && p->tune.launch == 0) {
+        int64_t tiles = 0;
+        for (const auto& sim : p->sims)
+            tiles += int64_t((sim.nx + kAdjBand - 1) / kAdjBand) * ((sim.ny + 1 + p->wo() - 1) / p->wo());
+        p->tune.launch = tiles <= kSingleLaunchTiles ? 1 : 3;
+    }
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
@@ -422,7 +500,7 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
 int etl_adjplan_destroy(etl_adjplan_t p) {
     if (p == nullptr) return ETL_OK;
     ETL_CHECK_INIT();
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     destroy_adj_plan(p);
     return ETL_OK;
 }
@@ -451,7 +529,7 @@ int etl_adjplan_sources(etl_adjplan_t p, int sim, int n, const int* comp, const 
     for (int k = 1; k < n; ++k)
         ETL_REQUIRE(!(comp[order[k]] == comp[order[k - 1]] && flat[order[k]] == flat[order[k - 1]]),
                     "etl_adjplan_sources: node listed twice (sum its columns)");
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     s.comp.resize(n);
     s.flat.resize(n);
     s.perm = order;
@@ -478,31 +556,28 @@ int etl_adjplan_sources(etl_adjplan_t p, int sim, int n, const int* comp, const 
 }
 
 int etl_adjplan_run(etl_adjplan_t p, int64_t n_first, int count) {
+    return etl_adjplan_run_rows(p, n_first, count, 0);
+}
+
+int etl_adjplan_set_lane(etl_adjplan_t p, int lane) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr && lane >= 0 && lane < kLanes, "etl_adjplan_set_lane: bad arguments");
+    if (lane != p->lane) {
+        if (int rc = sync_lanes()) return rc;
+        p->lane = lane;
+    }
+    return ETL_OK;
+}
+
+int etl_adjplan_run_rows(etl_adjplan_t p, int64_t n_first, int count, int hist_row0) {
     ETL_CHECK_INIT();
     ETL_REQUIRE(p != nullptr, "etl_adjplan_run: null plan");
     ETL_REQUIRE(count >= 0 && n_first >= 0 && n_first < (int64_t(1) << 31),
                 "etl_adjplan_run: bad step range");
     if (count == 0) return ETL_OK;
-    for (size_t k = 0; k < p->sims.size(); ++k) {
-        const auto& s = p->sims[k];
-        const bool box = s.desc.box_ni > 0 && s.desc.box_nj > 0;
-        ETL_REQUIRE(!box || count <= s.desc.hist_rows,
-                    "etl_adjplan_run: %d steps, simulation %d's history holds %d", count, int(k),
-                    s.desc.hist_rows);
-        ETL_REQUIRE(s.comp.empty() || n_first + count <= s.src_rows,
-                    "etl_adjplan_run: simulation %d's source table covers steps [0, %d)", int(k),
-                    s.src_rows);
-    }
     int rc;
-    if (p->dirty) {
-        if ((rc = build_adj_layout(p))) return rc;
-        ETL_HIP(hipStreamSynchronize(ctx().stream));
-        if ((rc = p->precision == ETL_F32 ? upload_adj_args_t<float>(p) : upload_adj_args_t<double>(p)))
-            return rc;
-    }
-    hipStream_t s = ctx().stream;
-    hipLaunchKernelGGL(set_adj_step_kernel, dim3(1), dim3(64), 0, s, p->step, int(n_first));
-    ETL_LAUNCHED();
+    if ((rc = adj_prepare_run(p, n_first, count, hist_row0))) return rc;
+    hipStream_t s = lane_stream(p->lane);
     if (p->graph == nullptr || p->graph_count != count || p->graph_parity != p->parity) {
         free_adj_graph(p);
         ETL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -64,17 +64,31 @@ __device__ __forceinline__ void add_run(R (&f)[RWR][V], int lr, int c, int c_lo,
     }
 }
 
+// add_run with the values already in registers (v[k] = the run's value at column c + k,
+// +0 outside it): the same sums.
+template <typename R, int V, int RWR>
+__device__ __forceinline__ void add_vals(R (&f)[RWR][V], int lr, const R (&v)[V]) {
+#pragma unroll
+    for (int l = 0; l < RWR; ++l) {
+        if (l != lr) continue;  // uniform
+#pragma unroll
+        for (int k = 0; k < V; ++k) f[l][k] = f[l][k] + v[k];
+    }
+}
+
+// Source runs of an extra tile whose values are loaded before the step loop.
+constexpr int kRunRegs = 2;
+
+// The reverse-step tile program of one wave: tile li = ((b & 7) * per_xcd + (b >> 3)) *
+// waves + wave of `tiles`, for workgroup b of n_groups (XCD-major, as the forward tiles).
 template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
-__global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __restrict__ args,
-                                                       const int4* __restrict__ tiles, int n_tiles,
-                                                       const int* __restrict__ step, int rtop,
-                                                       int parity) {
+__device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__ args,
+                                              const int4* __restrict__ tiles, int n_tiles, int b,
+                                              int n_groups, int waves, int wave,
+                                              const int* __restrict__ step, int rtop,
+                                              int parity) {
     constexpr int RW = BR + 2 * T;
-    const int waves = blockDim.x >> 6;
-    const int n_groups = (n_tiles + waves - 1) / waves;
-    const int b = blockIdx.x;
     const int per_xcd = (n_groups + 7) >> 3;
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int li = ((b & 7) * per_xcd + (b >> 3)) * waves + wave;
     if (li >= n_tiles) return;
     const int4 td = tiles[li];  // first row, tile column, extra-tile index, simulation
@@ -178,13 +192,61 @@ __global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __r
 #pragma unroll
             for (int k = 0; k < V; ++k) gxa[l][k] = gya[l][k] = 0.0;
     }
-    const int n_first = *step;
+    const int n_first = step[0];
+    const int hrow0 = step[1];  // history row of step n_first
+
+    // Loaded up front, so that no step waits on memory: the first kRunRegs source runs
+    // (descriptor and the T steps' values) and the forward Hz history of the box rows
+    // i0 - 1 .. i0 + BR - 1 at this lane's columns for the T steps.
+    int n_cached = 0;
+    int4v rd[kRunRegs];
+    R rv[EXTRA ? kRunRegs : 1][T][V];
+    R hv[EXTRA ? T : 1][EXTRA ? BR + 1 : 1][V];
+    const int hw = a.bnj + 1;
+    if constexpr (EXTRA) {
+        n_cached = min(run_hi - run_lo, kRunRegs);
+#pragma unroll
+        for (int qq = 0; qq < kRunRegs; ++qq) {
+            rd[qq] = int4v{0, 0, 0, 0};
+            if (qq < n_cached) rd[qq] = to_global_v(a.runs)[run_lo + qq];
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const int j = c + k;
+                    const bool in = qq < n_cached && j >= rd[qq].y && j < rd[qq].z;
+                    rv[qq][t][k] = in ? to_global(a.src)[int64_t(n_first + rtop - t) * a.n_src +
+                                                         rd[qq].w + (j - rd[qq].y)]
+                                      : R(0);
+                }
+        }
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int m = 0; m <= BR; ++m)
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const int hr = i0 - 1 + m - a.bi0 + 1, hc = c + k - a.bj0 + 1;
+                    const bool ok = boxed && hr >= 0 && hr <= a.bni && hc >= 0 && hc <= a.bnj;
+                    hv[t][m][k] = ok ? to_global(a.hist)[int64_t(hrow0 + rtop - t) *
+                                                             (a.bni + 1) * hw + hr * hw + hc]
+                                     : R(0);
+                }
+    }
 
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int r = rtop - t;  // history row and table row n_first + r of this reverse step
+        const int r = rtop - t;  // table row n_first + r, history row hrow0 + r of this step
         if constexpr (EXTRA) {
-            for (int q = run_lo; q < run_hi; ++q) {
+#pragma unroll
+            for (int qq = 0; qq < kRunRegs; ++qq) {
+                if (qq >= n_cached) break;  // uniform
+                const int comp = rd[qq].x >> 28, lr = (rd[qq].x & 0x0fffffff) - g0;
+                if (comp == 0) add_vals<R, V, RW>(ex, lr, rv[qq][t]);
+                else if (comp == 1) add_vals<R, V, RW + 1>(ey, lr, rv[qq][t]);
+                else add_vals<R, V, RW>(hz, lr, rv[qq][t]);
+            }
+            for (int q = run_lo + n_cached; q < run_hi; ++q) {
                 const int4v ru = to_global_v(a.runs)[q];
                 const int comp = ru.x >> 28, g = ru.x & 0x0fffffff;
                 const global_t<const R>* vals =
@@ -265,9 +327,6 @@ __global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __r
             }
         }
         // ---- B: Ex rows [t, RW - t), Ey rows [t + 1, RW - t)
-        const global_t<const R>* hrow =
-            EXTRA && boxed ? to_global(a.hist) + int64_t(r) * (a.bni + 1) * (a.bnj + 1) : nullptr;
-        const int hw = a.bnj + 1;
 #pragma unroll
         for (int l = t; l < RW - t; ++l) {
             const int g = g0 + l;
@@ -277,6 +336,10 @@ __global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __r
             R bxe = R(1);
             if (STRIP && xe) bxe = lane_value(pbxe, l);
             const bool owned = EXTRA && l >= T && l < T + BR;
+            // history of this row (m) and the one above (m - 1), and column j - 1
+            const int m = owned ? l - T + 1 : 1;
+            R h_left = R(0);
+            if (owned && boxed) h_left = from_prev_lane(hv[EXTRA ? t : 0][EXTRA ? m : 0][V - 1]);
 #pragma unroll
             for (int k = 0; k < V; ++k) {
                 const int j = c + k;
@@ -284,8 +347,10 @@ __global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __r
                 if (STRIP && yse[k]) pey[STRIP ? l : 0][k] = bye[k] * (pey[STRIP ? l : 0][k] + cx[l][k] * lex);
                 if (owned && boxed && mEx[k] && g >= a.bi0 && g < a.bi0 + a.bni && j >= a.bj0 &&
                     j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
-                    const int hb = (g - a.bi0 + 1) * hw + (j - a.bj0 + 1);
-                    const double u = (double(hrow[hb]) - double(hrow[hb - 1])) * double(idy);
+                    const R h0 = (k > 0) ? hv[EXTRA ? t : 0][EXTRA ? m : 0][k > 0 ? k - 1 : 0]
+                                         : h_left;
+                    const double u =
+                        (double(hv[EXTRA ? t : 0][EXTRA ? m : 0][k]) - double(h0)) * double(idy);
                     gxa[owned ? l - T : 0][k] += double(lex) * u;
                 }
                 const R ldp = (k > 0) ? ld[l][k - 1] : ld_l;
@@ -295,8 +360,9 @@ __global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __r
                     if (STRIP && xe) pex[STRIP ? l : 0][k] = bxe * (pex[STRIP ? l : 0][k] - cy[l][k] * ley);
                     if (owned && boxed && g >= a.bi0 && g < a.bi0 + a.bni && g >= 1 && g <= nx - 1 &&
                         j >= a.bj0 && j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
-                        const int hb = (g - a.bi0 + 1) * hw + (j - a.bj0 + 1);
-                        const double w = (double(hrow[hb]) - double(hrow[hb - hw])) * double(idx);
+                        const double w = (double(hv[EXTRA ? t : 0][EXTRA ? m : 0][k]) -
+                                          double(hv[EXTRA ? t : 0][EXTRA ? m - 1 : 0][k])) *
+                                         double(idx);
                         gya[owned ? l - T : 0][k] -= double(ley) * w;
                     }
                     ey[l][k] = ley + idx * (le[l - 1][k] - le[l][k]);
@@ -345,6 +411,18 @@ __global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __r
             }
         }
     }
+}
+
+template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
+__global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __restrict__ args,
+                                                       const int4* __restrict__ tiles, int n_tiles,
+                                                       const int* __restrict__ step, int rtop,
+                                                       int parity) {
+    const int waves = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    adj_tile_body<R, V, T, BR, STRIP, EXTRA>(args, tiles, n_tiles, blockIdx.x,
+                                             (n_tiles + waves - 1) / waves, waves, wave, step,
+                                             rtop, parity);
 }
 
 // One adjoint launch of depth T over all tiles (vec 1, 8-row interior tiles).

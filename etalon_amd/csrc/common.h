@@ -19,9 +19,16 @@ namespace etl {
 // Last error message of the calling thread.
 void set_error(const char* fmt, ...);
 
+// Streams of the library: lane 0 (`stream`) carries every call that names no lane;
+// plans may be moved to lane 1 so that two plans run concurrently
+// (etl_plan_set_lane, etl_adjplan_set_lane), ordered by etl_lane_wait.
+constexpr int kLanes = 2;
+
 struct Context {
     int device = -1;
     hipStream_t stream = nullptr;
+    hipStream_t lane[kLanes] = {nullptr};
+    hipEvent_t lane_ev[kLanes] = {nullptr};
     int cu_count = 0;
     char name[256] = {0};
     char arch[64] = {0};
@@ -29,6 +36,14 @@ struct Context {
 };
 Context& ctx();
 int require_init();
+// The stream of lane `k` (0 <= k < kLanes).
+inline hipStream_t lane_stream(int k) { return ctx().lane[k]; }
+// Wait on the host for the work of every lane.
+int sync_lanes();
+// Make every lane's later work wait for every lane's earlier work (no host wait).
+int join_lanes();
+// Make lane k's later work wait for the earlier work of the other lanes.
+int wait_other_lanes(int k);
 // Stream-ordered zero fill with a kernel (safe inside hipGraph capture).
 int zero_async(void* dst, size_t bytes);
 // Valid device memory that etl_malloc places before and after every array.

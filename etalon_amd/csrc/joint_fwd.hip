@@ -1,0 +1,26 @@
+// One-launch forward step blocks (joint.h), float and double, depths 1, 2, 4.
+#include "joint.h"
+
+namespace etl {
+
+template <typename R>
+int launch_joint_fwd_any(const FwdSet<R>* f, int tf, hipStream_t s) {
+    switch (tf) {
+        case 1: return launch_joint_t<R, 1, 0>(f, nullptr, s);
+        case 2: return launch_joint_t<R, 2, 0>(f, nullptr, s);
+        case 4: return launch_joint_t<R, 4, 0>(f, nullptr, s);
+    }
+    set_error("joint step: forward depth %d not compiled", tf);
+    return ETL_EARG;
+}
+
+template <>
+int launch_joint_fwd<float>(const FwdSet<float>* f, int tf, hipStream_t s) {
+    return launch_joint_fwd_any<float>(f, tf, s);
+}
+template <>
+int launch_joint_fwd<double>(const FwdSet<double>* f, int tf, hipStream_t s) {
+    return launch_joint_fwd_any<double>(f, tf, s);
+}
+
+}  // namespace etl

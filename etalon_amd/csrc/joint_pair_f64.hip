@@ -1,0 +1,18 @@
+// One-launch step blocks of a forward and an adjoint plan together (joint.h),
+// double, depths 1, 2, 4.
+#include "joint.h"
+
+namespace etl {
+
+template <>
+int launch_joint_pair<double>(const FwdSet<double>* f, const AdjSet<double>* a, int t, hipStream_t s) {
+    switch (t) {
+        case 1: return launch_joint_t<double, 1, 1>(f, a, s);
+        case 2: return launch_joint_t<double, 2, 2>(f, a, s);
+        case 4: return launch_joint_t<double, 4, 4>(f, a, s);
+    }
+    set_error("joint step: depth %d not compiled", t);
+    return ETL_EARG;
+}
+
+}  // namespace etl

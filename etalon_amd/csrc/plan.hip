@@ -15,15 +15,37 @@
 
 #include "common.h"
 #include "fused.h"
+#include "coplan.h"
+#include "joint.h"
 
 namespace etl {
 namespace {
 
-__global__ void set_counter_kernel(int* counter, int value) {
+__global__ void set_counter_kernel(int* counter, int value, int probe_row0) {
     if (threadIdx.x == 0) {
         counter[0] = value;
-        counter[1] = 0;      // block tally of the DFT kernel
-        counter[2] = value;  // first step of this advance: probe rows count from it
+        counter[1] = 0;                   // block tally of the DFT kernel
+        counter[2] = value - probe_row0;  // probe row of step counter[0]: counter[0] - counter[2]
+    }
+}
+
+// Checkpoint copies: entry e (one field of one simulation) copies `words` 32-bit
+// words between buffer `parity` of the field and slot `seg` of its checkpoint array.
+struct CkEntry {
+    uint32_t* ck;
+    uint32_t* f[2];
+    int64_t words;
+};
+
+__global__ void ckpt_copy_kernel(const CkEntry* __restrict__ tab, int seg, int parity,
+                                 int restore) {
+    const CkEntry e = tab[blockIdx.y];
+    uint32_t* __restrict__ ck = e.ck + seg * e.words;
+    uint32_t* __restrict__ f = e.f[parity];
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < e.words; i += stride) {
+        if (restore) f[i] = ck[i];
+        else ck[i] = f[i];
     }
 }
 
@@ -52,6 +74,8 @@ struct EtlPlan {
     int precision = 0;
     EtlTune tune{};
     int parity = 0;
+    unsigned layout_gen = 0;  // bumped by every layout rebuild (graphs hold its pointers)
+    int lane = 0;  // stream lane of its work
 
     // One simulation of the plan.
     struct Mon {
@@ -99,6 +123,10 @@ struct EtlPlan {
         void* ph = nullptr;                               // (ph_rows_cap, nq) complex
         int ph_rows_cap = 0;
 
+        // checkpoints: ck[k] = (ck_seg, size of field k) REALs
+        void* ck[7] = {nullptr};
+        int ck_seg = 0;
+
         int rows_of(int comp) const { return comp == 1 ? nx + 1 : nx; }
         int pitch_of(int comp) const { return comp == 0 ? ny + 1 : ny; }
     };
@@ -117,6 +145,10 @@ struct EtlPlan {
     int n_pairs = 0;
     hipStream_t s_strip = nullptr, s_extra = nullptr;  // beside the interior tiles
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
+
+    CkEntry* ck_tab = nullptr;  // checkpoint copy entries of every simulation
+    int ck_entries = 0;
+    bool ck_stale = true;
 
     int* counter = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -175,7 +207,8 @@ int upload(void** dst, const void* src, size_t bytes) {
 // otherwise it is split into kSubBand-row sub-tiles, each extra when it owns a
 // monitored node or its own window holds a source, else strip.
 int build_layout(EtlPlan* p) {
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
+    ++p->layout_gen;
     free_layout(p);
     free_graphs(p);
     const int wo = p->wo();
@@ -386,7 +419,7 @@ FusedArgs<R> make_args(const EtlPlan* p, const EtlPlan::Sim& s) {
 // valid.
 template <typename R>
 int upload_args_t(EtlPlan* p) {
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     std::vector<FusedArgs<R>> args;
     std::vector<TraceSim<R>> traces;
     for (const auto& s : p->sims) {
@@ -413,10 +446,28 @@ int upload_args(EtlPlan* p) {
 // number of fused launches in *launches.  `events` (optional) receives four events
 // per fused launch: fork, interior kernel start and end (on the main stream, where
 // only that kernel runs), join.
+// After `count` steps of a trace chunk: probe copies, then the DFT fold of the trace
+// rows (which advances the step counter) or a counter bump.
+template <typename R>
+int after_chunk(EtlPlan* p, int count, hipStream_t s) {
+    for (const auto& sim : p->sims)
+        for (const auto& pr : sim.probes) {
+            const int rc = launch_probe_copy<R>((const R*)sim.trace, sim.n_slots, pr.slot_base,
+                                                int(pr.flat.size()), (R*)pr.out, count,
+                                                p->counter, s);
+            if (rc) return rc;
+        }
+    if (p->n_pairs && count)
+        return launch_trace_dft<R>(static_cast<const TraceSim<R>*>(p->trace_sims_dev), count,
+                                   p->counter, p->pairs, p->pair_sim, p->n_pairs,
+                                   (R* const*)p->accs, s);
+    return launch_bump(p->counter, count, s);
+}
+
 template <typename R>
 int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
     const FusedArgs<R>* args = static_cast<const FusedArgs<R>*>(p->args_dev);
-    hipStream_t s = ctx().stream;
+    hipStream_t s = lane_stream(p->lane);
     const int4* t = p->tiles_dev;
     const TileList tl{t, p->n_interior, t + p->n_interior, p->n_strip,
                       t + p->n_interior + p->n_strip, p->n_extra,
@@ -425,6 +476,21 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
     int k = 0, n = 0;
     while (k < count) {
         const int depth = (count - k >= T) ? T : 1;
+        if (p->tune.launch == 1) {  // every tile kind in one kernel
+            if (events) {
+                ETL_HIP(hipEventRecord(events[4 * n], s));
+                ETL_HIP(hipEventRecord(events[4 * n + 1], s));
+            }
+            const FwdSet<R> fs = fwd_set<R>(args, tl, p->counter, k, parity ^ (n & 1));
+            if (int rc = launch_joint<R>(&fs, depth, nullptr, 0, s)) return rc;
+            if (events) {
+                ETL_HIP(hipEventRecord(events[4 * n + 2], s));
+                ETL_HIP(hipEventRecord(events[4 * n + 3], s));
+            }
+            k += depth;
+            ++n;
+            continue;
+        }
         if (events) ETL_HIP(hipEventRecord(events[4 * n], s));
         // fork: strip and extra tiles run on their own streams, joined before the next step
         ETL_HIP(hipEventRecord(p->ev_fork, s));
@@ -444,18 +510,7 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
         ++n;
     }
     *launches = n;
-    for (const auto& sim : p->sims)
-        for (const auto& pr : sim.probes) {
-            const int rc = launch_probe_copy<R>((const R*)sim.trace, sim.n_slots, pr.slot_base,
-                                                int(pr.flat.size()), (R*)pr.out, count,
-                                                p->counter, s);
-            if (rc) return rc;
-        }
-    if (p->n_pairs && count)
-        return launch_trace_dft<R>(static_cast<const TraceSim<R>*>(p->trace_sims_dev), count,
-                                   p->counter, p->pairs, p->pair_sim, p->n_pairs,
-                                   (R* const*)p->accs, s);
-    return launch_bump(p->counter, count, s);
+    return after_chunk<R>(p, count, s);
 }
 
 int enqueue(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
@@ -476,7 +531,7 @@ int upload_tables(EtlPlan* p) {
     bool any = false;
     for (const auto& s : p->sims) any = any || s.delta_stale || s.ph_stale;
     if (!any) return ETL_OK;
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     int rc;
     for (auto& s : p->sims) {
         if (s.delta_stale) {
@@ -513,7 +568,7 @@ int upload_tables(EtlPlan* p) {
 
 int ensure_graph(EtlPlan* p, int parity) {
     if (p->graph[parity]) return ETL_OK;
-    hipStream_t s = ctx().stream;
+    hipStream_t s = lane_stream(p->lane);
     ETL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int launches = 0;
     int rc = enqueue(p, p->tune.trace_chunk, parity, nullptr, &launches);
@@ -573,7 +628,7 @@ int check_tables(EtlPlan* p, int64_t n, int64_t count, int64_t* n0) {
     return ETL_OK;
 }
 
-int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_launch) {
+int prepare_steps(EtlPlan* p, int64_t n, int64_t count, int probe_row0) {
     int rc;
     if (p->dirty) {
         if ((rc = build_layout(p))) return rc;
@@ -586,14 +641,23 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_l
     }
     for (const auto& sim : p->sims)
         for (const auto& pr : sim.probes)
-            ETL_REQUIRE(count <= pr.out_rows,
-                        "etl_plan_advance: %lld steps, a probe holds %d rows", (long long)count,
-                        pr.out_rows);
+            ETL_REQUIRE(probe_row0 >= 0 && probe_row0 + count <= pr.out_rows,
+                        "etl_plan_advance: %lld steps from probe row %d, a probe holds %d rows",
+                        (long long)count, probe_row0, pr.out_rows);
     const int64_t start = n - n0;
     ETL_REQUIRE(start >= 0 && start + count < (int64_t(1) << 31), "etl_plan_advance: step range");
-    hipStream_t s = ctx().stream;
-    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(64), 0, s, p->counter, int(start));
+    hipStream_t s = lane_stream(p->lane);
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(64), 0, s, p->counter, int(start),
+                       probe_row0);
     ETL_LAUNCHED();
+    return ETL_OK;
+}
+
+int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_launch,
+              int probe_row0 = 0) {
+    int rc;
+    if ((rc = prepare_steps(p, n, count, probe_row0))) return rc;
+    hipStream_t s = lane_stream(p->lane);
     const int K = p->tune.trace_chunk;
     int64_t done = 0;
     int total = 0;
@@ -615,6 +679,48 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_l
         total += launches;
     }
     if (n_launch) *n_launch = total;
+    return ETL_OK;
+}
+
+// Elements of field k (ex ey hz psi_ex_y psi_ey_x psi_hz_x psi_hz_y) of a simulation.
+int64_t field_elems(const EtlPlan::Sim& s, int k) {
+    static const int comp_of[7] = {0, 1, 2, 0, 1, 2, 2};
+    return int64_t(s.rows_of(comp_of[k])) * s.pitch_of(comp_of[k]);
+}
+
+// (Re)upload the checkpoint copy entries of every simulation.
+int ck_upload(EtlPlan* p) {
+    std::vector<CkEntry> tab;
+    for (const auto& sim : p->sims)
+        for (int k = 0; k < 7; ++k)
+            tab.push_back(CkEntry{static_cast<uint32_t*>(sim.ck[k]),
+                                  {static_cast<uint32_t*>(sim.desc.field[k][0]),
+                                   static_cast<uint32_t*>(sim.desc.field[k][1])},
+                                  field_elems(sim, k) * p->elem() / 4});
+    etl_free(p->ck_tab);
+    p->ck_tab = nullptr;
+    if (int rc = upload((void**)&p->ck_tab, tab.data(), tab.size() * sizeof(CkEntry))) return rc;
+    p->ck_entries = int(tab.size());
+    p->ck_stale = false;
+    return ETL_OK;
+}
+
+// Save the state into (restore = 0) or load it from (restore = 1) checkpoint slot `seg`
+// of every simulation, on the plan's lane, in the buffers of the current parity.
+int ck_copy(EtlPlan* p, int seg, int restore) {
+    for (size_t k = 0; k < p->sims.size(); ++k)
+        ETL_REQUIRE(p->sims[k].ck_seg > seg && seg >= 0,
+                    "checkpoint %d: simulation %d has %d checkpoint slots", seg, int(k),
+                    p->sims[k].ck_seg);
+    if (p->ck_stale)
+        if (int rc = ck_upload(p)) return rc;
+    int64_t words = 0;
+    for (const auto& sim : p->sims)
+        for (int k = 0; k < 7; ++k) words = std::max(words, field_elems(sim, k) * p->elem() / 4);
+    const int gx = int(std::min<int64_t>(1024, (words + 2047) / 2048));
+    hipLaunchKernelGGL(ckpt_copy_kernel, dim3(unsigned(gx), unsigned(p->ck_entries)), dim3(256),
+                       0, lane_stream(p->lane), p->ck_tab, seg, p->parity, restore);
+    ETL_LAUNCHED();
     return ETL_OK;
 }
 
@@ -726,6 +832,7 @@ void destroy_plan(EtlPlan* p) {
     etl_free(p->args_dev);
     etl_free(p->trace_sims_dev);
     etl_free(p->counter);
+    etl_free(p->ck_tab);
     if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     if (p->ev_join) (void)hipEventDestroy(p->ev_join);
     if (p->ev_join2) (void)hipEventDestroy(p->ev_join2);
@@ -743,6 +850,41 @@ EtlPlan::Sim* sim_of(EtlPlan* p, int sim) {
 }
 
 }  // namespace
+
+namespace etl {
+
+int fwd_prepare(EtlPlan* p, int64_t n, int64_t count, int probe_row0) {
+    return prepare_steps(p, n, count, probe_row0);
+}
+
+void fwd_info(const EtlPlan* p, FwdInfo* out) {
+    const int4* t = p->tiles_dev;
+    out->precision = p->precision;
+    out->temporal = p->tune.temporal;
+    out->trace_chunk = p->tune.trace_chunk;
+    out->launch = p->tune.launch;
+    out->lane = p->lane;
+    out->vec = p->tune.vec;
+    out->band = p->tune.band_rows;
+    out->args = p->args_dev;
+    out->tl = TileList{t, p->n_interior, t + p->n_interior, p->n_strip,
+                       t + p->n_interior + p->n_strip, p->n_extra,
+                       t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
+    out->counter = p->counter;
+    out->parity = p->parity;
+    out->gen = p->layout_gen;
+}
+
+void fwd_set_parity(EtlPlan* p, int parity) { p->parity = parity; }
+
+int fwd_checkpoint(EtlPlan* p, int seg, int restore) { return ck_copy(p, seg, restore); }
+
+int fwd_after_chunk(EtlPlan* p, int rows, hipStream_t s) {
+    return p->precision == ETL_F32 ? after_chunk<float>(p, rows, s)
+                                   : after_chunk<double>(p, rows, s);
+}
+
+}  // namespace etl
 
 extern "C" {
 
@@ -771,6 +913,10 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
                 "etl_plan_create: trace_chunk must be a multiple of 2 * temporal, <= 4096");
     ETL_REQUIRE(t.waves == 1 || t.waves == 2 || t.waves == 4,
                 "etl_plan_create: waves per workgroup must be 1, 2 or 4 (got %d)", t.waves);
+    ETL_REQUIRE(t.launch == 0 || t.launch == 1 || t.launch == 3,
+                "etl_plan_create: launch must be 0, 1 or 3 (got %d)", t.launch);
+    ETL_REQUIRE(t.launch != 1 || (t.vec == 1 && t.band_rows == 8),
+                "etl_plan_create: one launch per step block needs vec 1 and band 8");
 
     EtlPlan* p = new EtlPlan();
     p->precision = precision;
@@ -778,6 +924,11 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     p->sims.resize(n_sims);
     int rc = ETL_OK;
     for (int k = 0; k < n_sims && rc == ETL_OK; ++k) rc = init_sim(p, &descs[k], p->sims[k]);
+    if (rc == ETL_OK && p->tune.launch == 0) {
+        int64_t tiles = 0;
+        for (const auto& sim : p->sims) tiles += int64_t(p->n_bands(sim)) * p->n_tj(sim);
+        p->tune.launch = (t.vec == 1 && t.band_rows == 8 && tiles <= kSingleLaunchTiles) ? 1 : 3;
+    }
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
@@ -810,7 +961,7 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
 int etl_plan_destroy(etl_plan_t p) {
     if (p == nullptr) return ETL_OK;
     ETL_CHECK_INIT();
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     destroy_plan(p);
     return ETL_OK;
 }
@@ -837,7 +988,7 @@ int etl_plan_set_sources_sim(etl_plan_t p, int sim, int n_nodes, const int* comp
         ETL_REQUIRE(!outer, "etl_plan_set_sources: node on the outer tangential-E boundary");
         src.push_back(make_int4(comp[k], int(i[k]), int(j[k]), 0));
     }
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     etl_free(s->src_dev);
     s->src_dev = nullptr;
     if (n_nodes) {
@@ -901,7 +1052,7 @@ int etl_plan_add_monitor_sim(etl_plan_t p, int sim, int comp, int n_nodes,
         ETL_REQUIRE(!(comp == 1 && row == s->nx),
                     "etl_plan_add_monitor: node on the outer tangential-E boundary");
     }
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     s->mons.push_back(m);
     p->dirty = true;
     *id = int(s->mons.size()) - 1;
@@ -941,7 +1092,7 @@ int etl_plan_add_probe_sim(etl_plan_t p, int sim, int comp, int n_nodes,
                 "etl_plan_add_probe: out must be an etl_malloc allocation of out_rows x n_nodes");
     pr.out = out;
     pr.out_rows = out_rows;
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     s->probes.push_back(std::move(pr));
     p->dirty = true;
     p->args_stale = true;
@@ -994,7 +1145,7 @@ int etl_plan_share_phasors(etl_plan_t p, int sim, int id, int owner) {
     ETL_REQUIRE(s->mons[owner].owner < 0 && s->mons[owner].n_f == s->mons[id].n_f,
                 "etl_plan_share_phasors: monitor %d must own a table of %d frequencies", owner,
                 s->mons[id].n_f);
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc_ = sync_lanes()) return rc_;
     s->mons[id].owner = owner;
     // the combined table loses monitor id's columns: drop it
     etl_free(s->ph);
@@ -1021,14 +1172,57 @@ int etl_plan_tune(etl_plan_t p, EtlTune* out) {
 }
 
 int etl_plan_advance(etl_plan_t p, int64_t n, int64_t count, int* parity) {
+    return etl_plan_advance_rows(p, n, count, 0, parity);
+}
+
+int etl_plan_advance_rows(etl_plan_t p, int64_t n, int64_t count, int probe_row0, int* parity) {
     ETL_CHECK_INIT();
     ETL_REQUIRE(p != nullptr, "etl_plan_advance: null plan");
     ETL_REQUIRE(n >= 0 && count >= 0, "etl_plan_advance: bad step range");
     if (count > 0) {
-        int rc = run_steps(p, n, count, nullptr, nullptr);
+        int rc = run_steps(p, n, count, nullptr, nullptr, probe_row0);
         if (rc) return rc;
     }
     if (parity) *parity = p->parity;
+    return ETL_OK;
+}
+
+int etl_plan_set_checkpoints_sim(etl_plan_t p, int sim, void* const* ptr, int n_seg) {
+    ETL_CHECK_INIT();
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr && ptr != nullptr && n_seg >= 1,
+                "etl_plan_set_checkpoints: bad arguments (simulation %d, %d slots)", sim, n_seg);
+    for (int k = 0; k < 7; ++k)
+        ETL_REQUIRE(ptr[k] != nullptr && guarded_bytes(ptr[k]) >=
+                        size_t(n_seg) * size_t(field_elems(*s, k)) * p->elem(),
+                    "etl_plan_set_checkpoints: field %d needs an etl_malloc array of %d x %lld "
+                    "REALs", k, n_seg, (long long)field_elems(*s, k));
+    for (int k = 0; k < 7; ++k) s->ck[k] = ptr[k];
+    s->ck_seg = n_seg;
+    p->ck_stale = true;
+    return ETL_OK;
+}
+
+int etl_plan_advance_checkpointed(etl_plan_t p, int64_t n, int64_t count, int K, int* parity) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr && K >= 1 && n >= 0 && count >= 0 && n % K == 0,
+                "etl_plan_advance_checkpointed: bad arguments (n %lld, K %d)", (long long)n, K);
+    for (int64_t c = n; c < n + count; c += K) {
+        if (int rc = ck_copy(p, int(c / K), 0)) return rc;
+        if (int rc = run_steps(p, c, std::min<int64_t>(K, n + count - c), nullptr, nullptr))
+            return rc;
+    }
+    if (parity) *parity = p->parity;
+    return ETL_OK;
+}
+
+int etl_plan_set_lane(etl_plan_t p, int lane) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr && lane >= 0 && lane < kLanes, "etl_plan_set_lane: bad arguments");
+    if (lane != p->lane) {
+        if (int rc = sync_lanes()) return rc;
+        p->lane = lane;
+    }
     return ETL_OK;
 }
 
@@ -1042,7 +1236,7 @@ int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, EtlTiming* out, int* p
     hipEvent_t t0, t1;
     ETL_HIP(hipEventCreate(&t0));
     ETL_HIP(hipEventCreate(&t1));
-    hipStream_t s = ctx().stream;
+    hipStream_t s = lane_stream(p->lane);
     ETL_HIP(hipEventRecord(t0, s));
     int launches = 0;
     int rc = run_steps(p, n, count, ev.data(), &launches);

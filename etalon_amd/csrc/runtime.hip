@@ -31,6 +31,29 @@ Context& ctx() {
     return c;
 }
 
+int sync_lanes() {
+    for (int k = 0; k < kLanes; ++k)
+        if (ctx().lane[k]) ETL_HIP(hipStreamSynchronize(ctx().lane[k]));
+    return ETL_OK;
+}
+
+int join_lanes() {
+    for (int k = 0; k < kLanes; ++k) ETL_HIP(hipEventRecord(ctx().lane_ev[k], ctx().lane[k]));
+    for (int k = 0; k < kLanes; ++k)
+        for (int m = 0; m < kLanes; ++m)
+            if (m != k) ETL_HIP(hipStreamWaitEvent(ctx().lane[k], ctx().lane_ev[m], 0));
+    return ETL_OK;
+}
+
+int wait_other_lanes(int k) {
+    for (int m = 0; m < kLanes; ++m)
+        if (m != k) {
+            ETL_HIP(hipEventRecord(ctx().lane_ev[m], ctx().lane[m]));
+            ETL_HIP(hipStreamWaitEvent(ctx().lane[k], ctx().lane_ev[m], 0));
+        }
+    return ETL_OK;
+}
+
 int require_init() {
     if (ctx().stream == nullptr) {
         set_error("libetalon_hip: etl_init() has not been called");
@@ -140,6 +163,11 @@ int etl_init(int device) {
     c.cu_count = prop.multiProcessorCount;
     c.mem = prop.totalGlobalMem;
     ETL_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.lane[0] = c.stream;
+    for (int k = 1; k < kLanes; ++k)
+        ETL_HIP(hipStreamCreateWithFlags(&c.lane[k], hipStreamNonBlocking));
+    for (int k = 0; k < kLanes; ++k)
+        ETL_HIP(hipEventCreateWithFlags(&c.lane_ev[k], hipEventDisableTiming));
     c.device = device;
     return ETL_OK;
 }
@@ -172,7 +200,7 @@ int etl_malloc(void** ptr, size_t bytes) {
     if (e != hipSuccess) {
         // Release the cache and retry once.
         std::lock_guard<std::mutex> lock(g_pool_mutex);
-        ETL_HIP(hipStreamSynchronize(ctx().stream));
+        if (int rc = sync_lanes()) return rc;
         for (auto& kv : g_free) (void)hipFree(kv.second);
         g_free.clear();
         g_cached_bytes = 0;
@@ -197,13 +225,14 @@ int etl_free(void* ptr) {
     const Block blk = it->second;
     g_live.erase(it);
     if (g_cached_bytes + blk.size <= kCacheLimit) {
-        // Stream-ordered reuse: every later use of this block is issued on the same
-        // stream, after the work that used it before.
+        // Stream-ordered reuse: every later use of this block is issued, on either
+        // lane, after the work that used it before (the lanes are joined here).
+        if (int rc = join_lanes()) return rc;
         g_free.emplace(blk.size, blk.base);
         g_cached_bytes += blk.size;
         return ETL_OK;
     }
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    if (int rc = sync_lanes()) return rc;
     ETL_HIP(hipFree(blk.base));
     return ETL_OK;
 }
@@ -212,6 +241,7 @@ int etl_memcpy_h2d(void* dst, const void* src, size_t bytes) {
     ETL_CHECK_INIT();
     if (bytes == 0) return ETL_OK;
     ETL_REQUIRE(dst && src, "etl_memcpy_h2d: null pointer");
+    if (int rc = wait_other_lanes(0)) return rc;  // after the work of every lane
     ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx().stream));
     ETL_HIP(hipStreamSynchronize(ctx().stream));
     return ETL_OK;
@@ -221,6 +251,7 @@ int etl_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     ETL_CHECK_INIT();
     if (bytes == 0) return ETL_OK;
     ETL_REQUIRE(dst && src, "etl_memcpy_d2h: null pointer");
+    if (int rc = wait_other_lanes(0)) return rc;  // after the work of every lane
     ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream));
     ETL_HIP(hipStreamSynchronize(ctx().stream));
     return ETL_OK;
@@ -230,6 +261,7 @@ int etl_memcpy_d2d(void* dst, const void* src, size_t bytes) {
     ETL_CHECK_INIT();
     if (bytes == 0) return ETL_OK;
     ETL_REQUIRE(dst && src, "etl_memcpy_d2d: null pointer");
+    if (int rc = wait_other_lanes(0)) return rc;  // after the work of every lane
     ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx().stream));
     return ETL_OK;
 }
@@ -242,7 +274,25 @@ int etl_zero(void* dst, size_t bytes) {
 
 int etl_synchronize(void) {
     ETL_CHECK_INIT();
-    ETL_HIP(hipStreamSynchronize(ctx().stream));
+    return sync_lanes();
+}
+
+int etl_lane_wait(int waiter, int signaler) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(waiter >= 0 && waiter < kLanes && signaler >= 0 && signaler < kLanes,
+                "etl_lane_wait: lanes are 0 .. %d", kLanes - 1);
+    if (waiter == signaler) return ETL_OK;
+    ETL_HIP(hipEventRecord(ctx().lane_ev[signaler], ctx().lane[signaler]));
+    ETL_HIP(hipStreamWaitEvent(ctx().lane[waiter], ctx().lane_ev[signaler], 0));
+    return ETL_OK;
+}
+
+int etl_memcpy_d2d_lane(void* dst, const void* src, size_t bytes, int lane) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(lane >= 0 && lane < kLanes, "etl_memcpy_d2d_lane: lanes are 0 .. %d", kLanes - 1);
+    if (bytes == 0) return ETL_OK;
+    ETL_REQUIRE(dst && src, "etl_memcpy_d2d_lane: null pointer");
+    ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx().lane[lane]));
     return ETL_OK;
 }
 

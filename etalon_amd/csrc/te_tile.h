@@ -186,24 +186,20 @@ __device__ __forceinline__ void trace_tile(const global_t<const int4v>* __restri
     __syncthreads();
 }
 
-// One wave per tile, `blockDim.x / 64` waves per workgroup (always one for EXTRA,
-// which synchronises the workgroup).  Workgroups are dealt to the XCDs round-robin,
-// so consecutive workgroup indices are spread XCD-major: each XCD receives a
-// contiguous range of tiles, whose neighbours share its L2.
+// The tile program of one wave: tile li = ((b & 7) * per_xcd + (b >> 3)) * waves + wave
+// of `tiles`, for workgroup b of n_groups.  Workgroups are dealt to the XCDs
+// round-robin, so consecutive workgroup indices are spread XCD-major: each XCD
+// receives a contiguous range of tiles, whose neighbours share its L2.  EXTRA tiles
+// synchronise the workgroup (one wave per workgroup) and stage rows through `lds`
+// (BR * 64 * V REALs).
 template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
-__global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restrict__ args,
-                                                   const int4* __restrict__ tiles, int n_tiles,
-                                                   const int* __restrict__ counter, int kstep0,
-                                                   int parity) {
-    constexpr int W = 64 * V;
+__device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
+                                          const int4* __restrict__ tiles, int n_tiles, int b,
+                                          int n_groups, int waves, int wave,
+                                          const int* __restrict__ counter, int kstep0, int parity,
+                                          R* __restrict__ lds) {
     constexpr int RW = BR + 2 * T;  // window rows (Ey: one more)
-    const int waves = blockDim.x >> 6;
-    const int n_groups = (n_tiles + waves - 1) / waves;
-    const int b = blockIdx.x;
     const int per_xcd = (n_groups + 7) >> 3;
-    // the wave index is uniform across the wave: readfirstlane puts it (and all that
-    // derives from it — the descriptor, the simulation's arguments) in SGPRs
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int li = ((b & 7) * per_xcd + (b >> 3)) * waves + wave;
     if (li >= n_tiles) return;
     const int4 td = tiles[li];  // first row, tile column, extra-tile index, simulation
@@ -317,7 +313,13 @@ __global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restric
         }
         for (int s = 0; s < src.n; ++s) {
             const int l = __builtin_amdgcn_readlane(src.row, s) - g0;
-            if (l >= 0 && l < RW) src.rows[__builtin_amdgcn_readlane(src.comp, s)] |= 1ull << l;
+            if (l < 0 || l >= RW) continue;
+            // constant indices keep src.rows in registers
+            const int comp = __builtin_amdgcn_readlane(src.comp, s);
+            const unsigned long long bit = 1ull << l;
+            if (comp == 0) src.rows[0] |= bit;
+            else if (comp == 1) src.rows[1] |= bit;
+            else src.rows[2] |= bit;
         }
         if (a.has_mon) {
             trace_base = to_global(a.trace);
@@ -332,8 +334,6 @@ __global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restric
             }
         }
     }
-    __shared__ R lds[EXTRA ? BR * W : 1];
-
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         // ---- H half-step, window rows [t, RW - t) ----
@@ -476,6 +476,22 @@ __global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restric
         ldv<R, V>(eyl, eyA, nx, ny, c, ny);
         store_row<R, V>(eyB + int64_t(nx) * ny, eyl, c, lo, min(hi, ny));
     }
+}
+
+// One wave per tile, `blockDim.x / 64` waves per workgroup (always one for EXTRA).
+template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
+__global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restrict__ args,
+                                                   const int4* __restrict__ tiles, int n_tiles,
+                                                   const int* __restrict__ counter, int kstep0,
+                                                   int parity) {
+    __shared__ R lds[EXTRA ? BR * 64 * V : 1];
+    const int waves = blockDim.x >> 6;
+    // the wave index is uniform across the wave: readfirstlane puts it (and all that
+    // derives from it — the descriptor, the simulation's arguments) in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    tile_body<R, V, T, BR, STRIP, EXTRA>(args, tiles, n_tiles, blockIdx.x,
+                                         (n_tiles + waves - 1) / waves, waves, wave, counter,
+                                         kstep0, parity, lds);
 }
 
 }  // namespace etl

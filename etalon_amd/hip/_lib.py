@@ -48,6 +48,7 @@ class EtlTune(C.Structure):
         ("trace_chunk", C.c_int),
         ("temporal", C.c_int),
         ("waves", C.c_int),
+        ("launch", C.c_int),
     ]
 
 
@@ -96,6 +97,8 @@ _SIGNATURES = {
     "etl_memcpy_h2d": [vp, vp, C.c_size_t],
     "etl_memcpy_d2h": [vp, vp, C.c_size_t],
     "etl_memcpy_d2d": [vp, vp, C.c_size_t],
+    "etl_memcpy_d2d_lane": [vp, vp, C.c_size_t, C.c_int],
+    "etl_lane_wait": [C.c_int, C.c_int],
     "etl_zero": [vp, C.c_size_t],
     "etl_synchronize": [],
     "etl_update_h_te": [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -127,9 +130,19 @@ _SIGNATURES = {
     "etl_adjplan_destroy": [vp],
     "etl_adjplan_sources": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), i64p, C.c_int, vp],
     "etl_adjplan_run": [vp, C.c_int64, C.c_int],
+    "etl_adjplan_run_rows": [vp, C.c_int64, C.c_int, C.c_int],
+    "etl_adjplan_set_lane": [vp, C.c_int],
+    "etl_coplan_create": [vp, vp, C.POINTER(vp)],
+    "etl_coplan_destroy": [vp],
+    "etl_coplan_run": [vp, C.c_int64, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int],
+    "etl_coplan_sweep": [vp, C.c_int64, C.c_int],
+    "etl_plan_set_checkpoints_sim": [vp, C.c_int, C.POINTER(vp), C.c_int],
+    "etl_plan_advance_checkpointed": [vp, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_int)],
     "etl_adjplan_lambda": [vp, C.c_int, C.c_int, C.POINTER(vp)],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],
     "etl_plan_advance": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)],
+    "etl_plan_advance_rows": [vp, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_int)],
+    "etl_plan_set_lane": [vp, C.c_int],
     "etl_plan_time": [vp, C.c_int64, C.c_int64, C.POINTER(EtlTiming), C.POINTER(C.c_int)],
 }
 

@@ -25,7 +25,9 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h, per simulation
 @dataclass
 class Tune:
     """Fused-kernel shape; 0 = the library default (vec 1, 8-row tiles, temporal
-    depth 4, 64-step trace chunks, one wave per workgroup)."""
+    depth 4, 64-step trace chunks, one wave per workgroup, and ``launch`` 1 — one
+    kernel per step block over every tile kind — for plans of at most 4096 tiles,
+    else 3 — one kernel per tile kind on three streams)."""
 
     band_rows: int = 0
     vec: int = 0
@@ -33,10 +35,11 @@ class Tune:
     trace_chunk: int = 0
     temporal: int = 0
     waves: int = 0
+    launch: int = 0
 
     def as_struct(self) -> _lib.EtlTune:
         return _lib.EtlTune(self.band_rows, self.vec, self.use_graph, self.trace_chunk,
-                            self.temporal, self.waves)
+                            self.temporal, self.waves, self.launch)
 
 
 def _describe(state, pair: dict) -> _lib.EtlTeDesc:
@@ -134,17 +137,37 @@ class FusedPlan:
         _lib.call("etl_plan_phasor_table_sim", self.handle, int(sim), int(mon_id), int(n0),
                   int(table.shape[0]), C.c_void_p(table.ctypes.data))
 
-    def advance(self, n0: int, count: int) -> int:
+    def advance(self, n0: int, count: int, probe_row0: int = 0) -> int:
+        """Steps [n0, n0 + count); probes write rows probe_row0 .. probe_row0 + count - 1."""
         parity = C.c_int()
-        _lib.call("etl_plan_advance", self.handle, int(n0), int(count), C.byref(parity))
+        _lib.call("etl_plan_advance_rows", self.handle, int(n0), int(count), int(probe_row0),
+                  C.byref(parity))
         self.parity = parity.value
         return self.parity
+
+    def set_checkpoints(self, arrays: Sequence, n_seg: int, sim: int = 0) -> None:
+        """Checkpoint storage: arrays[k] holds ``n_seg`` copies of field k (FIELDS order)."""
+        ptrs = (C.c_void_p * 7)(*[a.ptr for a in arrays])
+        _lib.call("etl_plan_set_checkpoints_sim", self.handle, int(sim), ptrs, int(n_seg))
+
+    def advance_checkpointed(self, n0: int, count: int, every: int) -> int:
+        """Steps [n0, n0 + count), the state saved into slot c // every before each
+        segment of ``every`` steps starting at step c."""
+        parity = C.c_int()
+        _lib.call("etl_plan_advance_checkpointed", self.handle, int(n0), int(count), int(every),
+                  C.byref(parity))
+        self.parity = parity.value
+        return self.parity
+
+    def set_lane(self, lane: int) -> None:
+        """Issue this plan's work on stream lane 0 or 1 (see etl_lane_wait)."""
+        _lib.call("etl_plan_set_lane", self.handle, int(lane))
 
     def resolved_tune(self) -> Tune:
         """The tuning in effect (library defaults filled in)."""
         t = _lib.EtlTune()
         _lib.call("etl_plan_tune", self.handle, C.byref(t))
-        return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal, t.waves)
+        return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal, t.waves, t.launch)
 
     def time(self, n0: int, count: int) -> dict:
         """HIP-event timing of ``count`` steps launched directly: ``launch_ms`` (average

@@ -6,8 +6,10 @@
  * message of the last failure on the calling thread is read with etl_last_error().
  * All "device pointers" are addresses on the library's device (etl_init), such as
  * those returned by etl_malloc().  No function takes or returns a torch type.
- * Calls are not thread-safe on one handle; the library issues all work on one HIP
- * stream per process (default priority), and etl_synchronize() waits for it.
+ * Calls are not thread-safe on one handle; the library issues its work on HIP
+ * streams of default priority: lane 0 carries every call, except plans moved to
+ * lane 1 (etl_plan_set_lane, etl_adjplan_set_lane) so that two plans run at once;
+ * etl_lane_wait orders the lanes and etl_synchronize() waits for both.
  *
  * Precision codes: ETL_F32 (REAL = float, complex = 2 x float interleaved) and
  * ETL_F64 (REAL = double).  Arrays are C-ordered (x, y) with y fastest:
@@ -58,6 +60,10 @@ int etl_memcpy_d2h(void* dst, const void* src, size_t bytes);  /* synchronous */
 int etl_memcpy_d2d(void* dst, const void* src, size_t bytes);  /* stream-ordered */
 int etl_zero(void* dst, size_t bytes);                         /* fill kernel, stream-ordered */
 int etl_synchronize(void);
+/* Lanes (see above): work issued on lane `waiter` after this call starts once the
+ * work issued on lane `signaler` before it has finished.  No reference counterpart. */
+int etl_lane_wait(int waiter, int signaler);
+int etl_memcpy_d2d_lane(void* dst, const void* src, size_t bytes, int lane);
 
 /* ------------------------------------------------------------------ protocol primitives
  * One launch each, in place, exactly the operations of the NumPy reference backend. */
@@ -133,6 +139,9 @@ typedef struct EtlTune {
     int trace_chunk;   /* steps per DFT flush: a multiple of 2 * temporal */
     int temporal;      /* steps fused per launch: 1 (none), 2 or 4 */
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
+    int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
+                        * band 8; one wave per workgroup); 3: one kernel per tile kind on
+                        * three streams; 0: 1 for grids of at most 4096 tiles, else 3 */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
@@ -186,6 +195,21 @@ int etl_plan_tune(etl_plan_t plan, EtlTune* out);
 /* Advance steps [n, n + count); the tables must cover them.  On return *parity
  * tells which buffer of each pair holds the state (0: field[k][0]). */
 int etl_plan_advance(etl_plan_t plan, int64_t n, int64_t count, int* parity);
+/* The same, with probe rows starting at row probe_row0 of each probe's buffer (step
+ * n + r to row probe_row0 + r; probe_row0 + count <= out_rows), so that one buffer can
+ * hold several segments (double buffering against a consumer on the other lane). */
+int etl_plan_advance_rows(etl_plan_t plan, int64_t n, int64_t count, int probe_row0,
+                          int* parity);
+/* Issue this plan's work on lane 0 or 1 (default 0). */
+int etl_plan_set_lane(etl_plan_t plan, int lane);
+/* Checkpoint storage of simulation `sim`: ptr[k] (etl_malloc) holds n_seg slots of
+ * field k (ex ey hz psi_ex_y psi_ey_x psi_hz_x psi_hz_y order, each slot the field's
+ * full array).  For the checkpointed adjoint; no reference counterpart. */
+int etl_plan_set_checkpoints_sim(etl_plan_t plan, int sim, void* const* ptr, int n_seg);
+/* Advance [n, n + count) (n a multiple of K), saving the state of every simulation
+ * into checkpoint slot c / K before each segment of K steps starting at step c. */
+int etl_plan_advance_checkpointed(etl_plan_t plan, int64_t n, int64_t count, int K,
+                                  int* parity);
 /* Event timing of `count` steps from step n (launched directly, not from the graph). */
 typedef struct EtlTiming {
     double launch_ms;        /* average fused launch, fork to join (every tile kind) */
@@ -249,7 +273,30 @@ int etl_adjplan_sources(etl_adjplan_t plan, int sim, int n, const int* comp, con
 /* Reverse steps n_first + count - 1 .. n_first of every simulation (history row r =
  * step n_first + r). */
 int etl_adjplan_run(etl_adjplan_t plan, int64_t n_first, int count);
+/* The same, with history row hist_row0 + r = step n_first + r (hist_row0 + count <=
+ * hist_rows). */
+int etl_adjplan_run_rows(etl_adjplan_t plan, int64_t n_first, int count, int hist_row0);
+/* Issue this plan's work on lane 0 or 1 (default 0). */
+int etl_adjplan_set_lane(etl_adjplan_t plan, int lane);
 int etl_adjplan_lambda(etl_adjplan_t plan, int sim, int field, void** ptr);
+
+/* A forward plan and an adjoint plan advanced together: forward steps [n, n +
+ * count_f) (probe rows from probe_row0) and reverse steps n_first + count_a - 1 ..
+ * n_first (history rows from hist_row0), step block i of each in one kernel launch,
+ * e.g. the recompute of checkpoint segment k - 1 beside the reverse steps of segment
+ * k.  Bit-identical to etl_plan_advance_rows then etl_adjplan_run_rows.  The forward
+ * plan needs vec 1 and band 8; both plans on one lane; the plans outlive the co-plan. */
+typedef struct EtlCoPlan* etl_coplan_t;
+int etl_coplan_create(etl_plan_t fwd, etl_adjplan_t adj, etl_coplan_t* out);
+int etl_coplan_destroy(etl_coplan_t co);
+int etl_coplan_run(etl_coplan_t co, int64_t n, int count_f, int probe_row0, int64_t n_first,
+                   int count_a, int hist_row0);
+/* The backward sweep of a checkpointed adjoint over steps [0, n_steps) in segments
+ * of K: restore checkpoint slot k - 1 into the forward plan (etl_plan_set_checkpoints),
+ * recompute segment k - 1 (probe rows (k - 1 & 1) K ..) beside the reverse steps of
+ * segment k (history rows (k & 1) K ..), from the last segment down; the probes and
+ * histories hold 2K rows. */
+int etl_coplan_sweep(etl_coplan_t co, int64_t n_steps, int K);
 
 #ifdef __cplusplus
 }

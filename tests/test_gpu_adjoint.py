@@ -106,12 +106,47 @@ def test_tile_and_plain_adjoint_agree(hip, depth):
     assert err < 1e-6
 
 
+@pytest.mark.parametrize("depth", [1, 2, 4])
+def test_adjoint_launch_modes_agree_bitwise(hip, depth):
+    """One kernel per step block (every tile kind) against one kernel per tile kind on
+    three streams, for the forward plans and the reverse steps."""
+    from etalon_amd.hip.plan import Tune
+
+    sim, design = design_problem("single", seed=23, steps=250, pixels=(7, 5))
+    eps = np.asarray(design.permittivity)
+    J1, g1 = AdjointProblem(sim, ["out"], tune=Tune(temporal=depth, launch=1)).run(eps)
+    J3, g3 = AdjointProblem(sim, ["out"], tune=Tune(temporal=depth, launch=3)).run(eps)
+    assert J1 == J3
+    assert np.array_equal(g1, g3)
+
+
 def test_checkpoint_and_store_modes_agree_bitwise(hip):
     sim, design = design_problem("single", seed=9, steps=300)
     eps = np.asarray(design.permittivity)
     _, g1 = AdjointProblem(sim, ["out"], checkpoint_every=64, mode="checkpoint").run(eps)
     _, g2 = AdjointProblem(sim, ["out"], mode="store").run(eps)
+    _, g3 = AdjointProblem(sim, ["out"], checkpoint_every=64, mode="checkpoint",
+                           schedule="sequential").run(eps)
     assert np.array_equal(g1, g2)
+    assert np.array_equal(g1, g3)
+
+
+@pytest.mark.parametrize("depth", [1, 4])
+def test_segment_schedules_agree_bitwise(hip, depth):
+    """The recompute of segment k - 1 beside the reverse steps of segment k — in one
+    launch per step block (joint) or on two streams (lanes) — against one after the
+    other, with 5 segments, the last one short (odd depth-1 tails)."""
+    from etalon_amd.hip.plan import Tune
+
+    sim, design = design_problem("single", seed=4, steps=301)
+    eps = np.asarray(design.permittivity)
+    tune = Tune(temporal=depth)
+    out = {s: AdjointProblem(sim, ["out"], checkpoint_every=64, schedule=s, tune=tune).run(eps)
+           for s in ("joint", "lanes", "sequential")}
+    J, g = out["sequential"]
+    for s in ("joint", "lanes"):
+        assert out[s][0] == J, s
+        assert np.array_equal(out[s][1], g), s
 
 
 def test_gradient_against_central_differences(hip):

@@ -41,13 +41,15 @@ def assert_same(a, b, what):
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
-@pytest.mark.parametrize("shape", [(1, 4, 8, 1), (1, 4, 8, 4), (1, 2, 16, 2), (1, 1, 8, 1)])
+@pytest.mark.parametrize("shape", [(1, 4, 8, 1, 3), (1, 4, 8, 4, 3), (1, 2, 16, 2, 3),
+                                   (1, 1, 8, 1, 3), (1, 4, 8, 1, 1), (1, 1, 8, 1, 1)])
 def test_batch_equals_each_simulation_alone(hip, precision, shape):
-    vec, depth, band, waves = shape
+    vec, depth, band, waves, launch = shape
     sims = small_batch(precision)
-    print(f"batch of {len(sims)} {precision}, shape vec={vec} T={depth} band={band} waves={waves}",
-          flush=True)
-    batch = BatchSolver(sims, tune=Tune(band_rows=band, vec=vec, temporal=depth, waves=waves))
+    print(f"batch of {len(sims)} {precision}, shape vec={vec} T={depth} band={band} waves={waves} "
+          f"launch={launch}", flush=True)
+    batch = BatchSolver(sims, tune=Tune(band_rows=band, vec=vec, temporal=depth, waves=waves,
+                                        launch=launch))
     batch.advance(batch.n_steps)
     for k, (sim, solver) in enumerate(zip(sims, batch.solvers)):
         assert_same(snapshot(solver), alone(sim), f"simulation {k}")

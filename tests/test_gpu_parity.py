@@ -156,19 +156,23 @@ def _oracle_full(precision, steps):
 TUNES_F32 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16), (1, 4, 16),
              (2, 1, 8), (2, 2, 8), (2, 4, 8)]
 TUNES_F64 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16)]
-SHAPES = [("single", np.float32, *s) for s in TUNES_F32] + [("double", np.float64, *s)
-                                                           for s in TUNES_F64]
+# launch 3: one kernel per tile kind on three streams; 1: one kernel per step block
+SHAPES = [("single", np.float32, *s, 3) for s in TUNES_F32] + \
+    [("double", np.float64, *s, 3) for s in TUNES_F64] + \
+    [(p, d, 1, t, 8, 1) for p, d in (("single", np.float32), ("double", np.float64))
+     for t in (1, 2, 4)]
 
 
-@pytest.mark.parametrize(("precision", "dtype", "vec", "depth", "band"), SHAPES)
-def test_fused_tile_shapes_bitwise(hip, precision, dtype, vec, depth, band):
-    """Temporal blocking depth, lane width and tile height never change a bit."""
+@pytest.mark.parametrize(("precision", "dtype", "vec", "depth", "band", "launch"), SHAPES)
+def test_fused_tile_shapes_bitwise(hip, precision, dtype, vec, depth, band, launch):
+    """Temporal blocking depth, lane width, tile height and launch mode never change a
+    bit."""
     from etalon_amd.hip.plan import Tune
     from etalon_amd.solver import FDTDSolver
 
     sim = full_sim(precision)
     gpu = FDTDSolver(sim, tune=Tune(band_rows=band, vec=vec, temporal=depth,
-                                    trace_chunk=8 * depth, use_graph=1))
+                                    trace_chunk=8 * depth, use_graph=1, launch=launch))
     for count in (3, 64, 129, 204):  # 400 steps: graph chunks, depth-1 tails, odd launches
         gpu.advance(count)
     _compare_solver(gpu, _oracle_full(precision, 400))
