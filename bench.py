@@ -4,6 +4,7 @@
                     [--no-cpu-baseline]
     python bench.py --workload batched [--batch N] [--batch-steps S] [--seed K]
     python bench.py --workload adjoint [--seed K] [--fd-pixels P]
+    python bench.py --workload batched-adjoint [--candidates N] [--phc-steps S] [--fd-pixels P]
 
 Workload (the throughput grid): 2000 × 2000 interior cells at dl = 0.05 µm plus a
 12-layer CPML on every face (2024 × 2024 cells), a silicon strip waveguide
@@ -38,6 +39,17 @@ float32, checkpoints every 64 steps.  value = wall time of J and ∂J/∂ε over
 wall time of one plain forward run; the gradient is checked against the float64
 gradient and float64 central differences on random pixels; the NumPy float64
 adjoint is timed on a bounded sample of steps of the same grid.
+
+``--workload batched-adjoint``: N (32) design candidates, each 1000 x 1000
+interior + 12 CPML, a triangular-lattice photonic crystal (a = 0.42 µm, air holes
+of r = 0.126 µm in ε 12.11) with a W1 line defect and a 4 x 4 µm design region of
+80 x 80 pixels, J = Σ over 9 wavelengths of |DFT Hz|² on the output line monitor,
+8000 steps, checkpoints every ~√N steps.  One fused plan runs every candidate's
+forward pass; the backward sweep runs the recompute and the reverse steps of every
+candidate in one joint launch per step block.  value = wall time of every J and
+∂J/∂ε over the wall time of one batched plain forward run of the candidates;
+candidate 0's gradient is checked against its float64 gradient and float64
+central differences.
 """
 
 from __future__ import annotations
@@ -370,6 +382,128 @@ def main_adjoint(args) -> int:
     return 0
 
 
+def run_batched_adjoint(args) -> dict:
+    from etalon_amd.adjoint import AdjointProblem
+    from etalon_amd.batch import BatchSolver
+    from etalon_amd.batch_adjoint import BatchAdjointProblem
+    from etalon_amd.workloads import phc_adjoint_candidates
+
+    t_setup = time.perf_counter()
+    sims, eps, names = phc_adjoint_candidates(n=args.candidates, seed=args.seed,
+                                              steps=args.phc_steps)
+    problems = [AdjointProblem(sim, names) for sim in sims]
+    batch = BatchAdjointProblem(problems)
+    print(f"batched adjoint: {len(sims)} candidates set up in "
+          f"{time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+    batch.run(eps)  # warm: plans, graphs, allocator
+    tape = batch.forward(eps)
+    t0 = time.perf_counter()
+    grads = batch.backward(tape)
+    t_grad = tape.forward_s + time.perf_counter() - t0
+    J32 = [float(j) for j in tape.J]
+    del tape
+    design_sims = [p.with_permittivity(e) for p, e in zip(problems, eps)]
+    BatchSolver(design_sims).run()  # warm
+    fwd = BatchSolver(design_sims)
+    t_fwd = fwd.run()[0].metadata["wall_time_s"]
+    del fwd
+    print(f"batched adjoint: forward {t_fwd:.3f} s, J and gradients {t_grad:.3f} s",
+          file=sys.stderr, flush=True)
+
+    # candidate 0 in float64: the adjoint gradient, and central differences on random
+    # pixels (the 2 P perturbed runs advanced as one float64 batch)
+    sims64, _, _ = phc_adjoint_candidates(n=1, seed=args.seed, steps=args.phc_steps,
+                                          precision="double")
+    p64 = AdjointProblem(sims64[0], names)
+    J64, g64 = p64.run(eps[0])
+    scale = float(np.abs(g64).max())
+    rng = np.random.default_rng(args.seed + 1)
+    picks = rng.choice(eps[0].size, size=args.fd_pixels, replace=False)
+    runs = []
+    for flat in picks:
+        p, q = divmod(int(flat), eps[0].shape[1])
+        h = 1e-4 * eps[0][p, q]
+        for sign in (1.0, -1.0):
+            e = eps[0].copy()
+            e[p, q] += sign * h
+            runs.append(p64.with_permittivity(e))
+    fd_batch = BatchSolver(runs)
+    fd_batch.run()
+    Jfd = [p64._objective(s) for s in fd_batch.solvers]
+    del fd_batch
+    fd_rel = []
+    for k, flat in enumerate(picks):
+        p, q = divmod(int(flat), eps[0].shape[1])
+        h = 1e-4 * eps[0][p, q]
+        fd_rel.append(abs(grads[0][p, q] - (Jfd[2 * k] - Jfd[2 * k + 1]) / (2 * h)) / scale)
+
+    # the NumPy float64 adjoint on a bounded sample: a grid of the same size (the cost
+    # per step does not depend on the permittivity map)
+    from oracle import adjoint_numpy as adj
+
+    short, short_design, _ = adjoint_design_sim("double", seed=args.seed,
+                                                steps=args.cpu_phc_steps, interior=1000,
+                                                region=4.0, pixels=80)
+    t0 = time.perf_counter()
+    adj.gradient(short, set(names), short_design)
+    cpu_wall = time.perf_counter() - t0
+    n_cells = sum(p.grid.nx * p.grid.ny for p in problems)
+    return {"t_grad": t_grad, "t_fwd": t_fwd, "J": J32,
+            "J64_0": J64, "grad_rel_f64": float(np.abs(grads[0] - g64).max() / scale),
+            "fd_rel": fd_rel, "cpu_wall": cpu_wall, "cells": n_cells, "n": len(problems),
+            "steps": problems[0].n_steps, "K": problems[0].K, "pixels": eps[0].shape,
+            "grid": (problems[0].grid.nx, problems[0].grid.ny), "holes": len(sims[0].structures) - 1}
+
+
+def main_batched_adjoint(args) -> int:
+    r = run_batched_adjoint(args)
+    fd = np.array(r["fd_rel"])
+    cpu_total = r["cpu_wall"] / args.cpu_phc_steps * r["steps"] * r["n"]
+    line = {
+        "metric": "batched adjoint grad time vs batched fwd",
+        "value": round(r["t_grad"] / r["t_fwd"], 3),
+        "unit": "x (wall of J and dJ/deps of every candidate / wall of one batched plain "
+                "forward run of them)",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": 1,
+        "ms_per_step": round(r["t_grad"] / r["steps"] * 1e3, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: {r['n']} seeded design candidates (seed {args.seed})",
+        "config": {"workload": f"batched adjoint: {r['n']} candidates, 1000x1000 interior + 12 "
+                               "CPML at dl 0.05 um, triangular-lattice photonic crystal (a 0.42 "
+                               "um, air holes r 0.126 um in eps 12.11) with a W1 line defect, "
+                               "4x4 um design region of 80x80 pixels (eps 2.085-12.11), J = "
+                               "sum over 9 wavelengths 1.50-1.60 um of |DFT Hz|^2 on the output "
+                               f"line monitor, {r['steps']} steps, checkpoints every {r['K']} "
+                               "steps; forward recompute and reverse steps of every candidate "
+                               "in one launch per step block",
+                   "grid": list(r["grid"]), "holes": r["holes"], "pixels": list(r["pixels"]),
+                   "cells_total": r["cells"]},
+        "t_grad_s": round(r["t_grad"], 3),
+        "t_fwd_s": round(r["t_fwd"], 3),
+        "candidates_per_s": round(r["n"] / r["t_grad"], 2),
+        "gcells_grad": round(3 * r["cells"] * r["steps"] / r["t_grad"] / 1e9, 1),
+        "gradient_check": {"candidate": 0,
+                           "vs_f64_adjoint_max_rel": float(f"{r['grad_rel_f64']:.3g}"),
+                           "fd_pixels": int(fd.size),
+                           "vs_f64_central_fd_max_rel": float(f"{fd.max():.3g}"),
+                           "J_f32": r["J"][0], "J_f64": r["J64_0"]},
+        "cpu_baseline": {"value": round(cpu_total, 1), "unit": "s per batch of gradients "
+                         "(extrapolated)", "cores": 1, "kind": "port",
+                         "sample": f"NumPy float64 adjoint of a 1024x1024 grid over "
+                                   f"{args.cpu_phc_steps} steps ({r['cpu_wall']:.1f} s), "
+                                   f"scaled to {r['steps']} steps x {r['n']} candidates; host "
+                                   f"{os.cpu_count()} CPUs, {_cpu_model()}",
+                         "vs_gpu_grad": round(cpu_total / r["t_grad"], 1)},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,7 +512,7 @@ def main() -> int:
     ap.add_argument("--precision", choices=("single", "double"), default="single")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=400)
-    ap.add_argument("--workload", choices=("throughput", "batched", "adjoint"),
+    ap.add_argument("--workload", choices=("throughput", "batched", "adjoint", "batched-adjoint"),
                     default="throughput")
     ap.add_argument("--fd-pixels", type=int, default=16)
     ap.add_argument("--cpu-adjoint-steps", type=int, default=300)
@@ -386,6 +520,9 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=4000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--candidates", type=int, default=32)
+    ap.add_argument("--phc-steps", type=int, default=8000)
+    ap.add_argument("--cpu-phc-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the FDTD "
@@ -396,6 +533,8 @@ def main() -> int:
         return main_batched(args)
     if args.workload == "adjoint":
         return main_adjoint(args)
+    if args.workload == "batched-adjoint":
+        return main_batched_adjoint(args)
 
     main_run = run(args.precision, args.steps, args.warmup)
     nx, ny = main_run["cells"]

@@ -71,6 +71,17 @@ class Tape:
     forward_s: float = 0.0
 
 
+@dataclass
+class _AdjointBuffers:
+    """λ, ∂J/∂cb and history arrays of one simulation's reverse run."""
+
+    lam: dict
+    g_ex: object
+    g_ey: object
+    hist: object
+    desc: object
+
+
 class AdjointProblem:
     def __init__(self, sim, objective: Sequence[str], checkpoint_every: Optional[int] = None,
                  mode: str = "checkpoint", tune=None, kernel: str = "tile",
@@ -110,6 +121,7 @@ class AdjointProblem:
             raise ValueError("the design region covers no E node")
         self.box = (int(ii.min()), int(jj.min()), int(ii.max() - ii.min() + 1),
                     int(jj.max() - jj.min() + 1))
+        self._box_checked = False
         for src in sim.sources:
             if src.component != "Hz":
                 raise ValueError("adjoint problems take Hz sources (an E source's strength "
@@ -124,6 +136,8 @@ class AdjointProblem:
         return self.sim.model_copy(update={"structures": (*self.sim.structures[:-1], design)})
 
     def _check_box_off_cpml(self, solver) -> None:
+        if self._box_checked:  # the CPML profiles depend on the grid only
+            return
         i0, j0, ni, nj = self.box
         s = solver.state
         for name, lo, hi in (("bx_e", i0 - 1, i0 + ni + 1), ("bx_h", i0 - 1, i0 + ni + 1),
@@ -133,6 +147,7 @@ class AdjointProblem:
             ik = getattr(s, name.replace("b", "ik", 1)).get()[lo:hi]
             if np.any(b != 1) or np.any(c != 0) or np.any(ik != 1):
                 raise ValueError("the design region must lie outside the CPML")
+        self._box_checked = True
 
     def _box_hz_nodes(self) -> np.ndarray:
         i0, j0, ni, nj = self.box
@@ -216,33 +231,15 @@ class AdjointProblem:
     def backward(self, tape: Tape) -> np.ndarray:
         """∂J/∂ε per design pixel (float64), from the forward tape."""
         solver = tape.solver
-        state = solver.state
-        real = solver.real
         N, K = self.n_steps, self.K
-        i0, j0, ni, nj = self.box
-        lam = {name: zeros(getattr(state, name).shape, real) for name in FIELDS}
-        g_ex = zeros((ni, nj), np.float64)
-        g_ey = zeros((ni, nj), np.float64)
-        box_nodes = self._box_hz_nodes()
         if self.mode == "store":
             hist = tape.history
         else:
             # two halves when recompute and reverse steps overlap
             rows = 2 * K if self.schedule != "sequential" else K
-            hist = zeros((rows, box_nodes.size), real)
-        d = _lib.EtlAdjointDesc()
-        d.precision = 0 if real == np.float32 else 1
-        d.nx, d.ny = int(state.nx), int(state.ny)
-        d.db, d.inv_dx, d.inv_dy = float(state.db), float(state.inv_dx), float(state.inv_dy)
-        for k, name in enumerate(FIELDS):
-            d.lam[k] = lam[name].ptr
-        d.cb_ex, d.cb_ey = state.cb_ex.ptr, state.cb_ey.ptr
-        for name in PROFILES:
-            setattr(d, name, getattr(state, name).ptr)
-        d.box_i0, d.box_j0, d.box_ni, d.box_nj = i0, j0, ni, nj
-        d.grad_ex, d.grad_ey = g_ex.ptr, g_ey.ptr
-        d.hist, d.hist_rows = hist.ptr, hist.shape[0]
-        rev = _Reverse(self.kernel, d, self.tune)
+            hist = zeros((rows, self._box_hz_nodes().size), solver.real)
+        bufs = self._adjoint_buffers(solver, hist)
+        rev = _Reverse(self.kernel, bufs.desc, self.tune)
         try:
             comp, flat, table = self._sources(solver)
             rev.sources(comp, flat, table)
@@ -253,16 +250,58 @@ class AdjointProblem:
             solver.backend.synchronize()
         finally:
             rev.close()
-        # ∂J/∂εr = −(cb/εr) ∂J/∂cb at the nodes, summed per pixel
+        return self._pixel_gradient(solver, bufs)
+
+    # ------------------------------------------------------------------ pieces
+    def _adjoint_buffers(self, solver, hist) -> "_AdjointBuffers":
+        """Zeroed λ and ∂J/∂cb arrays of one simulation and its adjoint descriptor."""
+        state, real = solver.state, solver.real
+        i0, j0, ni, nj = self.box
+        b = _AdjointBuffers(lam={name: zeros(getattr(state, name).shape, real) for name in FIELDS},
+                            g_ex=zeros((ni, nj), np.float64), g_ey=zeros((ni, nj), np.float64),
+                            hist=hist, desc=_lib.EtlAdjointDesc())
+        d = b.desc
+        d.precision = 0 if real == np.float32 else 1
+        d.nx, d.ny = int(state.nx), int(state.ny)
+        d.db, d.inv_dx, d.inv_dy = float(state.db), float(state.inv_dx), float(state.inv_dy)
+        for k, name in enumerate(FIELDS):
+            d.lam[k] = b.lam[name].ptr
+        d.cb_ex, d.cb_ey = state.cb_ex.ptr, state.cb_ey.ptr
+        for name in PROFILES:
+            setattr(d, name, getattr(state, name).ptr)
+        d.box_i0, d.box_j0, d.box_ni, d.box_nj = i0, j0, ni, nj
+        d.grad_ex, d.grad_ey = b.g_ex.ptr, b.g_ey.ptr
+        d.hist, d.hist_rows = hist.ptr, hist.shape[0]
+        return b
+
+    def _pixel_gradient(self, solver, bufs) -> np.ndarray:
+        """∂J/∂εr = −(cb/εr) ∂J/∂cb at the nodes, summed per pixel."""
+        i0, j0, _, _ = self.box
+        state = solver.state
         eps = solver.eps_r
         cb = {"Ex": state.cb_ex.get(), "Ey": state.cb_ey.get()}
-        gcb = {"Ex": g_ex.get(), "Ey": g_ey.get()}
+        gcb = {"Ex": bufs.g_ex.get(), "Ey": bufs.g_ey.get()}
         out = np.zeros(self.shape[0] * self.shape[1])
         for c in ("Ex", "Ey"):
             ii, jj, p = self.pixmap[c]
             g = -(cb[c][ii, jj].astype(np.float64) / eps[c][ii, jj]) * gcb[c][ii - i0, jj - j0]
             np.add.at(out, p, g)
         return out.reshape(self.shape)
+
+    def _configure_recompute(self, plan, solver, sim: int, hist) -> None:
+        """Sources, source table and the Hz probe on the box of simulation ``sim`` of a
+        recompute plan (no monitors: the recompute only feeds the history)."""
+        comp, ii, jj = [], [], []
+        for inj in solver.injections:
+            comp += [_COMP[inj.component]] * inj.i.size
+            ii += list(inj.i)
+            jj += list(inj.j)
+        plan.set_sources(np.array(comp, np.int32), np.array(ii, np.int64),
+                         np.array(jj, np.int64), sim=sim)
+        delta, _ = solver._tables(0, self.n_steps)
+        if delta is not None:
+            plan.source_table(0, delta, sim=sim)
+        plan.add_probe("Hz", self._box_hz_nodes(), hist, sim=sim)
 
     def _reverse_segments(self, tape: Tape, rev, hist) -> None:
         """Recompute each segment from its checkpoint (with the Hz probe on the box),
@@ -273,17 +312,7 @@ class AdjointProblem:
         N = self.n_steps
         plan = FusedPlan(solver.state, self.tune)
         try:
-            comp, ii, jj = [], [], []
-            for inj in solver.injections:
-                comp += [_COMP[inj.component]] * inj.i.size
-                ii += list(inj.i)
-                jj += list(inj.j)
-            plan.set_sources(np.array(comp, np.int32), np.array(ii, np.int64),
-                             np.array(jj, np.int64))
-            delta, _ = solver._tables(0, N)
-            if delta is not None:
-                plan.source_table(0, delta)
-            plan.add_probe("Hz", self._box_hz_nodes(), hist)
+            self._configure_recompute(plan, solver, 0, hist)
             # Schedules: "joint" — the recompute of segment k - 1 and the reverse steps
             # of segment k in one launch per step block (a co-plan); "lanes" — the
             # same two on two streams; "sequential" — one after the other.  The first
@@ -347,22 +376,28 @@ class _Reverse:
     one-launch-per-half-step kernels ("plain"); the same arithmetic."""
 
     def __init__(self, kernel: str, desc, tune) -> None:
+        """``desc``: one EtlAdjointDesc, or a list of them (tile kernels: one plan over
+        every simulation)."""
         self.kernel = kernel
         self.handle = C.c_void_p()
+        descs = desc if isinstance(desc, (list, tuple)) else [desc]
         if kernel == "tile":
             t = None
             if tune is not None:
                 t = _lib.EtlTune(0, 1, 1, 0, tune.temporal, tune.waves, tune.launch)
-            _lib.call("etl_adjplan_create", C.byref(desc), 1, C.byref(t) if t else None,
+            arr = (_lib.EtlAdjointDesc * len(descs))(*descs)
+            _lib.call("etl_adjplan_create", arr, len(descs), C.byref(t) if t else None,
                       C.byref(self.handle))
         else:
-            _lib.call("etl_adjoint_create", C.byref(desc), C.byref(self.handle))
+            if len(descs) != 1:
+                raise ValueError("the plain reverse kernels take one simulation")
+            _lib.call("etl_adjoint_create", C.byref(descs[0]), C.byref(self.handle))
 
-    def sources(self, comp, flat, table) -> None:
+    def sources(self, comp, flat, table, sim: int = 0) -> None:
         args = (int(comp.size), comp.ctypes.data_as(C.POINTER(C.c_int)),
                 flat.ctypes.data_as(_lib.i64p), int(table.shape[0]), C.c_void_p(table.ctypes.data))
         if self.kernel == "tile":
-            _lib.call("etl_adjplan_sources", self.handle, 0, *args)
+            _lib.call("etl_adjplan_sources", self.handle, int(sim), *args)
         else:
             _lib.call("etl_adjoint_sources", self.handle, *args)
 

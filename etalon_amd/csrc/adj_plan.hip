@@ -286,11 +286,12 @@ int enqueue_adj(EtlAdjPlan* p, int count, int parity, int* launches) {
             ++n;
             continue;
         }
+        // strip tiles on the main stream (dispatched first), interior tiles on a side one
         ETL_HIP(hipEventRecord(p->ev_fork, s));
         ETL_HIP(hipStreamWaitEvent(p->s_strip, p->ev_fork, 0));
         ETL_HIP(hipStreamWaitEvent(p->s_extra, p->ev_fork, 0));
-        int rc = launch_adj<R>(args, tl, depth, p->tune.waves, p->step, rtop, parity ^ (n & 1), s,
-                               p->s_strip, p->s_extra);
+        int rc = launch_adj<R>(args, tl, depth, p->tune.waves, p->step, rtop, parity ^ (n & 1),
+                               p->s_strip, s, p->s_extra);
         if (rc) return rc;
         ETL_HIP(hipEventRecord(p->ev_join, p->s_strip));
         ETL_HIP(hipEventRecord(p->ev_join2, p->s_extra));
@@ -354,7 +355,9 @@ int adj_strips(const EtlAdjointDesc& d, int* out) {
 
 namespace {
 
-int adj_prepare_run(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0) {
+// `step_out` non-null: the two step-record values are returned there instead of set.
+int adj_prepare_run(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0,
+                    int* step_out = nullptr) {
     ETL_REQUIRE(count >= 0 && n_first >= 0 && n_first < (int64_t(1) << 31),
                 "etl_adjplan_run: bad step range");
     for (size_t k = 0; k < p->sims.size(); ++k) {
@@ -374,6 +377,11 @@ int adj_prepare_run(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0) {
         if ((rc = p->precision == ETL_F32 ? upload_adj_args_t<float>(p) : upload_adj_args_t<double>(p)))
             return rc;
     }
+    if (step_out) {
+        step_out[0] = int(n_first);
+        step_out[1] = hist_row0;
+        return ETL_OK;
+    }
     hipStream_t s = lane_stream(p->lane);
     hipLaunchKernelGGL(set_adj_step_kernel, dim3(1), dim3(64), 0, s, p->step, int(n_first),
                        hist_row0);
@@ -385,14 +393,15 @@ int adj_prepare_run(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0) {
 
 namespace etl {
 
-int adj_prepare(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0) {
-    return adj_prepare_run(p, n_first, count, hist_row0);
+int adj_prepare(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0, int* step_out) {
+    return adj_prepare_run(p, n_first, count, hist_row0, step_out);
 }
 
 void adj_info(const EtlAdjPlan* p, AdjInfo* out) {
     const int4* t = p->tiles_dev;
     out->precision = p->precision;
     out->temporal = p->tune.temporal;
+    out->launch = p->tune.launch;
     out->lane = p->lane;
     out->args = p->args_dev;
     out->tl = TileList{t, p->n_interior, t + p->n_interior, p->n_strip,
@@ -469,12 +478,7 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
         if (rc == ETL_OK)
             rc = precision == ETL_F32 ? adj_strips<float>(d, s.strips) : adj_strips<double>(d, s.strips);
     }
-    if (rc == ETL_OK && p->tune.launch == 0) {
-        int64_t tiles = 0;
-        for (const auto& sim : p->sims)
-            tiles += int64_t((sim.nx + kAdjBand - 1) / kAdjBand) * ((sim.ny + 1 + p->wo() - 1) / p->wo());
-        p->tune.launch = tiles <= kSingleLaunchTiles ? 1 : 3;
-    }
+    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = 1;
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||

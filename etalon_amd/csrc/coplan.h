@@ -20,7 +20,7 @@ struct FwdInfo {
 };
 
 struct AdjInfo {
-    int precision, temporal, lane;
+    int precision, temporal, launch, lane;
     const void* args;  // AdjTileArgs<R>[n_sims] on the device
     TileList tl;
     int* step;
@@ -29,8 +29,10 @@ struct AdjInfo {
 };
 
 // Bring a forward plan up to date for steps [n, n + count) (layout, tables, argument
-// arrays, probe rows from probe_row0) and set its step counter on its lane.
-int fwd_prepare(EtlPlan* p, int64_t n, int64_t count, int probe_row0);
+// arrays, probe rows from probe_row0) and set its step counter on its lane — or, with
+// counter_out, return the three counter values to set instead.
+int fwd_prepare(EtlPlan* p, int64_t n, int64_t count, int probe_row0,
+                int* counter_out = nullptr);
 void fwd_info(const EtlPlan* p, FwdInfo* out);
 void fwd_set_parity(EtlPlan* p, int parity);
 // After a trace chunk of `rows` steps: probe copies, then the DFT fold (or a counter
@@ -41,8 +43,10 @@ int fwd_after_chunk(EtlPlan* p, int rows, hipStream_t s);
 int fwd_checkpoint(EtlPlan* p, int seg, int restore);
 
 // Bring an adjoint plan up to date for reverse steps n_first + count - 1 .. n_first
-// (history rows from hist_row0) and set its step record on its lane.
-int adj_prepare(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0);
+// (history rows from hist_row0) and set its step record on its lane — or, with
+// step_out, return the two values to set instead.
+int adj_prepare(EtlAdjPlan* p, int64_t n_first, int count, int hist_row0,
+                int* step_out = nullptr);
 void adj_info(const EtlAdjPlan* p, AdjInfo* out);
 void adj_set_parity(EtlAdjPlan* p, int parity);
 

@@ -25,6 +25,23 @@ struct EtlCoPlan {
 
 namespace {
 
+// The step counter of the forward plan and the step record of the adjoint plan of a
+// co-plan run, in one launch.
+__global__ void co_set_kernel(int* counter, int c0, int c1, int c2, int* step, int s0,
+                              int s1) {
+    if (threadIdx.x == 0) {
+        if (counter) {
+            counter[0] = c0;
+            counter[1] = c1;
+            counter[2] = c2;
+        }
+        if (step) {
+            step[0] = s0;
+            step[1] = s1;
+        }
+    }
+}
+
 void free_co_graphs(EtlCoPlan* c) {
     for (auto& e : c->graphs) {
         (void)hipGraphExecDestroy(e.exec);
@@ -59,6 +76,7 @@ int co_enqueue(EtlCoPlan* c, const FwdInfo& fi, int count_f, const AdjInfo& ai, 
         ab.push_back(make_int2(d, rtop));
         rtop -= d;
     }
+    const bool split = fi.launch != 1 || ai.launch != 1;
     size_t i = 0, j = 0;
     int nf = 0, na = 0;
     while (i < fb.size() || j < ab.size()) {
@@ -74,9 +92,18 @@ int co_enqueue(EtlCoPlan* c, const FwdInfo& fi, int count_f, const AdjInfo& ai, 
         if (do_a)
             as = adj_set<R>(static_cast<const AdjTileArgs<R>*>(ai.args), ai.tl, ai.step, ab[j].y,
                             ai.parity ^ (na & 1));
-        if (int rc = launch_joint<R>(do_f ? &fs : nullptr, do_f ? df : 0, do_a ? &as : nullptr,
-                                     do_a ? da : 0, s))
+        // Small plans (launch mode 1): one launch over every tile kind.  Large ones: the
+        // interior tiles of both plans in one launch (few registers, many waves per CU
+        // for the bandwidth-bound bulk), then the strip and source/monitor tiles.
+        if (split) {
+            for (int part : {int(kInteriorOnly), int(kEdgeOnly)})
+                if (int rc = launch_joint<R>(do_f ? &fs : nullptr, do_f ? df : 0,
+                                             do_a ? &as : nullptr, do_a ? da : 0, s, part))
+                    return rc;
+        } else if (int rc = launch_joint<R>(do_f ? &fs : nullptr, do_f ? df : 0,
+                                            do_a ? &as : nullptr, do_a ? da : 0, s)) {
             return rc;
+        }
         if (do_f) {
             if (fb[i].chunk_rows)
                 if (int rc = fwd_after_chunk(c->f, fb[i].chunk_rows, s)) return rc;
@@ -104,11 +131,15 @@ int co_run(EtlCoPlan* c, int64_t n, int count_f, int probe_row0, int64_t n_first
     adj_info(c->a, &ai);
     ETL_REQUIRE(fi.lane == ai.lane, "etl_coplan_run: the plans are on different lanes");
     int rc;
-    if (count_f > 0 && (rc = fwd_prepare(c->f, n, count_f, probe_row0))) return rc;
-    if (count_a > 0 && (rc = adj_prepare(c->a, n_first, count_a, hist_row0))) return rc;
+    int cv[3] = {0, 0, 0}, sv[2] = {0, 0};
+    if (count_f > 0 && (rc = fwd_prepare(c->f, n, count_f, probe_row0, cv))) return rc;
+    if (count_a > 0 && (rc = adj_prepare(c->a, n_first, count_a, hist_row0, sv))) return rc;
     fwd_info(c->f, &fi);  // after the layouts are up to date
     adj_info(c->a, &ai);
     hipStream_t s = lane_stream(fi.lane);
+    hipLaunchKernelGGL(co_set_kernel, dim3(1), dim3(64), 0, s, count_f > 0 ? fi.counter : nullptr,
+                       cv[0], cv[1], cv[2], count_a > 0 ? ai.step : nullptr, sv[0], sv[1]);
+    ETL_LAUNCHED();
     EtlCoPlan::Entry* hit = nullptr;
     for (auto& e : c->graphs)
         if (e.count_f == count_f && e.count_a == count_a && e.fpar == fi.parity &&

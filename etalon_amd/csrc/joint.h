@@ -37,10 +37,6 @@ struct AdjSet {
     int parity;
 };
 
-// Plans of at most this many tiles (summed over their simulations, at the full band)
-// default to one launch per step block.
-constexpr int64_t kSingleLaunchTiles = 4096;
-
 // Workgroups of a kind with n tiles: a multiple of 8, so that every range starts on
 // XCD 0 and the XCD-major order of tile_body holds within it.
 __host__ __device__ inline int joint_groups(int n) { return 8 * ((n + 7) / 8); }
@@ -75,101 +71,144 @@ inline bool joint_depths_supported(int tf, int ta) {
 
 // Instantiations, compiled in separate files: forward only, adjoint only, both.
 template <typename R>
-int launch_joint_fwd(const FwdSet<R>* f, int tf, hipStream_t s);
+int launch_joint_fwd(const FwdSet<R>* f, int tf, int part, hipStream_t s);
 template <typename R>
-int launch_joint_adj(const AdjSet<R>* a, int ta, hipStream_t s);
+int launch_joint_adj(const AdjSet<R>* a, int ta, int part, hipStream_t s);
 template <typename R>
-int launch_joint_pair(const FwdSet<R>* f, const AdjSet<R>* a, int t, hipStream_t s);
+int launch_joint_pair(const FwdSet<R>* f, const AdjSet<R>* a, int t, int part, hipStream_t s);
 
-// One step block: forward depth tf (0: no forward tiles), adjoint depth ta (0: none).
+// One step block: forward depth tf (0: no forward tiles), adjoint depth ta (0: none),
+// the tile kinds of `part` (a JointPart).
 template <typename R>
-int launch_joint(const FwdSet<R>* f, int tf, const AdjSet<R>* a, int ta, hipStream_t s) {
+int launch_joint(const FwdSet<R>* f, int tf, const AdjSet<R>* a, int ta, hipStream_t s,
+                 int part = 0) {
     if (!joint_depths_supported(tf, ta)) {
         set_error("joint step: no compiled depths (%d, %d)", tf, ta);
         return ETL_EARG;
     }
-    if (ta == 0) return launch_joint_fwd<R>(f, tf, s);
-    if (tf == 0) return launch_joint_adj<R>(a, ta, s);
-    return launch_joint_pair<R>(f, a, tf, s);
+    if (ta == 0) return launch_joint_fwd<R>(f, tf, part, s);
+    if (tf == 0) return launch_joint_adj<R>(a, ta, part, s);
+    return launch_joint_pair<R>(f, a, tf, part, s);
 }
 
-template <typename R, int TF, int TA>
-__global__ __launch_bounds__(64) void joint_kernel(FwdSet<R> f, AdjSet<R> a) {
-    __shared__ R lds[kSubBand * 64];
+// Which tile kinds a launch covers: every kind, the interior tiles only (light: few
+// registers, bandwidth-bound), or all but the interior tiles (the CPML strips and the
+// source/monitor tiles: heavy, latency-bound, few).
+enum JointPart { kAllKinds = 0, kInteriorOnly = 1, kEdgeOnly = 2 };
+
+__host__ __device__ constexpr bool part_has(int part, int kind) {
+    return part == kAllKinds || (part == kInteriorOnly ? kind == 0 : kind != 0);
+}
+
+// One kind K of a forward (adjoint) step block: when workgroup b falls in its range
+// it runs its tile and returns true, else b moves past the range.
+template <typename R, int T, int K, int BR, bool STRIP, bool EXTRA>
+__device__ __forceinline__ bool fwd_kind(const FwdSet<R>& f, int& b, R* lds) {
+    const int g = joint_groups(f.n[K]);
+    if (b >= g) {
+        b -= g;
+        return false;
+    }
+    tile_body<R, 1, T, BR, STRIP, EXTRA>(f.args, f.tiles[K], f.n[K], b, f.n[K], 1, 0, f.counter,
+                                         f.kstep0, f.parity, lds);
+    return true;
+}
+
+template <typename R, int T, int K, int BR, bool STRIP, bool EXTRA>
+__device__ __forceinline__ bool adj_kind(const AdjSet<R>& a, int& b) {
+    const int g = joint_groups(a.n[K]);
+    if (b >= g) {
+        b -= g;
+        return false;
+    }
+    adj_tile_body<R, 1, T, BR, STRIP, EXTRA>(a.args, a.tiles[K], a.n[K], b, a.n[K], 1, 0, a.step,
+                                             a.rtop, a.parity);
+    return true;
+}
+
+// Workgroup b of a joint launch, in the order: the forward plan's strip, extra and
+// extra-strip tiles, the adjoint plan's, then the forward and the adjoint interior
+// tiles.  The latency-bound CPML and source/monitor tiles come first, so they are
+// dispatched before the bulk of interior tiles fills the GPU and finish under it.
+template <typename R, int TF, int TA, int PART>
+__device__ __forceinline__ void joint_body(const FwdSet<R>& f, const AdjSet<R>& a) {
+    __shared__ R lds[part_has(PART, 2) ? kSubBand * 64 : 1];
     int b = blockIdx.x;
-    if constexpr (TF > 0) {
-        int g = joint_groups(f.n[0]);
-        if (b < g) {
-            tile_body<R, 1, TF, 8, false, false>(f.args, f.tiles[0], f.n[0], b, f.n[0], 1, 0,
-                                                 f.counter, f.kstep0, f.parity, lds);
-            return;
+#ifndef ETL_JOINT_EDGE_FIRST
+#define ETL_JOINT_EDGE_FIRST 1
+#endif
+    if constexpr (!ETL_JOINT_EDGE_FIRST && part_has(PART, 0)) {
+        if constexpr (TF > 0) {
+            if (fwd_kind<R, TF, 0, 8, false, false>(f, b, lds)) return;
         }
-        b -= g;
-        g = joint_groups(f.n[1]);
-        if (b < g) {
-            tile_body<R, 1, TF, kSubBand, true, false>(f.args, f.tiles[1], f.n[1], b, f.n[1], 1,
-                                                       0, f.counter, f.kstep0, f.parity, lds);
-            return;
+        if constexpr (TA > 0) {
+            if (adj_kind<R, TA, 0, 8, false, false>(a, b)) return;
         }
-        b -= g;
-        g = joint_groups(f.n[2]);
-        if (b < g) {
-            tile_body<R, 1, TF, kSubBand, false, true>(f.args, f.tiles[2], f.n[2], b, f.n[2], 1,
-                                                       0, f.counter, f.kstep0, f.parity, lds);
-            return;
-        }
-        b -= g;
-        g = joint_groups(f.n[3]);
-        if (b < g) {
-            tile_body<R, 1, TF, kSubBand, true, true>(f.args, f.tiles[3], f.n[3], b, f.n[3], 1,
-                                                      0, f.counter, f.kstep0, f.parity, lds);
-            return;
-        }
-        b -= g;
     }
-    if constexpr (TA > 0) {
-        int g = joint_groups(a.n[0]);
-        if (b < g) {
-            adj_tile_body<R, 1, TA, 8, false, false>(a.args, a.tiles[0], a.n[0], b, a.n[0], 1, 0,
-                                                     a.step, a.rtop, a.parity);
-            return;
+    if constexpr (part_has(PART, 1)) {
+        if constexpr (TF > 0) {
+            if (fwd_kind<R, TF, 1, kSubBand, true, false>(f, b, lds) ||
+                fwd_kind<R, TF, 2, kSubBand, false, true>(f, b, lds) ||
+                fwd_kind<R, TF, 3, kSubBand, true, true>(f, b, lds))
+                return;
         }
-        b -= g;
-        g = joint_groups(a.n[1]);
-        if (b < g) {
-            adj_tile_body<R, 1, TA, kSubBand, true, false>(a.args, a.tiles[1], a.n[1], b, a.n[1],
-                                                           1, 0, a.step, a.rtop, a.parity);
-            return;
+        if constexpr (TA > 0) {
+            if (adj_kind<R, TA, 1, kSubBand, true, false>(a, b) ||
+                adj_kind<R, TA, 2, kSubBand, false, true>(a, b) ||
+                adj_kind<R, TA, 3, kSubBand, true, true>(a, b))
+                return;
         }
-        b -= g;
-        g = joint_groups(a.n[2]);
-        if (b < g) {
-            adj_tile_body<R, 1, TA, kSubBand, false, true>(a.args, a.tiles[2], a.n[2], b, a.n[2],
-                                                           1, 0, a.step, a.rtop, a.parity);
-            return;
+    }
+    if constexpr (ETL_JOINT_EDGE_FIRST && part_has(PART, 0)) {
+        if constexpr (TF > 0) {
+            if (fwd_kind<R, TF, 0, 8, false, false>(f, b, lds)) return;
         }
-        b -= g;
-        g = joint_groups(a.n[3]);
-        if (b < g)
-            adj_tile_body<R, 1, TA, kSubBand, true, true>(a.args, a.tiles[3], a.n[3], b, a.n[3],
-                                                          1, 0, a.step, a.rtop, a.parity);
+        if constexpr (TA > 0) adj_kind<R, TA, 0, 8, false, false>(a, b);
     }
 }
 
-template <typename R, int TF, int TA>
+// kJointWaves: the waves per SIMD the register allocation of a joint kernel must
+// allow.  Its heaviest tiles (strip, source/monitor) need more registers than the
+// interior tiles; capped, they spill a little, and the interior tiles, which are most
+// of the work and bandwidth-bound, keep enough waves in flight.
+#ifndef ETL_JOINT_WAVES
+#define ETL_JOINT_WAVES 3
+#endif
+constexpr int kJointWaves = ETL_JOINT_WAVES;
+
+template <typename R, int TF, int TA, int PART>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kJointWaves)))
+void joint_kernel(FwdSet<R> f, AdjSet<R> a) {
+    joint_body<R, TF, TA, PART>(f, a);
+}
+
+template <typename R, int TF, int TA, int PART>
 int launch_joint_t(const FwdSet<R>* f, const AdjSet<R>* a, hipStream_t s) {
     int groups = 0;
     FwdSet<R> fz{};
     AdjSet<R> az{};
     for (int k = 0; k < 4; ++k) {
+        if (!part_has(PART, k)) continue;
         if (TF > 0) groups += joint_groups(f->n[k]);
         if (TA > 0) groups += joint_groups(a->n[k]);
     }
     if (groups == 0) return ETL_OK;
-    hipLaunchKernelGGL((joint_kernel<R, TF, TA>), dim3(unsigned(groups)), dim3(64), 0, s,
+    hipLaunchKernelGGL((joint_kernel<R, TF, TA, PART>), dim3(unsigned(groups)), dim3(64), 0, s,
                        TF > 0 ? *f : fz, TA > 0 ? *a : az);
     ETL_LAUNCHED();
     return ETL_OK;
+}
+
+// launch_joint_t at a run-time part.
+template <typename R, int TF, int TA>
+int launch_joint_part(const FwdSet<R>* f, const AdjSet<R>* a, int part, hipStream_t s) {
+    switch (part) {
+        case kAllKinds: return launch_joint_t<R, TF, TA, kAllKinds>(f, a, s);
+        case kInteriorOnly: return launch_joint_t<R, TF, TA, kInteriorOnly>(f, a, s);
+        case kEdgeOnly: return launch_joint_t<R, TF, TA, kEdgeOnly>(f, a, s);
+    }
+    set_error("joint step: unknown part %d", part);
+    return ETL_EARG;
 }
 
 }  // namespace etl

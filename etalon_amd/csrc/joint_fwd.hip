@@ -4,23 +4,23 @@
 namespace etl {
 
 template <typename R>
-int launch_joint_fwd_any(const FwdSet<R>* f, int tf, hipStream_t s) {
+int launch_joint_fwd_any(const FwdSet<R>* f, int tf, int part, hipStream_t s) {
     switch (tf) {
-        case 1: return launch_joint_t<R, 1, 0>(f, nullptr, s);
-        case 2: return launch_joint_t<R, 2, 0>(f, nullptr, s);
-        case 4: return launch_joint_t<R, 4, 0>(f, nullptr, s);
+        case 1: return launch_joint_part<R, 1, 0>(f, nullptr, part, s);
+        case 2: return launch_joint_part<R, 2, 0>(f, nullptr, part, s);
+        case 4: return launch_joint_part<R, 4, 0>(f, nullptr, part, s);
     }
     set_error("joint step: forward depth %d not compiled", tf);
     return ETL_EARG;
 }
 
 template <>
-int launch_joint_fwd<float>(const FwdSet<float>* f, int tf, hipStream_t s) {
-    return launch_joint_fwd_any<float>(f, tf, s);
+int launch_joint_fwd<float>(const FwdSet<float>* f, int tf, int part, hipStream_t s) {
+    return launch_joint_fwd_any<float>(f, tf, part, s);
 }
 template <>
-int launch_joint_fwd<double>(const FwdSet<double>* f, int tf, hipStream_t s) {
-    return launch_joint_fwd_any<double>(f, tf, s);
+int launch_joint_fwd<double>(const FwdSet<double>* f, int tf, int part, hipStream_t s) {
+    return launch_joint_fwd_any<double>(f, tf, part, s);
 }
 
 }  // namespace etl

@@ -5,11 +5,12 @@
 namespace etl {
 
 template <>
-int launch_joint_pair<double>(const FwdSet<double>* f, const AdjSet<double>* a, int t, hipStream_t s) {
+int launch_joint_pair<double>(const FwdSet<double>* f, const AdjSet<double>* a, int t, int part,
+                            hipStream_t s) {
     switch (t) {
-        case 1: return launch_joint_t<double, 1, 1>(f, a, s);
-        case 2: return launch_joint_t<double, 2, 2>(f, a, s);
-        case 4: return launch_joint_t<double, 4, 4>(f, a, s);
+        case 1: return launch_joint_part<double, 1, 1>(f, a, part, s);
+        case 2: return launch_joint_part<double, 2, 2>(f, a, part, s);
+        case 4: return launch_joint_part<double, 4, 4>(f, a, part, s);
     }
     set_error("joint step: depth %d not compiled", t);
     return ETL_EARG;

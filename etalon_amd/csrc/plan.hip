@@ -37,16 +37,27 @@ struct CkEntry {
     int64_t words;
 };
 
+// 16-byte moves where both ends are 16-byte aligned (the fields are: etl_malloc
+// blocks are 256-byte aligned, and a checkpoint slot starts at seg * words words),
+// word moves otherwise and for the tail.
 __global__ void ckpt_copy_kernel(const CkEntry* __restrict__ tab, int seg, int parity,
                                  int restore) {
     const CkEntry e = tab[blockIdx.y];
-    uint32_t* __restrict__ ck = e.ck + seg * e.words;
-    uint32_t* __restrict__ f = e.f[parity];
+    uint32_t* ck = e.ck + seg * e.words;
+    uint32_t* f = e.f[parity];
+    const uint32_t* src = restore ? ck : f;
+    uint32_t* dst = restore ? f : ck;
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < e.words; i += stride) {
-        if (restore) f[i] = ck[i];
-        else ck[i] = f[i];
+    const int64_t t0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t done = 0;
+    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0) {
+        const int64_t quads = e.words / 4;
+        const uint4* s4 = reinterpret_cast<const uint4*>(src);
+        uint4* d4 = reinterpret_cast<uint4*>(dst);
+        for (int64_t i = t0; i < quads; i += stride) d4[i] = s4[i];
+        done = quads * 4;
     }
+    for (int64_t i = done + t0; i < e.words; i += stride) dst[i] = src[i];
 }
 
 // flag[0] = 1 if psi is non-zero anywhere the fused kernel assumes it stays +0:
@@ -440,12 +451,6 @@ int upload_args(EtlPlan* p) {
     return p->precision == ETL_F32 ? upload_args_t<float>(p) : upload_args_t<double>(p);
 }
 
-// Enqueue `count` (<= trace_chunk) steps starting at buffer parity `parity`: fused
-// launches of depth T, then depth-1 launches for the remainder, and the DFT of the
-// trace rows (which advances the step counter) or a counter bump.  Returns the
-// number of fused launches in *launches.  `events` (optional) receives four events
-// per fused launch: fork, interior kernel start and end (on the main stream, where
-// only that kernel runs), join.
 // After `count` steps of a trace chunk: probe copies, then the DFT fold of the trace
 // rows (which advances the step counter) or a counter bump.
 template <typename R>
@@ -464,6 +469,12 @@ int after_chunk(EtlPlan* p, int count, hipStream_t s) {
     return launch_bump(p->counter, count, s);
 }
 
+// Enqueue `count` (<= trace_chunk) steps starting at buffer parity `parity`: fused
+// launches of depth T, then depth-1 launches for the remainder, and the DFT of the
+// trace rows (which advances the step counter) or a counter bump.  Returns the
+// number of fused launches in *launches.  `events` (optional) receives four events
+// per fused launch: fork, interior kernel start and end (on the side stream where
+// only that kernel runs), join.
 template <typename R>
 int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* launches) {
     const FusedArgs<R>* args = static_cast<const FusedArgs<R>*>(p->args_dev);
@@ -492,16 +503,21 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
             continue;
         }
         if (events) ETL_HIP(hipEventRecord(events[4 * n], s));
-        // fork: strip and extra tiles run on their own streams, joined before the next step
+        // fork: the strip tiles run on the main stream, which needs no cross-stream wait
+        // and so is dispatched first — the latency-bound CPML tiles get their CUs before
+        // the bulk of interior tiles fills the GPU; the interior tiles (side stream
+        // s_strip) and the extra tiles (s_extra) wait for the fork; all join before the
+        // next step.
+        hipStream_t s_int = p->s_strip;
         ETL_HIP(hipEventRecord(p->ev_fork, s));
-        ETL_HIP(hipStreamWaitEvent(p->s_strip, p->ev_fork, 0));
+        ETL_HIP(hipStreamWaitEvent(s_int, p->ev_fork, 0));
         ETL_HIP(hipStreamWaitEvent(p->s_extra, p->ev_fork, 0));
-        if (events) ETL_HIP(hipEventRecord(events[4 * n + 1], s));
+        if (events) ETL_HIP(hipEventRecord(events[4 * n + 1], s_int));
         int rc = launch_fused<R>(args, tl, p->tune.vec, depth, p->tune.band_rows, p->tune.waves,
-                                 p->counter, k, parity ^ (n & 1), s, p->s_strip, p->s_extra);
+                                 p->counter, k, parity ^ (n & 1), s_int, s, p->s_extra);
         if (rc) return rc;
-        if (events) ETL_HIP(hipEventRecord(events[4 * n + 2], s));
-        ETL_HIP(hipEventRecord(p->ev_join, p->s_strip));
+        if (events) ETL_HIP(hipEventRecord(events[4 * n + 2], s_int));
+        ETL_HIP(hipEventRecord(p->ev_join, s_int));
         ETL_HIP(hipEventRecord(p->ev_join2, p->s_extra));
         ETL_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
         ETL_HIP(hipStreamWaitEvent(s, p->ev_join2, 0));
@@ -628,7 +644,10 @@ int check_tables(EtlPlan* p, int64_t n, int64_t count, int64_t* n0) {
     return ETL_OK;
 }
 
-int prepare_steps(EtlPlan* p, int64_t n, int64_t count, int probe_row0) {
+// `counter_out` non-null: the three counter values are returned there for the caller
+// to set (a co-plan sets both plans' counters in one launch) instead of launched here.
+int prepare_steps(EtlPlan* p, int64_t n, int64_t count, int probe_row0,
+                  int* counter_out = nullptr) {
     int rc;
     if (p->dirty) {
         if ((rc = build_layout(p))) return rc;
@@ -646,6 +665,12 @@ int prepare_steps(EtlPlan* p, int64_t n, int64_t count, int probe_row0) {
                         (long long)count, probe_row0, pr.out_rows);
     const int64_t start = n - n0;
     ETL_REQUIRE(start >= 0 && start + count < (int64_t(1) << 31), "etl_plan_advance: step range");
+    if (counter_out) {
+        counter_out[0] = int(start);
+        counter_out[1] = 0;
+        counter_out[2] = int(start) - probe_row0;
+        return ETL_OK;
+    }
     hipStream_t s = lane_stream(p->lane);
     hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(64), 0, s, p->counter, int(start),
                        probe_row0);
@@ -717,7 +742,8 @@ int ck_copy(EtlPlan* p, int seg, int restore) {
     int64_t words = 0;
     for (const auto& sim : p->sims)
         for (int k = 0; k < 7; ++k) words = std::max(words, field_elems(sim, k) * p->elem() / 4);
-    const int gx = int(std::min<int64_t>(1024, (words + 2047) / 2048));
+    // one 16-byte move per thread and block row for the largest field, at most 2048 blocks
+    const int gx = int(std::max<int64_t>(1, std::min<int64_t>(2048, (words / 4 + 255) / 256)));
     hipLaunchKernelGGL(ckpt_copy_kernel, dim3(unsigned(gx), unsigned(p->ck_entries)), dim3(256),
                        0, lane_stream(p->lane), p->ck_tab, seg, p->parity, restore);
     ETL_LAUNCHED();
@@ -853,8 +879,8 @@ EtlPlan::Sim* sim_of(EtlPlan* p, int sim) {
 
 namespace etl {
 
-int fwd_prepare(EtlPlan* p, int64_t n, int64_t count, int probe_row0) {
-    return prepare_steps(p, n, count, probe_row0);
+int fwd_prepare(EtlPlan* p, int64_t n, int64_t count, int probe_row0, int* counter_out) {
+    return prepare_steps(p, n, count, probe_row0, counter_out);
 }
 
 void fwd_info(const EtlPlan* p, FwdInfo* out) {
@@ -924,11 +950,7 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     p->sims.resize(n_sims);
     int rc = ETL_OK;
     for (int k = 0; k < n_sims && rc == ETL_OK; ++k) rc = init_sim(p, &descs[k], p->sims[k]);
-    if (rc == ETL_OK && p->tune.launch == 0) {
-        int64_t tiles = 0;
-        for (const auto& sim : p->sims) tiles += int64_t(p->n_bands(sim)) * p->n_tj(sim);
-        p->tune.launch = (t.vec == 1 && t.band_rows == 8 && tiles <= kSingleLaunchTiles) ? 1 : 3;
-    }
+    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = (t.vec == 1 && t.band_rows == 8) ? 1 : 3;
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||

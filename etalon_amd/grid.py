@@ -143,15 +143,53 @@ def _paint(values: np.ndarray, x: np.ndarray, y: np.ndarray, s, z: float) -> Non
         raise TypeError(f"unknown structure type {kind!r}")
 
 
+def _paint_cylinders(values: np.ndarray, xs: np.ndarray, ys: np.ndarray, cyls) -> None:
+    """Paint cylinders of one permittivity at once: each one's test (the Cylinder rule
+    of ``_paint``, the same arithmetic) runs on the block of nodes around it only, so a
+    lattice of thousands of holes costs a few vector operations."""
+    cx = np.array([c.center[0] for c in cyls], dtype=np.float64)
+    cy = np.array([c.center[1] for c in cyls], dtype=np.float64)
+    r = np.array([c.radius for c in cyls], dtype=np.float64)
+    reach = float(r.max()) + _TOL
+    dx = float(np.min(np.diff(xs))) if xs.size > 1 else 1.0
+    dy = float(np.min(np.diff(ys))) if ys.size > 1 else 1.0
+    hx, hy = int(np.ceil(reach / dx)) + 1, int(np.ceil(reach / dy)) + 1
+    ii = np.clip(np.searchsorted(xs, cx)[:, None] + np.arange(-hx, hx + 1)[None, :], 0, xs.size - 1)
+    jj = np.clip(np.searchsorted(ys, cy)[:, None] + np.arange(-hy, hy + 1)[None, :], 0, ys.size - 1)
+    inside = ((xs[ii][:, :, None] - cx[:, None, None]) ** 2
+              + (ys[jj][:, None, :] - cy[:, None, None]) ** 2) <= (r[:, None, None] + _TOL) ** 2
+    n, a, b = np.nonzero(inside)
+    values[ii[n, a], jj[n, b]] = cyls[0].medium.permittivity
+
+
 def permittivity_on_nodes(sim, grid: YeeGrid2D) -> dict[str, np.ndarray]:
-    """ε_r at the Ex and Ey nodes, staircased (etalon/grid/materials.py:12-35)."""
+    """ε_r at the Ex and Ey nodes, staircased (etalon/grid/materials.py:12-35).
+    Consecutive cylinders of one permittivity (a photonic-crystal lattice) are painted
+    together; the order among them does not matter, their value being the same."""
     eps: dict[str, np.ndarray] = {}
+    structures = list(sim.structures)
     for component in ("Ex", "Ey"):
         xs, ys = grid.coords(component)
         x, y = np.meshgrid(xs, ys, indexing="ij")
         values = np.full(x.shape, sim.medium.permittivity)
-        for s in sim.structures:
-            _paint(values, x, y, s, grid.z)
+        k = 0
+        while k < len(structures):
+            s = structures[k]
+            if s.type != "Cylinder":
+                _paint(values, x, y, s, grid.z)
+                k += 1
+                continue
+            m = k + 1
+            while (m < len(structures) and structures[m].type == "Cylinder"
+                   and structures[m].medium.permittivity == s.medium.permittivity):
+                m += 1
+            run = [c for c in structures[k:m] if _in_plane(grid.z, c.center[2], c.height)]
+            if len(run) == 1:
+                _paint(values, x, y, run[0], grid.z)
+            elif run:
+                _paint_cylinders(values, np.asarray(xs, np.float64), np.asarray(ys, np.float64),
+                                 run)
+            k = m
         eps[component] = values
     return eps
 

@@ -26,8 +26,8 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h, per simulation
 class Tune:
     """Fused-kernel shape; 0 = the library default (vec 1, 8-row tiles, temporal
     depth 4, 64-step trace chunks, one wave per workgroup, and ``launch`` 1 — one
-    kernel per step block over every tile kind — for plans of at most 4096 tiles,
-    else 3 — one kernel per tile kind on three streams)."""
+    kernel per step block over every tile kind — where the shape allows it (vec 1,
+    band 8), else 3 — one kernel per tile kind on three streams)."""
 
     band_rows: int = 0
     vec: int = 0

@@ -145,3 +145,71 @@ def adjoint_design_sim(precision: str = "single", seed: int = 0, steps: int = 40
                                 wavelengths=np.linspace(1.50, 1.60, 5), components=("Hz",))],
         run_time=_run_time_for(steps), courant=0.99, precision=precision, backend="hip")
     return sim, design, ("out",)
+
+
+LATTICE = 0.42       # photonic-crystal lattice constant, µm
+HOLE_RADIUS = 0.126  # air holes, µm
+
+
+def phc_holes(half: float, lattice: float = LATTICE, radius: float = HOLE_RADIUS,
+              keep_out: float = 0.0) -> list:
+    """Air holes of a triangular lattice (rows along x, spacing ``lattice``, every
+    other row shifted by half a period, rows ``lattice`` √3/2 apart) covering
+    |x|, |y| <= half, without the row at y = 0 (a W1 line-defect waveguide) and
+    without holes whose centre lies in the square |x|, |y| < keep_out."""
+    air = et.Medium(n=1.0)
+    dy = lattice * np.sqrt(3.0) / 2.0
+    n_rows = int(np.ceil(half / dy))
+    n_cols = int(np.ceil(half / lattice)) + 1
+    holes = []
+    for row in range(-n_rows, n_rows + 1):
+        if row == 0:
+            continue
+        y = row * dy
+        shift = 0.5 * lattice if row % 2 else 0.0
+        for col in range(-n_cols, n_cols + 1):
+            x = col * lattice + shift
+            if abs(x) > half or abs(y) > half:
+                continue
+            if abs(x) < keep_out and abs(y) < keep_out:
+                continue
+            holes.append(et.Cylinder(center=(float(x), float(y), 0.0), radius=radius,
+                                     medium=air))
+    return holes
+
+
+def phc_adjoint_candidates(n: int = 32, seed: int = 0, steps: int = 8000, interior: int = 1000,
+                           region: float = 4.0, pixels: int = 80,
+                           precision: str = "single") -> tuple[list, list, tuple]:
+    """``n`` design candidates on ``interior`` x ``interior`` cells at dl = 0.05 µm +
+    12-layer CPML: a silicon slab (ε 12.11) with a triangular lattice of air holes
+    (a = 0.42 µm, r = 0.126 µm) and a W1 line defect along y = 0, a ``region`` µm
+    square design region (``pixels`` x ``pixels``, seeded ε in [2.085, 12.11], one
+    draw per candidate) at the centre, an Hz source in the defect 4 µm before the
+    region, a 3 µm Hz line monitor across the defect 4 µm after it at 9 wavelengths
+    1.50–1.60 µm; ``steps`` steps.  Returns (simulations, design
+    permittivities, objective names); the candidates share every structure but the
+    design region."""
+    rng = np.random.default_rng(seed)
+    size = interior * DL
+    half = size / 2.0
+    silicon = et.Medium(n=float(np.sqrt(SILICON)))
+    base_design = et.PixelGrid(center=(0.0, 0.0, 0.0), size=(region, region, 0.0),
+                               permittivity=np.full((pixels, pixels), SILICON))
+    base = et.Simulation(
+        size=(size, size, 0.0), grid=et.UniformGrid(dl=DL), medium=silicon,
+        structures=[*phc_holes(half + 12 * DL, keep_out=region / 2.0), base_design],
+        boundary=et.PML(layers=12),
+        sources=[et.PointSource(center=(-region / 2.0 - 4.0, 0.0, 0.0), component="Hz",
+                                pulse=et.GaussianPulse(wavelength=1.55, bandwidth=0.15))],
+        monitors=[et.DFTMonitor(name="out", center=(region / 2.0 + 4.0, 0.0, 0.0),
+                                size=(0.0, 3.0, 0.0),
+                                wavelengths=np.linspace(1.50, 1.60, 9), components=("Hz",))],
+        run_time=_run_time_for(steps), courant=0.99, precision=precision, backend="hip")
+    sims, eps = [], []
+    for _ in range(n):
+        e = rng.uniform(OXIDE, SILICON, size=(pixels, pixels))
+        design = base_design.model_copy(update={"permittivity": e})
+        sims.append(base.model_copy(update={"structures": (*base.structures[:-1], design)}))
+        eps.append(e)
+    return sims, eps, ("out",)

@@ -141,7 +141,7 @@ typedef struct EtlTune {
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
                         * band 8; one wave per workgroup); 3: one kernel per tile kind on
-                        * three streams; 0: 1 for grids of at most 4096 tiles, else 3 */
+                        * three streams; 0: 1 where the shape allows it, else 3 */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */

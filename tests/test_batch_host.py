@@ -43,3 +43,40 @@ def test_batch_rejects_mixed_precision_and_step_counts():
         BatchSolver(a + c)
     with pytest.raises(ValueError, match="at least one"):
         BatchSolver([])
+
+
+def test_batch_adjoint_needs_one_step_count():
+    from etalon_amd.adjoint import AdjointProblem
+    from etalon_amd.batch_adjoint import BatchAdjointProblem
+    from tests.adjoint_sims import design_problem
+
+    a, _ = design_problem("single", steps=200)
+    b, _ = design_problem("single", steps=260)
+    with pytest.raises(ValueError, match="step count"):
+        BatchAdjointProblem([AdjointProblem(a, ["out"]), AdjointProblem(b, ["out"])])
+    with pytest.raises(ValueError, match="tile kernels"):
+        BatchAdjointProblem([AdjointProblem(a, ["out"], kernel="plain")])
+
+
+def test_cylinder_lattice_painting_equals_the_oracle():
+    """Runs of cylinders are painted together; the result equals painting them one by
+    one (the oracle), also with holes across the grid edge, overlapping holes and
+    other structures between runs."""
+    import etalon_amd as et
+    from etalon_amd.workloads import phc_holes
+
+    holes = phc_holes(2.2, keep_out=0.5)
+    box = et.Box(center=(0.3, 0.2, 0.0), size=(1.0, 0.4, 0.0), medium=et.Medium(n=2.0))
+    big = et.Cylinder(center=(-1.0, 0.5, 0.0), radius=0.5, medium=et.Medium(n=1.0))
+    sim = et.Simulation(
+        size=(4.0, 3.6, 0.0), grid=et.UniformGrid(dl=0.05), medium=et.Medium(n=3.48),
+        structures=[*holes[:40], box, big, *holes[40:]], boundary=et.PML(layers=8),
+        sources=[et.PointSource(center=(-1.5, 0.0, 0.0), component="Hz",
+                                pulse=et.GaussianPulse(wavelength=1.55, bandwidth=0.2))],
+        run_time=0.05, precision="double", backend="hip")
+    g = build_yee_grid(sim)
+    mine = permittivity_on_nodes(sim, g)
+    theirs = ref.permittivity(sim, ref.make_grid(sim))
+    for comp in ("Ex", "Ey"):
+        assert np.array_equal(mine[comp], theirs[comp]), comp
+        assert (mine[comp] == 1.0).sum() > 100, "holes painted"
