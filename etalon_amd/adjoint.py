@@ -40,7 +40,7 @@ from etalon_amd.hip import _lib
 from etalon_amd.hip.backend import zeros
 from etalon_amd.sampling import dft_phasors
 from etalon_amd.solver import FDTDSolver
-from etalon_amd.state import FIELDS, PROFILES
+from etalon_amd.state import FIELDS, PROFILES, cb_arrays
 
 _COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 
@@ -72,6 +72,36 @@ class Tape:
 
 
 @dataclass
+class _Session:
+    """What a design loop keeps between gradients of one geometry."""
+
+    solver: FDTDSolver
+    checkpoints: dict
+    starts: list
+    bufs: Optional["_AdjointBuffers"] = None
+    rev: Optional["_Reverse"] = None
+    rplan: Optional[object] = None
+    co: C.c_void_p = field(default_factory=C.c_void_p)
+
+    def close(self) -> None:
+        if self.co:
+            _lib.call("etl_coplan_destroy", self.co)
+            self.co = C.c_void_p()
+        if self.rplan is not None:
+            self.rplan.close()
+            self.rplan = None
+        if self.rev is not None:
+            self.rev.close()
+            self.rev = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
 class _AdjointBuffers:
     """λ, ∂J/∂cb and history arrays of one simulation's reverse run."""
 
@@ -85,7 +115,7 @@ class _AdjointBuffers:
 class AdjointProblem:
     def __init__(self, sim, objective: Sequence[str], checkpoint_every: Optional[int] = None,
                  mode: str = "checkpoint", tune=None, kernel: str = "tile",
-                 schedule: str = "joint"):
+                 schedule: str = "joint", reuse: bool = True):
         if not sim.structures or sim.structures[-1].type != "PixelGrid":
             raise ValueError("the design region must be the simulation's last structure, a "
                              "PixelGrid")
@@ -104,6 +134,11 @@ class AdjointProblem:
         # checkpoint mode: how the recompute of segment k - 1 and the reverse steps of
         # segment k share the GPU (the plain kernels run them one after the other)
         self.schedule = schedule if kernel == "tile" else "sequential"
+        # a design loop's gradients (checkpoint mode, tile kernels, joint schedule) keep
+        # the solver, plans, graphs and buffers of the first one: a new permittivity
+        # only replaces cb_ex and cb_ey.  A tape is then valid until the next forward.
+        self.reuse = bool(reuse) and mode == "checkpoint" and self.schedule == "joint"
+        self._session: Optional[_Session] = None
         self.sim = sim
         self.objective = tuple(objective)
         self.mode = mode
@@ -158,6 +193,8 @@ class AdjointProblem:
     # ------------------------------------------------------------------ forward
     def forward(self, eps: np.ndarray) -> Tape:
         """Run the simulation with design permittivity ``eps``; J and the tape."""
+        if self.reuse and self._session is not None:
+            return self._forward_again(eps)
         sim = self.with_permittivity(eps)
         solver = FDTDSolver(sim, tune=self.tune)
         self._check_box_off_cpml(solver)
@@ -171,6 +208,7 @@ class AdjointProblem:
         for mon_id, table in zip(solver._plan_ids, phasors):
             if table is not None:
                 plan.phasor_table(mon_id, 0, table)
+        solver._plan_tables = (0, N)
         tape = Tape(solver=solver, J=0.0)
         if self.mode == "store":
             tape.history = zeros((N, self._box_hz_nodes().size), solver.real)
@@ -186,11 +224,50 @@ class AdjointProblem:
             plan.set_checkpoints([tape.checkpoints[name] for name in FIELDS], n_ck)
             plan.advance_checkpointed(0, N, K)  # slot k = the state at step k K
             plan.bind()
+            if self.reuse:
+                self._session = _Session(solver=solver, checkpoints=tape.checkpoints,
+                                         starts=tape.starts)
         solver.n = N
         backend.synchronize()
         tape.forward_s = time.perf_counter() - t0
         tape.J = self._objective(solver)
         return tape
+
+    def _forward_again(self, eps: np.ndarray) -> Tape:
+        """The forward pass on the session's solver: the design's nodes take the new
+        permittivity (the PixelGrid rule, the last structure wins there), cb_ex and
+        cb_ey are recomputed and uploaded, fields and monitors go back to zero, and the
+        same plan, tables and checkpoint storage run again."""
+        eps = np.asarray(eps, dtype=np.float64)
+        if eps.shape != self.shape:
+            raise ValueError(f"design permittivity must be {self.shape}, got {eps.shape}")
+        ses = self._session
+        solver = ses.solver
+        t0 = time.perf_counter()
+        flat = eps.ravel()
+        for c in ("Ex", "Ey"):
+            ii, jj, p = self.pixmap[c]
+            solver.eps_r[c][ii, jj] = flat[p]
+        cb = cb_arrays(solver.grid, solver.eps_r)
+        solver.state.cb_ex.set(cb["cb_ex"].astype(solver.real))
+        solver.state.cb_ey.set(cb["cb_ey"].astype(solver.real))
+        solver.sim = self.with_permittivity(eps)
+        solver.reset()
+        plan = solver._ensure_plan()
+        plan.advance_checkpointed(0, self.n_steps, self.K)
+        plan.bind()
+        solver.n = self.n_steps
+        solver.backend.synchronize()
+        tape = Tape(solver=solver, J=0.0, checkpoints=ses.checkpoints, starts=ses.starts)
+        tape.forward_s = time.perf_counter() - t0
+        tape.J = self._objective(solver)
+        return tape
+
+    def close(self) -> None:
+        """Release the session (plans, graphs, buffers) of a design loop."""
+        if self._session is not None:
+            self._session.close()
+            self._session = None
 
     def _objective(self, solver) -> float:
         J = 0.0
@@ -230,6 +307,8 @@ class AdjointProblem:
     # ------------------------------------------------------------------ backward
     def backward(self, tape: Tape) -> np.ndarray:
         """∂J/∂ε per design pixel (float64), from the forward tape."""
+        if self.reuse and self._session is not None and tape.solver is self._session.solver:
+            return self._backward_session(tape)
         solver = tape.solver
         N, K = self.n_steps, self.K
         if self.mode == "store":
@@ -251,6 +330,35 @@ class AdjointProblem:
         finally:
             rev.close()
         return self._pixel_gradient(solver, bufs)
+
+    def _backward_session(self, tape: Tape) -> np.ndarray:
+        """The joint-schedule backward pass on the session's plans: built on the first
+        gradient, reset (λ and ∂J/∂cb to zero, the new adjoint-source table in place)
+        on the next ones."""
+        from etalon_amd.hip.plan import FusedPlan
+
+        ses = self._session
+        solver = tape.solver
+        N, K = self.n_steps, self.K
+        comp, flat, table = self._sources(solver)
+        if ses.rev is None:
+            hist = zeros((2 * K, self._box_hz_nodes().size), solver.real)
+            ses.bufs = self._adjoint_buffers(solver, hist)
+            ses.rev = _Reverse("tile", ses.bufs.desc, self.tune)
+            ses.rev.sources(comp, flat, table)
+            ses.rplan = FusedPlan(solver.state, self.tune)
+            self._configure_recompute(ses.rplan, solver, 0, hist)
+            ses.rplan.set_checkpoints([ses.checkpoints[name] for name in FIELDS],
+                                      len(ses.starts))
+            _lib.call("etl_coplan_create", ses.rplan.handle, ses.rev.handle, C.byref(ses.co))
+        else:
+            _lib.call("etl_adjplan_reset", ses.rev.handle)
+            ses.bufs.g_ex.fill_zero()
+            ses.bufs.g_ey.fill_zero()
+            ses.rev.sources(comp, flat, table)
+        _lib.call("etl_coplan_sweep", ses.co, N, K)
+        solver.backend.synchronize()
+        return self._pixel_gradient(solver, ses.bufs)
 
     # ------------------------------------------------------------------ pieces
     def _adjoint_buffers(self, solver, hist) -> "_AdjointBuffers":

@@ -478,7 +478,7 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
         if (rc == ETL_OK)
             rc = precision == ETL_F32 ? adj_strips<float>(d, s.strips) : adj_strips<double>(d, s.strips);
     }
-    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = 1;
+    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = precision == ETL_F32 ? 1 : 3;
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
@@ -533,6 +533,25 @@ int etl_adjplan_sources(etl_adjplan_t p, int sim, int n, const int* comp, const 
     for (int k = 1; k < n; ++k)
         ETL_REQUIRE(!(comp[order[k]] == comp[order[k - 1]] && flat[order[k]] == flat[order[k - 1]]),
                     "etl_adjplan_sources: node listed twice (sum its columns)");
+    const size_t el = p->elem();
+    auto sorted_table = [&]() {
+        std::vector<unsigned char> sorted(size_t(rows) * n * el);
+        const unsigned char* tb = static_cast<const unsigned char*>(table);
+        for (int r = 0; r < rows; ++r)
+            for (int k = 0; k < n; ++k)
+                std::memcpy(&sorted[(size_t(r) * n + k) * el], tb + (size_t(r) * n + order[k]) * el,
+                            el);
+        return sorted;
+    };
+    // The same nodes and rows as before (the next gradient of a design loop): only the
+    // table changes, in place; the layout and the graphs stay.
+    bool same = s.src != nullptr && n > 0 && rows == s.src_rows && size_t(n) == s.comp.size();
+    for (int k = 0; same && k < n; ++k)
+        same = s.comp[k] == comp[order[k]] && s.flat[k] == flat[order[k]];
+    if (same) {
+        const std::vector<unsigned char> sorted = sorted_table();
+        return etl_memcpy_h2d(s.src, sorted.data(), sorted.size());
+    }
     if (int rc_ = sync_lanes()) return rc_;
     s.comp.resize(n);
     s.flat.resize(n);
@@ -545,12 +564,7 @@ int etl_adjplan_sources(etl_adjplan_t p, int sim, int n, const int* comp, const 
     s.src = nullptr;
     s.src_rows = rows;
     if (n && rows) {
-        const size_t el = p->elem();
-        std::vector<unsigned char> sorted(size_t(rows) * n * el);
-        const unsigned char* tb = static_cast<const unsigned char*>(table);
-        for (int r = 0; r < rows; ++r)
-            for (int k = 0; k < n; ++k)
-                std::memcpy(&sorted[(size_t(r) * n + k) * el], tb + (size_t(r) * n + order[k]) * el, el);
+        const std::vector<unsigned char> sorted = sorted_table();
         int rc = upload_bytes(&s.src, sorted.data(), sorted.size());
         if (rc) return rc;
     }
@@ -602,6 +616,23 @@ int etl_adjplan_run_rows(etl_adjplan_t p, int64_t n_first, int count, int hist_r
     }
     ETL_HIP(hipGraphLaunch(p->graph, s));
     p->parity ^= p->graph_launches & 1;
+    return ETL_OK;
+}
+
+int etl_adjplan_reset(etl_adjplan_t p) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_adjplan_reset: null plan");
+    static const int comp_of[7] = {0, 1, 2, 0, 1, 2, 2};
+    for (auto& s : p->sims)
+        for (int f = 0; f < 7; ++f) {
+            const int c = comp_of[f];
+            const size_t bytes = size_t(c == 1 ? s.nx + 1 : s.nx) * (c == 0 ? s.ny + 1 : s.ny) *
+                                 p->elem();
+            int rc = zero_async(s.desc.lam[f], bytes);
+            if (rc == ETL_OK) rc = zero_async(s.lamB[f], bytes);
+            if (rc) return rc;
+        }
+    p->parity = 0;
     return ETL_OK;
 }
 

@@ -167,17 +167,19 @@ __device__ __forceinline__ void joint_body(const FwdSet<R>& f, const AdjSet<R>& 
     }
 }
 
-// kJointWaves: the waves per SIMD the register allocation of a joint kernel must
+// kJointWaves: the waves per SIMD the register allocation of a float joint kernel must
 // allow.  Its heaviest tiles (strip, source/monitor) need more registers than the
 // interior tiles; capped, they spill a little, and the interior tiles, which are most
-// of the work and bandwidth-bound, keep enough waves in flight.
+// of the work and bandwidth-bound, keep enough waves in flight.  Double tiles need
+// twice the registers: capped they would spill in every tile, so they are not.
 #ifndef ETL_JOINT_WAVES
 #define ETL_JOINT_WAVES 3
 #endif
 constexpr int kJointWaves = ETL_JOINT_WAVES;
 
 template <typename R, int TF, int TA, int PART>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kJointWaves)))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? kJointWaves : 1)))
 void joint_kernel(FwdSet<R> f, AdjSet<R> a) {
     joint_body<R, TF, TA, PART>(f, a);
 }

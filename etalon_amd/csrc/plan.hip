@@ -950,7 +950,10 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     p->sims.resize(n_sims);
     int rc = ETL_OK;
     for (int k = 0; k < n_sims && rc == ETL_OK; ++k) rc = init_sim(p, &descs[k], p->sims[k]);
-    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = (t.vec == 1 && t.band_rows == 8) ? 1 : 3;
+    // float: one launch per step block (vec 1, band 8); double tiles need twice the
+    // registers, and their strip tiles in the joint kernel would set its occupancy
+    if (rc == ETL_OK && p->tune.launch == 0)
+        p->tune.launch = (precision == ETL_F32 && t.vec == 1 && t.band_rows == 8) ? 1 : 3;
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
@@ -1277,11 +1280,14 @@ int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, EtlTiming* out, int* p
         ETL_HIP(hipEventElapsedTime(&tot, t0, t1));
         const int L = std::max(launches, 1);
         out->launch_ms = all / L;
-        out->interior_ms = p->n_interior ? interior / L : 0.0;
+        out->kernel_ms = (p->n_interior || p->tune.launch == 1) ? interior / L : 0.0;
         out->total_ms = tot;
-        out->interior_cells = p->interior_cells;
+        int64_t all_cells = 0;
+        for (const auto& sim : p->sims) all_cells += int64_t(sim.nx) * sim.ny;
+        out->kernel_cells = p->tune.launch == 1 ? all_cells : p->interior_cells;
         out->launches = launches;
         out->depth = p->tune.temporal;
+        out->launch = p->tune.launch;
         if (parity) *parity = p->parity;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);

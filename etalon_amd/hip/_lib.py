@@ -78,11 +78,12 @@ class EtlAdjointDesc(C.Structure):
 class EtlTiming(C.Structure):
     _fields_ = [
         ("launch_ms", C.c_double),
-        ("interior_ms", C.c_double),
+        ("kernel_ms", C.c_double),
         ("total_ms", C.c_double),
-        ("interior_cells", C.c_int64),
+        ("kernel_cells", C.c_int64),
         ("launches", C.c_int),
         ("depth", C.c_int),
+        ("launch", C.c_int),
     ]
 
 
@@ -132,6 +133,7 @@ _SIGNATURES = {
     "etl_adjplan_run": [vp, C.c_int64, C.c_int],
     "etl_adjplan_run_rows": [vp, C.c_int64, C.c_int, C.c_int],
     "etl_adjplan_set_lane": [vp, C.c_int],
+    "etl_adjplan_reset": [vp],
     "etl_coplan_create": [vp, vp, C.POINTER(vp)],
     "etl_coplan_destroy": [vp],
     "etl_coplan_run": [vp, C.c_int64, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int],

@@ -26,8 +26,8 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h, per simulation
 class Tune:
     """Fused-kernel shape; 0 = the library default (vec 1, 8-row tiles, temporal
     depth 4, 64-step trace chunks, one wave per workgroup, and ``launch`` 1 — one
-    kernel per step block over every tile kind — where the shape allows it (vec 1,
-    band 8), else 3 — one kernel per tile kind on three streams)."""
+    kernel per step block over every tile kind — for float where the shape allows it
+    (vec 1, band 8), else 3 — one kernel per tile kind on three streams)."""
 
     band_rows: int = 0
     vec: int = 0
@@ -171,9 +171,11 @@ class FusedPlan:
 
     def time(self, n0: int, count: int) -> dict:
         """HIP-event timing of ``count`` steps launched directly: ``launch_ms`` (average
-        fused launch, all tile kinds), ``interior_ms`` (the interior-tile kernel alone,
-        which does most of the work), ``total_ms``, ``interior_cells`` (Hz cells that
-        kernel advances per step), ``launches`` and ``depth`` (steps per launch)."""
+        fused launch, all tile kinds), ``kernel_ms`` (the kernel that advances the
+        interior tiles — with ``launch`` 1 the one kernel of every tile kind, with 3 the
+        interior-tile kernel alone), ``total_ms``, ``kernel_cells`` (Hz cells that
+        kernel advances per step), ``launches``, ``depth`` (steps per launch) and
+        ``launch``."""
         t, parity = _lib.EtlTiming(), C.c_int()
         _lib.call("etl_plan_time", self.handle, int(n0), int(count), C.byref(t), C.byref(parity))
         self.parity = parity.value

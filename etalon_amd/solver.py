@@ -80,6 +80,7 @@ class FDTDSolver:
         self.tune = tune
         self._plan = None
         self._plan_ids: list[int] = []
+        self._plan_tables = (0, 0)
 
     # ---------------------------------------------------------------- one step
     def _inject(self, magnetic: bool) -> None:
@@ -135,6 +136,7 @@ class FDTDSolver:
         plan = FusedPlan(self.state, self.tune)
         self._plan_ids = self.register(plan, 0)
         self._plan = plan
+        self._plan_tables = (0, 0)  # steps [a, b) the plan's tables cover
         return plan
 
     def register(self, plan, sim: int) -> list[int]:
@@ -224,12 +226,17 @@ class FDTDSolver:
         done = 0
         while done < count:
             rows = min(count - done, _PLAN_TABLE_ROWS)
-            delta, phasors = self._tables(self.n, rows)
-            if delta is not None:
-                plan.source_table(self.n, delta)
-            for mon_id, table in zip(self._plan_ids, phasors):
-                if table is not None:
-                    plan.phasor_table(mon_id, self.n, table)
+            have = self._plan_tables
+            if not (have[0] <= self.n and self.n + rows <= have[1]):
+                # the tables depend on the step numbers only: a run from step 0 again
+                # (after reset) reuses what the plan holds
+                delta, phasors = self._tables(self.n, rows)
+                if delta is not None:
+                    plan.source_table(self.n, delta)
+                for mon_id, table in zip(self._plan_ids, phasors):
+                    if table is not None:
+                        plan.phasor_table(mon_id, self.n, table)
+                self._plan_tables = (self.n, self.n + rows)
             plan.advance(self.n, rows)
             plan.bind()
             self.n += rows

@@ -52,11 +52,17 @@ class TEState:
     ny: int
 
 
+def cb_arrays(grid: YeeGrid2D, eps_r: dict) -> dict:
+    """cb_ex, cb_ey = Δt / (ε0 εr) at the Ex and Ey nodes, float64
+    (etalon/fdtd/state.py:80-81)."""
+    return {"cb_ex": grid.dt / (EPSILON_0 * eps_r["Ex"]),
+            "cb_ey": grid.dt / (EPSILON_0 * eps_r["Ey"])}
+
+
 def te_coefficients(grid: YeeGrid2D, eps_r: dict, cpml_x: CPMLAxis, cpml_y: CPMLAxis) -> dict:
     """The float64 coefficient arrays and scalars of a TE state, before the cast."""
     return {
-        "cb_ex": grid.dt / (EPSILON_0 * eps_r["Ex"]),
-        "cb_ey": grid.dt / (EPSILON_0 * eps_r["Ey"]),
+        **cb_arrays(grid, eps_r),
         "bx_e": cpml_x.b_e, "cx_e": cpml_x.c_e, "ikx_e": cpml_x.inv_kappa_e,
         "bx_h": cpml_x.b_h, "cx_h": cpml_x.c_h, "ikx_h": cpml_x.inv_kappa_h,
         "by_e": cpml_y.b_e, "cy_e": cpml_y.c_e, "iky_e": cpml_y.inv_kappa_e,

@@ -141,7 +141,8 @@ typedef struct EtlTune {
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
                         * band 8; one wave per workgroup); 3: one kernel per tile kind on
-                        * three streams; 0: 1 where the shape allows it, else 3 */
+                        * three streams; 0: 1 for float where the shape allows it,
+                        * else 3 */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
@@ -213,11 +214,14 @@ int etl_plan_advance_checkpointed(etl_plan_t plan, int64_t n, int64_t count, int
 /* Event timing of `count` steps from step n (launched directly, not from the graph). */
 typedef struct EtlTiming {
     double launch_ms;        /* average fused launch, fork to join (every tile kind) */
-    double interior_ms;      /* average duration of the interior-tile kernel alone */
+    double kernel_ms;        /* average duration of the kernel that advances the interior
+                              * tiles: with launch 1 the one kernel of every tile kind,
+                              * with launch 3 the interior-tile kernel alone */
     double total_ms;         /* the whole call */
-    int64_t interior_cells;  /* Hz cells that kernel advances per step (all simulations) */
+    int64_t kernel_cells;    /* Hz cells that kernel advances per step (all simulations) */
     int launches;            /* fused launches timed */
     int depth;               /* steps per launch (the plan's temporal depth) */
+    int launch;              /* the plan's launch mode (1 or 3) */
 } EtlTiming;
 int etl_plan_time(etl_plan_t plan, int64_t n, int64_t count, EtlTiming* out, int* parity);
 
@@ -279,6 +283,10 @@ int etl_adjplan_run_rows(etl_adjplan_t plan, int64_t n_first, int count, int his
 /* Issue this plan's work on lane 0 or 1 (default 0). */
 int etl_adjplan_set_lane(etl_adjplan_t plan, int lane);
 int etl_adjplan_lambda(etl_adjplan_t plan, int sim, int field, void** ptr);
+/* Zero lambda (both buffers of every simulation) for the next gradient; the layout,
+ * graphs and sources stay.  etl_adjplan_sources with the same nodes and rows as before
+ * only replaces the table. */
+int etl_adjplan_reset(etl_adjplan_t plan);
 
 /* A forward plan and an adjoint plan advanced together: forward steps [n, n +
  * count_f) (probe rows from probe_row0) and reverse steps n_first + count_a - 1 ..

@@ -181,3 +181,21 @@ def test_torch_autograd_function(hip):
     J2, g = prob.run(np.asarray(design.permittivity))
     assert float(J.detach()) == pytest.approx(J2, rel=1e-12)
     np.testing.assert_allclose(eps.grad.numpy(), g, rtol=1e-12, atol=0)
+
+
+def test_design_loop_reuses_the_session_bitwise(hip):
+    """Gradients of a design loop on one AdjointProblem (the session's solver, plans,
+    graphs and buffers kept; only cb replaced) equal fresh problems' bit for bit, for
+    a new permittivity and back to the first."""
+    sim, design = design_problem("single", seed=12, steps=301)
+    rng = np.random.default_rng(3)
+    eps1 = np.asarray(design.permittivity)
+    eps2 = rng.uniform(2.085, 12.11, size=eps1.shape)
+    loop = AdjointProblem(sim, ["out"], checkpoint_every=64)
+    for eps in (eps1, eps2, eps1):
+        J, g = loop.run(eps)
+        J0, g0 = AdjointProblem(sim, ["out"], checkpoint_every=64, reuse=False).run(eps)
+        assert J == J0
+        assert np.array_equal(g, g0)
+    assert loop._session is not None and loop._session.rev is not None
+    loop.close()
