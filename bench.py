@@ -277,6 +277,9 @@ def main_batched(args) -> int:
                        "median": float(f"{np.median(errors):.3g}"),
                        "per_monitor": [float(f"{e:.3g}") for e in errors]},
         "tune": r["tune"],
+        "autotune": [{**{k: t[k] for k in ("vec", "temporal", "band_rows", "waves", "launch")},
+                      "gcells": round(t["gcells"], 1)}
+                     for t in (r["tune_report"] or {}).get("timings", [])],
         "setup_s": round(r["setup_s"], 2),
     }
     print(json.dumps(line), flush=True)

@@ -10,25 +10,25 @@ int launch_depth(const AdjTileArgs<R>* args, const TileList& tl, int waves, cons
                  hipStream_t s_extra) {
     auto grid = [](int n, int w) { return dim3(unsigned(8 * (((n + w - 1) / w + 7) / 8))); };
     if (tl.n_interior) {
-        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, 8, false, false>), grid(tl.n_interior, waves),
+        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, 8, 0, false>), grid(tl.n_interior, waves),
                            dim3(64 * waves), 0, s_int, args, tl.interior, tl.n_interior, step, rtop,
                            parity);
         ETL_LAUNCHED();
     }
     if (tl.n_strip) {
-        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, kSubBand, true, false>),
+        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, kSubBand, 3, false>),
                            grid(tl.n_strip, waves), dim3(64 * waves), 0, s_strip, args, tl.strip,
                            tl.n_strip, step, rtop, parity);
         ETL_LAUNCHED();
     }
     if (tl.n_extra_strip) {  // after the strip tiles, on their stream
-        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, kSubBand, true, true>),
+        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, kSubBand, 3, true>),
                            grid(tl.n_extra_strip, 1), dim3(64), 0, s_strip, args, tl.extra_strip,
                            tl.n_extra_strip, step, rtop, parity);
         ETL_LAUNCHED();
     }
     if (tl.n_extra) {
-        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, kSubBand, false, true>), grid(tl.n_extra, 1),
+        hipLaunchKernelGGL((adj_tile_kernel<R, 1, T, kSubBand, 0, true>), grid(tl.n_extra, 1),
                            dim3(64), 0, s_extra, args, tl.extra, tl.n_extra, step, rtop, parity);
         ETL_LAUNCHED();
     }

@@ -52,6 +52,7 @@ struct EtlAdjPlan {
     void* args_dev = nullptr;
     int4* tiles_dev = nullptr;
     int n_interior = 0, n_strip = 0, n_extra = 0, n_extra_strip = 0;
+    int n_strip_y = 0, n_strip_x = 0;  // the strip tiles: y only, then x only, then both
     int* step = nullptr;
     hipStream_t s_strip = nullptr, s_extra = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
@@ -103,7 +104,7 @@ int build_adj_layout(EtlAdjPlan* p) {
     free_adj_layout(p);
     free_adj_graph(p);
     const int T = p->tune.temporal, wo = p->wo(), W = 64;
-    std::vector<int4> interior, strip, extra, extra_strip;
+    std::vector<int4> interior, strip_y, strip_x, strip, extra, extra_strip;
     for (int si = 0; si < int(p->sims.size()); ++si) {
         auto& sim = p->sims[si];
         const int nx = sim.nx, ny = sim.ny;
@@ -138,11 +139,12 @@ int build_adj_layout(EtlAdjPlan* p) {
         std::vector<int4> sim_extra;
         auto sub_touches = [&](int r0, int r1, int tj) {
             const int cs = tj * wo - T;
-            return (r0 - T < x_lo) || (r1 + T > x_hi) || (cs < y_lo) || (cs + W > y_hi);
+            return (r0 - T < std::max(x_lo, 1)) || (r1 + T > x_hi) || (cs < y_lo) ||
+                   (cs + W > y_hi);
         };
         for (int tb = 0; tb < nb; ++tb) {
             const int i0 = tb * kAdjBand, i1 = std::min(i0 + kAdjBand, nx);
-            const bool rows_touch = (i0 - T < x_lo) || (i1 + T > x_hi);
+            const bool rows_touch = (i0 - T < std::max(x_lo, 1)) || (i1 + T > x_hi);
             for (int tj = 0; tj < ntj; ++tj) {
                 const int cs = tj * wo - T;
                 const bool cols_touch = (cs < y_lo) || (cs + W > y_hi);
@@ -156,8 +158,11 @@ int build_adj_layout(EtlAdjPlan* p) {
                         const int4 d = make_int4(r0, tj, int(sim_extra.size()), si);
                         sim_extra.push_back(d);
                         (sub_touches(r0, r1, tj) ? extra_strip : extra).push_back(d);
-                    } else {
-                        strip.push_back(make_int4(r0, tj, 0, si));
+                    } else {  // by the strips the sub-tile meets; corners go to `strip`
+                        const int kind =
+                            strip_kind(r0, r1, tj * wo - T, T, W, x_lo, x_hi, y_lo, y_hi);
+                        (kind == 1 ? strip_y : kind == 2 ? strip_x : strip)
+                            .push_back(make_int4(r0, tj, 0, si));
                     }
                 }
             }
@@ -196,6 +201,8 @@ int build_adj_layout(EtlAdjPlan* p) {
         }
     }
     std::vector<int4> tiles(interior);
+    tiles.insert(tiles.end(), strip_y.begin(), strip_y.end());
+    tiles.insert(tiles.end(), strip_x.begin(), strip_x.end());
     tiles.insert(tiles.end(), strip.begin(), strip.end());
     tiles.insert(tiles.end(), extra.begin(), extra.end());
     tiles.insert(tiles.end(), extra_strip.begin(), extra_strip.end());
@@ -204,7 +211,9 @@ int build_adj_layout(EtlAdjPlan* p) {
         (rc = upload_bytes((void**)&p->tiles_dev, tiles.data(), tiles.size() * sizeof(int4))))
         return rc;
     p->n_interior = int(interior.size());
-    p->n_strip = int(strip.size());
+    p->n_strip_y = int(strip_y.size());
+    p->n_strip_x = int(strip_x.size());
+    p->n_strip = int(strip_y.size() + strip_x.size() + strip.size());
     p->n_extra = int(extra.size());
     p->n_extra_strip = int(extra_strip.size());
     p->dirty = false;
@@ -271,9 +280,8 @@ template <typename R>
 int enqueue_adj(EtlAdjPlan* p, int count, int parity, int* launches) {
     const AdjTileArgs<R>* args = static_cast<const AdjTileArgs<R>*>(p->args_dev);
     const int4* t = p->tiles_dev;
-    const TileList tl{t, p->n_interior, t + p->n_interior, p->n_strip,
-                      t + p->n_interior + p->n_strip, p->n_extra,
-                      t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
+    const TileList tl = make_tile_list(t, p->n_interior, p->n_strip, p->n_strip_y,
+                                       p->n_strip_x, p->n_extra, p->n_extra_strip);
     hipStream_t s = lane_stream(p->lane);
     const int T = p->tune.temporal;
     int rtop = count - 1, n = 0;
@@ -404,9 +412,8 @@ void adj_info(const EtlAdjPlan* p, AdjInfo* out) {
     out->launch = p->tune.launch;
     out->lane = p->lane;
     out->args = p->args_dev;
-    out->tl = TileList{t, p->n_interior, t + p->n_interior, p->n_strip,
-                       t + p->n_interior + p->n_strip, p->n_extra,
-                       t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
+    out->tl = make_tile_list(t, p->n_interior, p->n_strip, p->n_strip_y, p->n_strip_x, p->n_extra,
+                                  p->n_extra_strip);
     out->step = p->step;
     out->parity = p->parity;
     out->gen = p->layout_gen;

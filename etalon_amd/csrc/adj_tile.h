@@ -81,13 +81,16 @@ constexpr int kRunRegs = 2;
 
 // The reverse-step tile program of one wave: tile li = ((b & 7) * per_xcd + (b >> 3)) *
 // waves + wave of `tiles`, for workgroup b of n_groups (XCD-major, as the forward tiles).
-template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
+template <typename R, int V, int T, int BR, int S, bool EXTRA>
 __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__ args,
                                               const int4* __restrict__ tiles, int n_tiles, int b,
                                               int n_groups, int waves, int wave,
                                               const int* __restrict__ step, int rtop,
                                               int parity) {
     constexpr int RW = BR + 2 * T;
+    // S as in tile_body: bit 0 the y strips (lambda psi_hz_y, psi_ex_y), bit 1 the x
+    // strips (lambda psi_hz_x, psi_ey_x)
+    constexpr bool SY = (S & 1) != 0, SX = (S & 2) != 0;
     const int per_xcd = (n_groups + 7) >> 3;
     const int li = ((b & 7) * per_xcd + (b >> 3)) * waves + wave;
     if (li >= n_tiles) return;
@@ -123,10 +126,11 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
     const G* __restrict__ cby = to_global(a.cb_ey);
 
     // ---- the window: lambda_Ex, lambda_Hz, cb_ex rows 0..RW-1; lambda_Ey, cb_ey 0..RW
-    constexpr int PR = STRIP ? RW : 1;
-    constexpr int PE = STRIP ? RW + 1 : 1;
+    constexpr int PY = SY ? RW : 1;
+    constexpr int PX = SX ? RW : 1;
+    constexpr int PE = SX ? RW + 1 : 1;
     R ex[RW][V], hz[RW][V], ey[RW + 1][V], cx[RW][V], cy[RW + 1][V];
-    R phy[PR][V], phx[PR][V], pey[PR][V], pex[PE][V];
+    R phy[PY][V], phx[PX][V], pey[PY][V], pex[PE][V];
 #pragma unroll
     for (int l = 0; l <= RW; ++l) {
         const int g = g0 + l;
@@ -136,15 +140,15 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
             ldv<R, V>(ex[l], exA, gx, px, c, px);
             ldv<R, V>(hz[l], hzA, gx, ny, c, ny);
             ldv<R, V>(cx[l], cbx, gx, px, c, px);
-            if constexpr (STRIP) {
-                ldv<R, V>(phy[l], phyA, gx, ny, c, ny);
-                ldv<R, V>(phx[l], phxA, gx, ny, c, ny);
-                ldv<R, V>(pey[l], pexA, gx, px, c, px);
+            if constexpr (SY) {
+                ldv<R, V>(phy[SY ? l : 0], phyA, gx, ny, c, ny);
+                ldv<R, V>(pey[SY ? l : 0], pexA, gx, px, c, px);
             }
+            if constexpr (SX) ldv<R, V>(phx[SX ? l : 0], phxA, gx, ny, c, ny);
         }
         ldv<R, V>(ey[l], eyA, ge, ny, c, ny);
         ldv<R, V>(cy[l], cby, ge, ny, c, ny);
-        if constexpr (STRIP) ldv<R, V>(pex[STRIP ? l : 0], peyA, ge, ny, c, ny);
+        if constexpr (SX) ldv<R, V>(pex[SX ? l : 0], peyA, ge, ny, c, ny);
     }
 
     const R db = a.db, idx = a.inv_dx, idy = a.inv_dy;
@@ -154,8 +158,8 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
     for (int k = 0; k < V; ++k) {
         const int j = c + k;
         mEx[k] = j >= 1 && j <= ny - 1;  // Ex nodes the forward updates
-        ysh[k] = STRIP && j >= 0 && j < ny && (j < yh_lo || j >= yh_hi);
-        yse[k] = STRIP && mEx[k] && (j < ye_lo || j >= ye_hi);
+        ysh[k] = SY && j >= 0 && j < ny && (j < yh_lo || j >= yh_hi);
+        yse[k] = SY && mEx[k] && (j < ye_lo || j >= ye_hi);
         byh[k] = ysh[k] ? to_global(a.by_h)[j] : R(1);
         cyh[k] = ysh[k] ? to_global(a.cy_h)[j] : R(0);
         ikyh[k] = ysh[k] ? to_global(a.iky_h)[j] : R(1);
@@ -165,7 +169,7 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
     }
     // x-strip row profiles, one window row per lane (RW + 1 <= 64)
     R pbxh = R(1), pcxh = R(0), pikxh = R(1), pbxe = R(1), pcxe = R(0), pikxe = R(1);
-    if constexpr (STRIP) {
+    if constexpr (SX) {
         const int gl = g0 + min(lane, RW);
         const int gh = min(max(gl, 0), nx - 1), ge = min(max(gl, 0), nx);
         pbxh = to_global(a.bx_h)[gh];
@@ -264,7 +268,7 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
             const bool row_ok = g >= 1 && g <= nx - 1;
             bool xe = false;
             R ikx = R(1), cxv = R(0);
-            if constexpr (STRIP) {
+            if constexpr (SX) {
                 xe = row_ok && (g < xe_lo || g >= xe_hi);
                 if (xe) {
                     ikx = lane_value(pikxe, l);
@@ -275,7 +279,7 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
             for (int k = 0; k < V; ++k) {
                 const R lw = -cy[l][k] * ey[l][k];
                 R v = lw;
-                if (STRIP && xe) v = ikx * lw + cxv * (pex[STRIP ? l : 0][k] + lw);
+                if (SX && xe) v = ikx * lw + cxv * (pex[SX ? l : 0][k] + lw);
                 out[k] = row_ok ? v : R(0);
             }
         };
@@ -288,7 +292,7 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
             for (int k = 0; k < V; ++k) {
                 const R lu = cx[l][k] * ex[l][k];
                 R v = lu;
-                if (STRIP && yse[k]) v = ikye[k] * lu + cye[k] * (pey[STRIP ? l : 0][k] + lu);
+                if (SY && yse[k]) v = ikye[k] * lu + cye[k] * (pey[SY ? l : 0][k] + lu);
                 la[k] = mEx[k] ? v : R(0);
             }
             lb_row(l + 1, lb_next);
@@ -296,7 +300,7 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
             const int g = g0 + l;
             bool xh = false;
             R bx = R(1), cxh = R(0), ikx = R(1);
-            if constexpr (STRIP) {
+            if constexpr (SX) {
                 xh = g < xh_lo || g >= xh_hi;
                 if (xh) {
                     bx = lane_value(pbxh, l);
@@ -309,17 +313,17 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
                 const R la_n = (k + 1 < V) ? la[k + 1] : la_r;
                 const R lh = hz[l][k] + idy * (la[k] - la_n) + idx * (lb_cur[k] - lb_next[k]);
                 hz[l][k] = lh;
-                if (STRIP && ysh[k]) {
-                    const R lpy = phy[STRIP ? l : 0][k] + db * lh;
+                if (SY && ysh[k]) {
+                    const R lpy = phy[SY ? l : 0][k] + db * lh;
                     ld[l][k] = db * ikyh[k] * lh + cyh[k] * lpy;
-                    phy[STRIP ? l : 0][k] = byh[k] * lpy;
+                    phy[SY ? l : 0][k] = byh[k] * lpy;
                 } else {
                     ld[l][k] = db * lh;
                 }
-                if (STRIP && xh) {
-                    const R lpx = phx[STRIP ? l : 0][k] - db * lh;
+                if (SX && xh) {
+                    const R lpx = phx[SX ? l : 0][k] - db * lh;
                     le[l][k] = -db * ikx * lh + cxh * lpx;
-                    phx[STRIP ? l : 0][k] = bx * lpx;
+                    phx[SX ? l : 0][k] = bx * lpx;
                 } else {
                     le[l][k] = -db * lh;
                 }
@@ -332,9 +336,9 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
             const int g = g0 + l;
             const R ld_l = from_prev_lane(ld[l][V - 1]);
             const bool ey_row = l >= t + 1;
-            const bool xe = STRIP && ey_row && g >= 1 && g <= nx - 1 && (g < xe_lo || g >= xe_hi);
+            const bool xe = SX && ey_row && g >= 1 && g <= nx - 1 && (g < xe_lo || g >= xe_hi);
             R bxe = R(1);
-            if (STRIP && xe) bxe = lane_value(pbxe, l);
+            if (SX && xe) bxe = lane_value(pbxe, l);
             const bool owned = EXTRA && l >= T && l < T + BR;
             // history of this row (m) and the one above (m - 1), and column j - 1
             const int m = owned ? l - T + 1 : 1;
@@ -344,7 +348,7 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
             for (int k = 0; k < V; ++k) {
                 const int j = c + k;
                 const R lex = ex[l][k];
-                if (STRIP && yse[k]) pey[STRIP ? l : 0][k] = bye[k] * (pey[STRIP ? l : 0][k] + cx[l][k] * lex);
+                if (SY && yse[k]) pey[SY ? l : 0][k] = bye[k] * (pey[SY ? l : 0][k] + cx[l][k] * lex);
                 if (owned && boxed && mEx[k] && g >= a.bi0 && g < a.bi0 + a.bni && j >= a.bj0 &&
                     j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
                     const R h0 = (k > 0) ? hv[EXTRA ? t : 0][EXTRA ? m : 0][k > 0 ? k - 1 : 0]
@@ -357,7 +361,7 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
                 ex[l][k] = lex + idy * (ldp - ld[l][k]);
                 if (ey_row) {
                     const R ley = ey[l][k];
-                    if (STRIP && xe) pex[STRIP ? l : 0][k] = bxe * (pex[STRIP ? l : 0][k] - cy[l][k] * ley);
+                    if (SX && xe) pex[SX ? l : 0][k] = bxe * (pex[SX ? l : 0][k] - cy[l][k] * ley);
                     if (owned && boxed && g >= a.bi0 && g < a.bi0 + a.bni && g >= 1 && g <= nx - 1 &&
                         j >= a.bj0 && j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
                         const double w = (double(hv[EXTRA ? t : 0][EXTRA ? m : 0][k]) -
@@ -380,21 +384,25 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
         store_row<R, V>(hzB + int64_t(g) * ny, hz[l], c, lo, min(hi, ny));
         store_row<R, V>(exB + int64_t(g) * px, ex[l], c, lo, min(hi, px));
         store_row<R, V>(eyB + int64_t(g) * ny, ey[l], c, lo, min(hi, ny));
-        if constexpr (STRIP) {
-            bool mh[V], moy[V], moe[V];
+        if constexpr (SY) {
+            bool moy[V], moe[V];
 #pragma unroll
             for (int k = 0; k < V; ++k) {
                 const bool own = (c + k >= lo && c + k < hi);
-                mh[k] = own && c + k < ny;
                 moy[k] = own && ysh[k];
                 moe[k] = own && yse[k];
             }
-            store_masked<R, V>(phyB + int64_t(g) * ny, phy[STRIP ? l : 0], c, moy);
-            store_masked<R, V>(pexB + int64_t(g) * px, pey[STRIP ? l : 0], c, moe);
+            store_masked<R, V>(phyB + int64_t(g) * ny, phy[SY ? l : 0], c, moy);
+            store_masked<R, V>(pexB + int64_t(g) * px, pey[SY ? l : 0], c, moe);
+        }
+        if constexpr (SX) {
+            bool mh[V];
+#pragma unroll
+            for (int k = 0; k < V; ++k) mh[k] = (c + k >= lo && c + k < hi) && c + k < ny;
             if (g < xh_lo || g >= xh_hi)
-                store_masked<R, V>(phxB + int64_t(g) * ny, phx[STRIP ? l : 0], c, mh);
+                store_masked<R, V>(phxB + int64_t(g) * ny, phx[SX ? l : 0], c, mh);
             if (g >= 1 && (g < xe_lo || g >= xe_hi))
-                store_masked<R, V>(peyB + int64_t(g) * ny, pex[STRIP ? l : 0], c, mh);
+                store_masked<R, V>(peyB + int64_t(g) * ny, pex[SX ? l : 0], c, mh);
         }
         if constexpr (EXTRA) {
             if (boxed) {
@@ -413,14 +421,14 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
     }
 }
 
-template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
+template <typename R, int V, int T, int BR, int S, bool EXTRA>
 __global__ __launch_bounds__(256) void adj_tile_kernel(const AdjTileArgs<R>* __restrict__ args,
                                                        const int4* __restrict__ tiles, int n_tiles,
                                                        const int* __restrict__ step, int rtop,
                                                        int parity) {
     const int waves = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    adj_tile_body<R, V, T, BR, STRIP, EXTRA>(args, tiles, n_tiles, blockIdx.x,
+    adj_tile_body<R, V, T, BR, S, EXTRA>(args, tiles, n_tiles, blockIdx.x,
                                              (n_tiles + waves - 1) / waves, waves, wave, step,
                                              rtop, parity);
 }

@@ -71,13 +71,42 @@ constexpr int kSubBand = 4;
 struct TileList {
     const int4* interior;
     int n_interior;
-    const int4* strip;
+    const int4* strip;  // the y-strip tiles, then the x-strip tiles, then the corners
     int n_strip;
     const int4* extra;  // extra tiles off the CPML strips (no psi)
     int n_extra;
     const int4* extra_strip;  // extra tiles that meet a strip (rare)
     int n_extra_strip;
+    int n_strip_y;  // strip tiles meeting only y strips (columns)
+    int n_strip_x;  // then those meeting only x strips (rows)
 };
+
+// The tile list of descriptors stored in the order interior, strip (y, x, corner),
+// extra, extra-strip.
+inline TileList make_tile_list(const int4* t, int n_interior, int n_strip, int n_strip_y,
+                               int n_strip_x, int n_extra, int n_extra_strip) {
+    return TileList{t,
+                    n_interior,
+                    t + n_interior,
+                    n_strip,
+                    t + n_interior + n_strip,
+                    n_extra,
+                    t + n_interior + n_strip + n_extra,
+                    n_extra_strip,
+                    n_strip_y,
+                    n_strip_x};
+}
+
+// Which strips a sub-tile's window (rows [r0 - T, r1 + T), columns [cs, cs + W))
+// meets: bit 0 the y strips (columns outside [y_lo, y_hi)), bit 1 the x strips (rows
+// outside [x_lo, x_hi), and the first row, where Ey stays fixed).  A strip sub-tile
+// that meets neither (the sub-tiles of a band that does) is run as an x-strip tile.
+inline int strip_kind(int r0, int r1, int cs, int T, int W, int x_lo, int x_hi, int y_lo,
+                      int y_hi) {
+    const bool rows = (r0 - T < (x_lo > 1 ? x_lo : 1)) || (r1 + T > x_hi);
+    const bool cols = (cs < y_lo) || (cs + W > y_hi);
+    return (cols ? 1 : 0) | (rows || !cols ? 2 : 0);
+}
 
 // Register-tile shapes compiled into the library: (precision, vec, depth, band rows).
 // Every shape of depth T > 1 comes with its depth-1 sibling for step-count tails.

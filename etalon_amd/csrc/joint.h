@@ -13,25 +13,29 @@
 
 namespace etl {
 
-// Tiles of a forward step block: kinds 0 interior (8 rows), 1 strip, 2 extra,
-// 3 extra-strip (kSubBand rows); `counter`, `kstep0`, `parity` as tile_kernel takes.
+// Tiles of a step block by kind: 0 interior (8 rows), then kSubBand-row sub-tiles:
+// 1 y-strip, 2 x-strip, 3 corner strip, 4 extra, 5 extra-strip.
+constexpr int kJointKinds = 6;
+
+// A forward step block; `counter`, `kstep0`, `parity` as tile_kernel takes.
 template <typename R>
 struct FwdSet {
     const FusedArgs<R>* args;
-    const int4* tiles[4];
-    int n[4];
+    const int4* tiles[kJointKinds];
+    int n[kJointKinds];
     const int* counter;
     int kstep0;
     int parity;
+    int band;  // rows of an interior tile (8, or 16 for float forward-only launches)
 };
 
-// Tiles of an adjoint step block, the same kinds; `step`, `rtop`, `parity` as
-// adj_tile_kernel takes.
+// An adjoint step block, the same kinds; `step`, `rtop`, `parity` as adj_tile_kernel
+// takes.
 template <typename R>
 struct AdjSet {
     const AdjTileArgs<R>* args;
-    const int4* tiles[4];
-    int n[4];
+    const int4* tiles[kJointKinds];
+    int n[kJointKinds];
     const int* step;
     int rtop;
     int parity;
@@ -41,26 +45,46 @@ struct AdjSet {
 // XCD 0 and the XCD-major order of tile_body holds within it.
 __host__ __device__ inline int joint_groups(int n) { return 8 * ((n + 7) / 8); }
 
+// The kinds of a tile list: pointers and counts.
+inline void split_kinds(const TileList& tl, const int4* (&t)[kJointKinds], int (&n)[kJointKinds]) {
+    const int ny_ = tl.n_strip_y, nx_ = tl.n_strip_x, nxy = tl.n_strip - ny_ - nx_;
+    t[0] = tl.interior;
+    n[0] = tl.n_interior;
+    t[1] = tl.strip;
+    n[1] = ny_;
+    t[2] = tl.strip + ny_;
+    n[2] = nx_;
+    t[3] = tl.strip + ny_ + nx_;
+    n[3] = nxy;
+    t[4] = tl.extra;
+    n[4] = tl.n_extra;
+    t[5] = tl.extra_strip;
+    n[5] = tl.n_extra_strip;
+}
+
 template <typename R>
 inline FwdSet<R> fwd_set(const FusedArgs<R>* args, const TileList& tl, const int* counter,
-                         int kstep0, int parity) {
-    return FwdSet<R>{args,
-                     {tl.interior, tl.strip, tl.extra, tl.extra_strip},
-                     {tl.n_interior, tl.n_strip, tl.n_extra, tl.n_extra_strip},
-                     counter,
-                     kstep0,
-                     parity};
+                         int kstep0, int parity, int band = 8) {
+    FwdSet<R> f{};
+    f.args = args;
+    split_kinds(tl, f.tiles, f.n);
+    f.counter = counter;
+    f.kstep0 = kstep0;
+    f.parity = parity;
+    f.band = band;
+    return f;
 }
 
 template <typename R>
 inline AdjSet<R> adj_set(const AdjTileArgs<R>* args, const TileList& tl, const int* step,
                          int rtop, int parity) {
-    return AdjSet<R>{args,
-                     {tl.interior, tl.strip, tl.extra, tl.extra_strip},
-                     {tl.n_interior, tl.n_strip, tl.n_extra, tl.n_extra_strip},
-                     step,
-                     rtop,
-                     parity};
+    AdjSet<R> a{};
+    a.args = args;
+    split_kinds(tl, a.tiles, a.n);
+    a.step = step;
+    a.rtop = rtop;
+    a.parity = parity;
+    return a;
 }
 
 // Depths launch_joint accepts for (forward, adjoint): (T, 0), (0, T), (T, T), T in 1, 2, 4.
@@ -102,26 +126,26 @@ __host__ __device__ constexpr bool part_has(int part, int kind) {
 
 // One kind K of a forward (adjoint) step block: when workgroup b falls in its range
 // it runs its tile and returns true, else b moves past the range.
-template <typename R, int T, int K, int BR, bool STRIP, bool EXTRA>
+template <typename R, int T, int K, int BR, int S, bool EXTRA>
 __device__ __forceinline__ bool fwd_kind(const FwdSet<R>& f, int& b, R* lds) {
     const int g = joint_groups(f.n[K]);
     if (b >= g) {
         b -= g;
         return false;
     }
-    tile_body<R, 1, T, BR, STRIP, EXTRA>(f.args, f.tiles[K], f.n[K], b, f.n[K], 1, 0, f.counter,
+    tile_body<R, 1, T, BR, S, EXTRA>(f.args, f.tiles[K], f.n[K], b, f.n[K], 1, 0, f.counter,
                                          f.kstep0, f.parity, lds);
     return true;
 }
 
-template <typename R, int T, int K, int BR, bool STRIP, bool EXTRA>
+template <typename R, int T, int K, int BR, int S, bool EXTRA>
 __device__ __forceinline__ bool adj_kind(const AdjSet<R>& a, int& b) {
     const int g = joint_groups(a.n[K]);
     if (b >= g) {
         b -= g;
         return false;
     }
-    adj_tile_body<R, 1, T, BR, STRIP, EXTRA>(a.args, a.tiles[K], a.n[K], b, a.n[K], 1, 0, a.step,
+    adj_tile_body<R, 1, T, BR, S, EXTRA>(a.args, a.tiles[K], a.n[K], b, a.n[K], 1, 0, a.step,
                                              a.rtop, a.parity);
     return true;
 }
@@ -130,40 +154,44 @@ __device__ __forceinline__ bool adj_kind(const AdjSet<R>& a, int& b) {
 // extra-strip tiles, the adjoint plan's, then the forward and the adjoint interior
 // tiles.  The latency-bound CPML and source/monitor tiles come first, so they are
 // dispatched before the bulk of interior tiles fills the GPU and finish under it.
-template <typename R, int TF, int TA, int PART>
+template <typename R, int TF, int TA, int PART, int JB>
 __device__ __forceinline__ void joint_body(const FwdSet<R>& f, const AdjSet<R>& a) {
-    __shared__ R lds[part_has(PART, 2) ? kSubBand * 64 : 1];
+    __shared__ R lds[part_has(PART, 4) ? kSubBand * 64 : 1];  // the extra tiles' staging
     int b = blockIdx.x;
 #ifndef ETL_JOINT_EDGE_FIRST
 #define ETL_JOINT_EDGE_FIRST 1
 #endif
     if constexpr (!ETL_JOINT_EDGE_FIRST && part_has(PART, 0)) {
         if constexpr (TF > 0) {
-            if (fwd_kind<R, TF, 0, 8, false, false>(f, b, lds)) return;
+            if (fwd_kind<R, TF, 0, JB, 0, false>(f, b, lds)) return;
         }
         if constexpr (TA > 0) {
-            if (adj_kind<R, TA, 0, 8, false, false>(a, b)) return;
+            if (adj_kind<R, TA, 0, 8, 0, false>(a, b)) return;
         }
     }
     if constexpr (part_has(PART, 1)) {
         if constexpr (TF > 0) {
-            if (fwd_kind<R, TF, 1, kSubBand, true, false>(f, b, lds) ||
-                fwd_kind<R, TF, 2, kSubBand, false, true>(f, b, lds) ||
-                fwd_kind<R, TF, 3, kSubBand, true, true>(f, b, lds))
+            if (fwd_kind<R, TF, 1, kSubBand, 1, false>(f, b, lds) ||
+                fwd_kind<R, TF, 2, kSubBand, 2, false>(f, b, lds) ||
+                fwd_kind<R, TF, 3, kSubBand, 3, false>(f, b, lds) ||
+                fwd_kind<R, TF, 4, kSubBand, 0, true>(f, b, lds) ||
+                fwd_kind<R, TF, 5, kSubBand, 3, true>(f, b, lds))
                 return;
         }
         if constexpr (TA > 0) {
-            if (adj_kind<R, TA, 1, kSubBand, true, false>(a, b) ||
-                adj_kind<R, TA, 2, kSubBand, false, true>(a, b) ||
-                adj_kind<R, TA, 3, kSubBand, true, true>(a, b))
+            if (adj_kind<R, TA, 1, kSubBand, 1, false>(a, b) ||
+                adj_kind<R, TA, 2, kSubBand, 2, false>(a, b) ||
+                adj_kind<R, TA, 3, kSubBand, 3, false>(a, b) ||
+                adj_kind<R, TA, 4, kSubBand, 0, true>(a, b) ||
+                adj_kind<R, TA, 5, kSubBand, 3, true>(a, b))
                 return;
         }
     }
     if constexpr (ETL_JOINT_EDGE_FIRST && part_has(PART, 0)) {
         if constexpr (TF > 0) {
-            if (fwd_kind<R, TF, 0, 8, false, false>(f, b, lds)) return;
+            if (fwd_kind<R, TF, 0, JB, 0, false>(f, b, lds)) return;
         }
-        if constexpr (TA > 0) adj_kind<R, TA, 0, 8, false, false>(a, b);
+        if constexpr (TA > 0) adj_kind<R, TA, 0, 8, 0, false>(a, b);
     }
 }
 
@@ -177,37 +205,37 @@ __device__ __forceinline__ void joint_body(const FwdSet<R>& f, const AdjSet<R>& 
 #endif
 constexpr int kJointWaves = ETL_JOINT_WAVES;
 
-template <typename R, int TF, int TA, int PART>
+template <typename R, int TF, int TA, int PART, int JB>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? kJointWaves : 1)))
 void joint_kernel(FwdSet<R> f, AdjSet<R> a) {
-    joint_body<R, TF, TA, PART>(f, a);
+    joint_body<R, TF, TA, PART, JB>(f, a);
 }
 
-template <typename R, int TF, int TA, int PART>
+template <typename R, int TF, int TA, int PART, int JB = 8>
 int launch_joint_t(const FwdSet<R>* f, const AdjSet<R>* a, hipStream_t s) {
     int groups = 0;
     FwdSet<R> fz{};
     AdjSet<R> az{};
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kJointKinds; ++k) {
         if (!part_has(PART, k)) continue;
         if (TF > 0) groups += joint_groups(f->n[k]);
         if (TA > 0) groups += joint_groups(a->n[k]);
     }
     if (groups == 0) return ETL_OK;
-    hipLaunchKernelGGL((joint_kernel<R, TF, TA, PART>), dim3(unsigned(groups)), dim3(64), 0, s,
-                       TF > 0 ? *f : fz, TA > 0 ? *a : az);
+    hipLaunchKernelGGL((joint_kernel<R, TF, TA, PART, JB>), dim3(unsigned(groups)), dim3(64), 0,
+                       s, TF > 0 ? *f : fz, TA > 0 ? *a : az);
     ETL_LAUNCHED();
     return ETL_OK;
 }
 
 // launch_joint_t at a run-time part.
-template <typename R, int TF, int TA>
+template <typename R, int TF, int TA, int JB = 8>
 int launch_joint_part(const FwdSet<R>* f, const AdjSet<R>* a, int part, hipStream_t s) {
     switch (part) {
-        case kAllKinds: return launch_joint_t<R, TF, TA, kAllKinds>(f, a, s);
-        case kInteriorOnly: return launch_joint_t<R, TF, TA, kInteriorOnly>(f, a, s);
-        case kEdgeOnly: return launch_joint_t<R, TF, TA, kEdgeOnly>(f, a, s);
+        case kAllKinds: return launch_joint_t<R, TF, TA, kAllKinds, JB>(f, a, s);
+        case kInteriorOnly: return launch_joint_t<R, TF, TA, kInteriorOnly, JB>(f, a, s);
+        case kEdgeOnly: return launch_joint_t<R, TF, TA, kEdgeOnly, JB>(f, a, s);
     }
     set_error("joint step: unknown part %d", part);
     return ETL_EARG;

@@ -5,12 +5,20 @@ namespace etl {
 
 template <typename R>
 int launch_joint_fwd_any(const FwdSet<R>* f, int tf, int part, hipStream_t s) {
-    switch (tf) {
-        case 1: return launch_joint_part<R, 1, 0>(f, nullptr, part, s);
-        case 2: return launch_joint_part<R, 2, 0>(f, nullptr, part, s);
-        case 4: return launch_joint_part<R, 4, 0>(f, nullptr, part, s);
+    if (f->band == 8) {
+        switch (tf) {
+            case 1: return launch_joint_part<R, 1, 0>(f, nullptr, part, s);
+            case 2: return launch_joint_part<R, 2, 0>(f, nullptr, part, s);
+            case 4: return launch_joint_part<R, 4, 0>(f, nullptr, part, s);
+        }
+    } else if constexpr (sizeof(R) == 4) {  // 16-row interior tiles: float only
+        if (f->band == 16) switch (tf) {
+                case 1: return launch_joint_part<R, 1, 0, 16>(f, nullptr, part, s);
+                case 2: return launch_joint_part<R, 2, 0, 16>(f, nullptr, part, s);
+                case 4: return launch_joint_part<R, 4, 0, 16>(f, nullptr, part, s);
+            }
     }
-    set_error("joint step: forward depth %d not compiled", tf);
+    set_error("joint step: forward depth %d, band %d not compiled", tf, f->band);
     return ETL_EARG;
 }
 

@@ -149,6 +149,7 @@ struct EtlPlan {
     void* trace_sims_dev = nullptr;  // TraceSim<R>[n_sims]
     int4* tiles_dev = nullptr;       // interior, then strip, then extra tile descriptors
     int n_interior = 0, n_strip = 0, n_extra = 0, n_extra_strip = 0;
+    int n_strip_y = 0, n_strip_x = 0;  // the strip tiles: y only, then x only, then both
     int64_t interior_cells = 0;      // Hz cells the interior tiles own
     int4* pairs = nullptr;           // DFT pairs of every simulation
     int* pair_sim = nullptr;
@@ -227,7 +228,7 @@ int build_layout(EtlPlan* p) {
     const int W = 64 * p->tune.vec;
     const int band = p->tune.band_rows;
 
-    std::vector<int4> interior, strip, extra, extra_strip, pairs;
+    std::vector<int4> interior, strip_y, strip_x, strip, extra, extra_strip, pairs;
     std::vector<int> pair_sim;
     std::vector<void*> accs;
     int64_t interior_cells = 0;
@@ -294,12 +295,14 @@ int build_layout(EtlPlan* p) {
         std::vector<int4> sim_extra;  // this simulation's extra sub-tiles, numbered in order
         auto sub_touches = [&](int r0, int r1, int tj) {
             const int cs = tj * wo - T;
-            return (r0 - T < x_lo) || (r1 + T > x_hi) || (cs < y_lo) || (cs + W > y_hi);
+            return (r0 - T < std::max(x_lo, 1)) || (r1 + T > x_hi) || (cs < y_lo) ||
+                   (cs + W > y_hi);
         };
         for (int tb = 0; tb < nb; ++tb) {
             const int i0 = tb * band;
             const int i1 = std::min(i0 + band, nx);
-            const bool rows_touch = (i0 - T < x_lo) || (i1 + T > x_hi);
+            // the first row counts as a strip row: Ey row 0 stays fixed
+            const bool rows_touch = (i0 - T < std::max(x_lo, 1)) || (i1 + T > x_hi);
             for (int tj = 0; tj < ntj; ++tj) {
                 const int cs = tj * wo - T;
                 const bool cols_touch = (cs < y_lo) || (cs + W > y_hi);
@@ -315,8 +318,11 @@ int build_layout(EtlPlan* p) {
                         const int4 d = make_int4(r0, tj, int(sim_extra.size()), si);
                         sim_extra.push_back(d);
                         (sub_touches(r0, r1, tj) ? extra_strip : extra).push_back(d);
-                    } else {
-                        strip.push_back(make_int4(r0, tj, 0, si));
+                    } else {  // by the strips the sub-tile meets; corners go to `strip`
+                        const int kind =
+                            strip_kind(r0, r1, tj * wo - T, T, W, x_lo, x_hi, y_lo, y_hi);
+                        (kind == 1 ? strip_y : kind == 2 ? strip_x : strip)
+                            .push_back(make_int4(r0, tj, 0, si));
                     }
                 }
             }
@@ -354,6 +360,8 @@ int build_layout(EtlPlan* p) {
             return rc;
     }
     std::vector<int4> tiles(interior);
+    tiles.insert(tiles.end(), strip_y.begin(), strip_y.end());
+    tiles.insert(tiles.end(), strip_x.begin(), strip_x.end());
     tiles.insert(tiles.end(), strip.begin(), strip.end());
     tiles.insert(tiles.end(), extra.begin(), extra.end());
     tiles.insert(tiles.end(), extra_strip.begin(), extra_strip.end());
@@ -362,7 +370,9 @@ int build_layout(EtlPlan* p) {
         (rc = upload((void**)&p->tiles_dev, tiles.data(), tiles.size() * sizeof(int4))))
         return rc;
     p->n_interior = int(interior.size());
-    p->n_strip = int(strip.size());
+    p->n_strip_y = int(strip_y.size());
+    p->n_strip_x = int(strip_x.size());
+    p->n_strip = int(strip_y.size() + strip_x.size() + strip.size());
     p->n_extra = int(extra.size());
     p->n_extra_strip = int(extra_strip.size());
     p->interior_cells = interior_cells;
@@ -480,9 +490,8 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
     const FusedArgs<R>* args = static_cast<const FusedArgs<R>*>(p->args_dev);
     hipStream_t s = lane_stream(p->lane);
     const int4* t = p->tiles_dev;
-    const TileList tl{t, p->n_interior, t + p->n_interior, p->n_strip,
-                      t + p->n_interior + p->n_strip, p->n_extra,
-                      t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
+    const TileList tl = make_tile_list(t, p->n_interior, p->n_strip, p->n_strip_y,
+                                       p->n_strip_x, p->n_extra, p->n_extra_strip);
     const int T = p->tune.temporal;
     int k = 0, n = 0;
     while (k < count) {
@@ -492,7 +501,8 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
                 ETL_HIP(hipEventRecord(events[4 * n], s));
                 ETL_HIP(hipEventRecord(events[4 * n + 1], s));
             }
-            const FwdSet<R> fs = fwd_set<R>(args, tl, p->counter, k, parity ^ (n & 1));
+            const FwdSet<R> fs =
+                fwd_set<R>(args, tl, p->counter, k, parity ^ (n & 1), p->tune.band_rows);
             if (int rc = launch_joint<R>(&fs, depth, nullptr, 0, s)) return rc;
             if (events) {
                 ETL_HIP(hipEventRecord(events[4 * n + 2], s));
@@ -893,9 +903,8 @@ void fwd_info(const EtlPlan* p, FwdInfo* out) {
     out->vec = p->tune.vec;
     out->band = p->tune.band_rows;
     out->args = p->args_dev;
-    out->tl = TileList{t, p->n_interior, t + p->n_interior, p->n_strip,
-                       t + p->n_interior + p->n_strip, p->n_extra,
-                       t + p->n_interior + p->n_strip + p->n_extra, p->n_extra_strip};
+    out->tl = make_tile_list(t, p->n_interior, p->n_strip, p->n_strip_y, p->n_strip_x, p->n_extra,
+                                  p->n_extra_strip);
     out->counter = p->counter;
     out->parity = p->parity;
     out->gen = p->layout_gen;
@@ -941,8 +950,10 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
                 "etl_plan_create: waves per workgroup must be 1, 2 or 4 (got %d)", t.waves);
     ETL_REQUIRE(t.launch == 0 || t.launch == 1 || t.launch == 3,
                 "etl_plan_create: launch must be 0, 1 or 3 (got %d)", t.launch);
-    ETL_REQUIRE(t.launch != 1 || (t.vec == 1 && t.band_rows == 8),
-                "etl_plan_create: one launch per step block needs vec 1 and band 8");
+    ETL_REQUIRE(t.launch != 1 || (t.vec == 1 && (t.band_rows == 8 ||
+                                                 (t.band_rows == 16 && precision == ETL_F32))),
+                "etl_plan_create: one launch per step block needs vec 1 and band 8 (or 16 in "
+                "float)");
 
     EtlPlan* p = new EtlPlan();
     p->precision = precision;

@@ -10,25 +10,25 @@ int launch_shape(const FusedArgs<R>* args, const TileList& tl, int waves, const 
                  hipStream_t s_extra) {
     auto grid = [](int n, int w) { return dim3(unsigned(8 * (((n + w - 1) / w + 7) / 8))); };
     if (tl.n_interior) {
-        hipLaunchKernelGGL((tile_kernel<R, V, T, BR, false, false>), grid(tl.n_interior, waves),
+        hipLaunchKernelGGL((tile_kernel<R, V, T, BR, 0, false>), grid(tl.n_interior, waves),
                            dim3(64 * waves), 0, s_int, args, tl.interior, tl.n_interior, counter,
                            kstep0, parity);
         ETL_LAUNCHED();
     }
     if (tl.n_strip) {
-        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, true, false>), grid(tl.n_strip, waves),
+        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, 3, false>), grid(tl.n_strip, waves),
                            dim3(64 * waves), 0, s_strip, args, tl.strip, tl.n_strip, counter,
                            kstep0, parity);
         ETL_LAUNCHED();
     }
     if (tl.n_extra_strip) {  // after the strip tiles, on their stream
-        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, true, true>), grid(tl.n_extra_strip, 1),
+        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, 3, true>), grid(tl.n_extra_strip, 1),
                            dim3(64), 0, s_strip, args, tl.extra_strip, tl.n_extra_strip, counter,
                            kstep0, parity);
         ETL_LAUNCHED();
     }
     if (tl.n_extra) {
-        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, false, true>), grid(tl.n_extra, 1),
+        hipLaunchKernelGGL((tile_kernel<R, V, T, kSubBand, 0, true>), grid(tl.n_extra, 1),
                            dim3(64), 0, s_extra, args, tl.extra, tl.n_extra, counter, kstep0,
                            parity);
         ETL_LAUNCHED();

@@ -26,10 +26,12 @@
 // lives in VGPRs and the only waits are the one after the load burst.  Thousands of
 // tiles per launch keep many loads in flight per SIMD.
 //
-// Variants.  STRIP tiles touch a CPML strip: they load, carry and store psi where
-// the 1D profiles differ from (b, c, 1/kappa) = (1, 0, 1); elsewhere psi is +0 and the
-// update reduces to d*1 + (+0), exactly the reference's arithmetic (the +0 fixes the
-// sign of a zero difference).  EXTRA tiles also inject sources and trace monitors.
+// Variants.  Strip tiles (S != 0) touch a CPML strip: they load, carry and store the
+// psi arrays of the strips they meet (S bit 0: the y strips, psi_hz_y and psi_ex_y;
+// bit 1: the x strips, psi_hz_x and psi_ey_x) where the 1D profiles differ from
+// (b, c, 1/kappa) = (1, 0, 1); elsewhere psi is +0 and the update reduces to
+// d*1 + (+0), exactly the reference's arithmetic (the +0 fixes the sign of a zero
+// difference).  EXTRA tiles also inject sources and trace monitors.
 #pragma once
 
 #include "common.h"
@@ -192,13 +194,17 @@ __device__ __forceinline__ void trace_tile(const global_t<const int4v>* __restri
 // receives a contiguous range of tiles, whose neighbours share its L2.  EXTRA tiles
 // synchronise the workgroup (one wave per workgroup) and stage rows through `lds`
 // (BR * 64 * V REALs).
-template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
+template <typename R, int V, int T, int BR, int S, bool EXTRA>
 __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                                           const int4* __restrict__ tiles, int n_tiles, int b,
                                           int n_groups, int waves, int wave,
                                           const int* __restrict__ counter, int kstep0, int parity,
                                           R* __restrict__ lds) {
     constexpr int RW = BR + 2 * T;  // window rows (Ey: one more)
+    // S: which CPML strips the window meets — bit 0 the y strips (columns), bit 1 the x
+    // strips (rows, and the grid's first and last rows).  Only those psi arrays are
+    // loaded, updated and stored, and only those boundary predicates evaluated.
+    constexpr bool SY = (S & 1) != 0, SX = (S & 2) != 0;
     const int per_xcd = (n_groups + 7) >> 3;
     const int li = ((b & 7) * per_xcd + (b >> 3)) * waves + wave;
     if (li >= n_tiles) return;
@@ -237,9 +243,10 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     const G* __restrict__ cby = to_global(a.cb_ey);
 
     // ---- one burst of loads: the whole window ----
-    constexpr int PR = STRIP ? RW : 1;  // psi rows held (none in interior tiles)
+    constexpr int PY = SY ? RW : 1;  // psi rows held (none in interior tiles)
+    constexpr int PX = SX ? RW : 1;
     R ex[RW][V], hz[RW][V], ey[RW + 1][V], cx[RW][V], cy[RW][V];
-    R phy[PR][V], phx[PR][V], pey[PR][V], pex[PR][V];
+    R phy[PY][V], phx[PX][V], pey[PY][V], pex[PX][V];
 #pragma unroll
     for (int l = 0; l < RW; ++l) {
         const int g = g0 + l;
@@ -250,11 +257,13 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
         ldv<R, V>(ey[l], eyA, ge, ny, c, ny);
         ldv<R, V>(cx[l], cbx, gx, px, c, px);
         ldv<R, V>(cy[l], cby, ge, ny, c, ny);
-        if constexpr (STRIP) {
-            ldv<R, V>(phy[l], phyA, gx, ny, c, ny);
-            ldv<R, V>(phx[l], phxA, gx, ny, c, ny);
-            ldv<R, V>(pey[l], pexA, gx, px, c, px);
-            ldv<R, V>(pex[l], peyA, ge, ny, c, ny);
+        if constexpr (SY) {
+            ldv<R, V>(phy[SY ? l : 0], phyA, gx, ny, c, ny);
+            ldv<R, V>(pey[SY ? l : 0], pexA, gx, px, c, px);
+        }
+        if constexpr (SX) {
+            ldv<R, V>(phx[SX ? l : 0], phxA, gx, ny, c, ny);
+            ldv<R, V>(pex[SX ? l : 0], peyA, ge, ny, c, ny);
         }
     }
     ldv<R, V>(ey[RW], eyA, min(max(g0 + RW, 0), nx), ny, c, ny);
@@ -269,8 +278,8 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     for (int k = 0; k < V; ++k) {
         const int j = c + k;
         mH[k] = (j >= 0 && j < ny);
-        ysh[k] = STRIP && mH[k] && (j < yh_lo || j >= yh_hi);
-        yse[k] = STRIP && j >= 1 && j <= ny - 1 && (j < ye_lo || j >= ye_hi);
+        ysh[k] = SY && mH[k] && (j < yh_lo || j >= yh_hi);
+        yse[k] = SY && j >= 1 && j <= ny - 1 && (j < ye_lo || j >= ye_hi);
         byh[k] = ysh[k] ? to_global(a.by_h)[j] : R(1);
         cyh[k] = ysh[k] ? to_global(a.cy_h)[j] : R(0);
         ikyh[k] = ysh[k] ? to_global(a.iky_h)[j] : R(1);
@@ -282,7 +291,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     // Row profiles of the x strips: lane l holds those of window row l (RW <= 64),
     // read back with v_readlane where a row needs them.
     R pbxh = R(1), pcxh = R(0), pikxh = R(1), pbxe = R(1), pcxe = R(0), pikxe = R(1);
-    if constexpr (STRIP) {
+    if constexpr (SX) {
         const int gl = g0 + min(lane, RW - 1);
         const int gh = min(max(gl, 0), nx - 1), ge = min(max(gl, 0), nx);
         pbxh = to_global(a.bx_h)[gh];
@@ -342,7 +351,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
             const int g = g0 + l;
             bool xh = false;
             R bx = R(1), cxp = R(0), ikx = R(1);
-            if constexpr (STRIP) {
+            if constexpr (SX) {
                 xh = g < xh_lo || g >= xh_hi;
                 if (xh) {
                     bx = lane_value(pbxh, l);
@@ -357,16 +366,16 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                 const R d = (exn - ex[l][k]) * idy;
                 const R e = (ey[l + 1][k] - ey[l][k]) * idx;
                 R ty, tx;
-                if (STRIP && ysh[k]) {
-                    const R py = byh[k] * phy[STRIP ? l : 0][k] + cyh[k] * d;
-                    phy[STRIP ? l : 0][k] = py;
+                if (SY && ysh[k]) {
+                    const R py = byh[k] * phy[SY ? l : 0][k] + cyh[k] * d;
+                    phy[SY ? l : 0][k] = py;
                     ty = d * ikyh[k] + py;
                 } else {
                     ty = d + zero;
                 }
-                if (STRIP && xh) {
-                    const R pxv = bx * phx[STRIP ? l : 0][k] + cxp * e;
-                    phx[STRIP ? l : 0][k] = pxv;
+                if (SX && xh) {
+                    const R pxv = bx * phx[SX ? l : 0][k] + cxp * e;
+                    phx[SX ? l : 0][k] = pxv;
                     tx = e * ikx + pxv;
                 } else {
                     tx = e + zero;
@@ -382,12 +391,13 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
         for (int l = t; l < RW - t; ++l) {
             const int g = g0 + l;
             const R hl = from_prev_lane(hz[l][V - 1]);  // lane 0: outside the window
-            // Interior tiles lie inside rows [1, nx - 1] and columns [1, ny - 1].
-            const bool ex_row = !STRIP || (g >= 0 && g <= nx - 1);
-            const bool ey_row = (l >= t + 1) && (!STRIP || (g >= 1 && g <= nx - 1));
+            // Without SX the window lies inside rows [1, nx - 1], without SY inside
+            // columns [1, ny - 1].
+            const bool ex_row = !SX || (g >= 0 && g <= nx - 1);
+            const bool ey_row = (l >= t + 1) && (!SX || (g >= 1 && g <= nx - 1));
             bool xe = false;
             R bxe = R(1), cxe = R(0), ikxe = R(1);
-            if constexpr (STRIP) {
+            if constexpr (SX) {
                 xe = ey_row && (g < xe_lo || g >= xe_hi);
                 if (xe) {
                     bxe = lane_value(pbxe, l);
@@ -400,13 +410,13 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                 const int j = c + k;
                 // Ex(g, j); j = 0, ny stay unchanged (a select, not a branch)
                 if (ex_row) {
-                    const bool upd = !STRIP || (j >= 1 && j <= ny - 1);
+                    const bool upd = !SY || (j >= 1 && j <= ny - 1);
                     const R hleft = (k == 0) ? hl : hz[l][k - 1];
                     const R d = (hz[l][k] - hleft) * idy;
                     R tt;
-                    if (STRIP && yse[k]) {
-                        const R p = bye[k] * pey[STRIP ? l : 0][k] + cye[k] * d;
-                        pey[STRIP ? l : 0][k] = p;
+                    if (SY && yse[k]) {
+                        const R p = bye[k] * pey[SY ? l : 0][k] + cye[k] * d;
+                        pey[SY ? l : 0][k] = p;
                         tt = d * ikye[k] + p;
                     } else {
                         tt = d + zero;
@@ -417,9 +427,9 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                 if (ey_row) {  // Ey(g, j), g in [1, nx - 1]; rows 0 and nx stay unchanged
                     const R d = (hz[l][k] - hz[l > 0 ? l - 1 : 0][k]) * idx;
                     R tt;
-                    if (STRIP && xe) {
-                        const R p = bxe * pex[STRIP ? l : 0][k] + cxe * d;
-                        pex[STRIP ? l : 0][k] = p;
+                    if (SX && xe) {
+                        const R p = bxe * pex[SX ? l : 0][k] + cxe * d;
+                        pex[SX ? l : 0][k] = p;
                         tt = d * ikxe + p;
                     } else {
                         tt = d + zero;
@@ -453,21 +463,25 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
         store_row<R, V>(hzB + int64_t(g) * ny, hz[l], c, lo, min(hi, ny));
         store_row<R, V>(exB + int64_t(g) * px, ex[l], c, lo, min(hi, px));
         store_row<R, V>(eyB + int64_t(g) * ny, ey[l], c, lo, min(hi, ny));
-        if constexpr (STRIP) {
-            bool mo[V], moy[V], moe[V];
+        if constexpr (SY) {
+            bool moy[V], moe[V];
 #pragma unroll
             for (int k = 0; k < V; ++k) {
                 const bool own = (c + k >= lo && c + k < hi);
-                mo[k] = own && mH[k];
                 moy[k] = own && ysh[k];
                 moe[k] = own && yse[k];
             }
-            store_masked<R, V>(phyB + int64_t(g) * ny, phy[STRIP ? l : 0], c, moy);
-            store_masked<R, V>(pexB + int64_t(g) * px, pey[STRIP ? l : 0], c, moe);
+            store_masked<R, V>(phyB + int64_t(g) * ny, phy[SY ? l : 0], c, moy);
+            store_masked<R, V>(pexB + int64_t(g) * px, pey[SY ? l : 0], c, moe);
+        }
+        if constexpr (SX) {
+            bool mo[V];
+#pragma unroll
+            for (int k = 0; k < V; ++k) mo[k] = (c + k >= lo && c + k < hi) && mH[k];
             if (g < xh_lo || g >= xh_hi)
-                store_masked<R, V>(phxB + int64_t(g) * ny, phx[STRIP ? l : 0], c, mo);
+                store_masked<R, V>(phxB + int64_t(g) * ny, phx[SX ? l : 0], c, mo);
             if (g >= 1 && (g < xe_lo || g >= xe_hi))
-                store_masked<R, V>(peyB + int64_t(g) * ny, pex[STRIP ? l : 0], c, mo);
+                store_masked<R, V>(peyB + int64_t(g) * ny, pex[SX ? l : 0], c, mo);
         }
     }
     // the last band copies Ey row nx (outer tangential E)
@@ -479,7 +493,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
 }
 
 // One wave per tile, `blockDim.x / 64` waves per workgroup (always one for EXTRA).
-template <typename R, int V, int T, int BR, bool STRIP, bool EXTRA>
+template <typename R, int V, int T, int BR, int S, bool EXTRA>
 __global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restrict__ args,
                                                    const int4* __restrict__ tiles, int n_tiles,
                                                    const int* __restrict__ counter, int kstep0,
@@ -489,7 +503,7 @@ __global__ __launch_bounds__(256) void tile_kernel(const FusedArgs<R>* __restric
     // the wave index is uniform across the wave: readfirstlane puts it (and all that
     // derives from it — the descriptor, the simulation's arguments) in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    tile_body<R, V, T, BR, STRIP, EXTRA>(args, tiles, n_tiles, blockIdx.x,
+    tile_body<R, V, T, BR, S, EXTRA>(args, tiles, n_tiles, blockIdx.x,
                                          (n_tiles + waves - 1) / waves, waves, wave, counter,
                                          kstep0, parity, lds);
 }

@@ -16,6 +16,7 @@ SI; arrays are cast once to the run's precision.
 
 from __future__ import annotations
 
+import functools
 import time
 from typing import Any, Optional
 
@@ -35,6 +36,19 @@ from etalon_amd.state import FIELDS, build_te_state
 _PHASOR_TABLE_ENTRIES = 1 << 22
 # Steps per fused-plan table refill (source deltas and phasors).
 _PLAN_TABLE_ROWS = 1 << 14
+
+
+@functools.lru_cache(maxsize=16)
+def _phasor_table(n0: int, rows: int, dt: float, magnetic: bool, freqs: bytes,
+                  dtype: str) -> np.ndarray:
+    """Phasors of steps [n0, n0 + rows) at H times (n+½)Δt or E times (n+1)Δt, cast once.
+    The same for every simulation of one Δt and frequency list (a batch of one grid
+    step shares it); read-only."""
+    steps = np.arange(n0, n0 + rows, dtype=np.float64)
+    t = (steps + 0.5) * dt if magnetic else (steps + 1.0) * dt
+    table = dft_phasors(np.frombuffer(freqs, np.float64), t, dt).astype(np.dtype(dtype))
+    table.setflags(write=False)
+    return table
 
 
 class _MonitorAccumulator:
@@ -184,14 +198,15 @@ class FDTDSolver:
         if self.injections:
             delta = np.concatenate([inj.delta_table(n0, rows, dt) for inj in self.injections],
                                    axis=1).astype(self.real)
-        steps = np.arange(n0, n0 + rows, dtype=np.float64)
         phasors = []
         for mon, owner in zip(self.monitors, self._phasor_owner()):
             if owner is not None:  # the plan reads the owner's table
                 phasors.append(None)
                 continue
-            t = (steps + 0.5) * dt if mon.sampling.is_magnetic else (steps + 1.0) * dt
-            phasors.append(dft_phasors(mon.sampling.frequencies, t, dt).astype(self.complex))
+            smp = mon.sampling
+            phasors.append(_phasor_table(n0, rows, dt, smp.is_magnetic,
+                                         np.asarray(smp.frequencies, np.float64).tobytes(),
+                                         np.dtype(self.complex).str))
         return delta, phasors
 
     def fused_supported(self) -> bool:

@@ -140,9 +140,9 @@ typedef struct EtlTune {
     int temporal;      /* steps fused per launch: 1 (none), 2 or 4 */
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
-                        * band 8; one wave per workgroup); 3: one kernel per tile kind on
-                        * three streams; 0: 1 for float where the shape allows it,
-                        * else 3 */
+                        * band 8, or 16 in float; one wave per workgroup); 3: one kernel
+                        * per tile kind on three streams; 0: 1 for float where the shape
+                        * allows it, else 3 */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
