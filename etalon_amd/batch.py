@@ -53,6 +53,10 @@ class BatchSolver:
         self.tune = tune
         self._plan = None
         self._ids: list[list[int]] = []
+        # A batch always runs on the fused plan: build it (second field buffers,
+        # tile work-list, monitor layout) with the states rather than in the first
+        # advance.
+        self._ensure_plan()
 
     @property
     def cells(self) -> int:
@@ -65,6 +69,7 @@ class BatchSolver:
 
             plan = FusedPlan([s.state for s in self.solvers], self.tune)
             self._ids = [s.register(plan, k) for k, s in enumerate(self.solvers)]
+            plan.prepare()
             self._plan = plan
         return self._plan
 

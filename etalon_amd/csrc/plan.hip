@@ -923,6 +923,26 @@ int fwd_after_chunk(EtlPlan* p, int rows, hipStream_t s) {
 
 extern "C" {
 
+namespace {
+
+// Default tile rows: 16 for float plans of one launch per step block with enough
+// 16-row tiles to give every CU of the device 8 of them (taller tiles re-read fewer
+// halo rows per step block), else 8 (more, smaller tiles to fill the device).
+int default_band(const EtlTeDesc* descs, int n_sims, int precision, const EtlTune& t) {
+    const int vec = t.vec == 0 ? 1 : t.vec;
+    const int depth = t.temporal == 0 ? 4 : t.temporal;
+    if (precision != ETL_F32 || vec != 1 || t.launch == 3 ||
+        !tile_shape_supported(precision, 1, depth, 16) || !tile_shape_supported(precision, 1, 1, 16))
+        return 8;
+    const int64_t cols = 64 - 2 * depth;
+    int64_t tiles = 0;
+    for (int k = 0; k < n_sims; ++k)
+        tiles += int64_t((descs[k].nx + 15) / 16) * ((descs[k].ny + cols - 1) / cols);
+    return tiles >= int64_t(8) * std::max(ctx().cu_count, 1) ? 16 : 8;
+}
+
+}  // namespace
+
 int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tune,
                           etl_plan_t* out) {
     ETL_CHECK_INIT();
@@ -935,7 +955,7 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     if (tune) t = *tune;
     if (t.vec == 0) t.vec = 1;
     if (t.temporal == 0) t.temporal = 4;
-    if (t.band_rows == 0) t.band_rows = 8;
+    if (t.band_rows == 0) t.band_rows = default_band(descs, n_sims, precision, t);
     if (t.trace_chunk == 0) t.trace_chunk = 64;
     if (t.waves == 0) t.waves = 1;
     if (tune == nullptr) t.use_graph = 1;
@@ -961,10 +981,12 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     p->sims.resize(n_sims);
     int rc = ETL_OK;
     for (int k = 0; k < n_sims && rc == ETL_OK; ++k) rc = init_sim(p, &descs[k], p->sims[k]);
-    // float: one launch per step block (vec 1, band 8); double tiles need twice the
-    // registers, and their strip tiles in the joint kernel would set its occupancy
+    // one launch per step block wherever the joint kernel is compiled for the shape
     if (rc == ETL_OK && p->tune.launch == 0)
-        p->tune.launch = (precision == ETL_F32 && t.vec == 1 && t.band_rows == 8) ? 1 : 3;
+        p->tune.launch =
+            (t.vec == 1 && (t.band_rows == 8 || (t.band_rows == 16 && precision == ETL_F32)))
+                ? 1
+                : 3;
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
@@ -1205,6 +1227,12 @@ int etl_plan_tune(etl_plan_t p, EtlTune* out) {
     ETL_REQUIRE(p && out, "etl_plan_tune: null argument");
     *out = p->tune;
     return ETL_OK;
+}
+
+int etl_plan_prepare(etl_plan_t p) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_plan_prepare: null plan");
+    return p->dirty ? build_layout(p) : ETL_OK;
 }
 
 int etl_plan_advance(etl_plan_t p, int64_t n, int64_t count, int* parity) {

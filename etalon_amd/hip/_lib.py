@@ -142,6 +142,7 @@ _SIGNATURES = {
     "etl_plan_advance_checkpointed": [vp, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_int)],
     "etl_adjplan_lambda": [vp, C.c_int, C.c_int, C.POINTER(vp)],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],
+    "etl_plan_prepare": [vp],
     "etl_plan_advance": [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int)],
     "etl_plan_advance_rows": [vp, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_int)],
     "etl_plan_set_lane": [vp, C.c_int],

@@ -24,10 +24,11 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h, per simulation
 
 @dataclass
 class Tune:
-    """Fused-kernel shape; 0 = the library default (vec 1, 8-row tiles, temporal
-    depth 4, 64-step trace chunks, one wave per workgroup, and ``launch`` 1 — one
-    kernel per step block over every tile kind — for float where the shape allows it
-    (vec 1, band 8), else 3 — one kernel per tile kind on three streams)."""
+    """Fused-kernel shape; 0 = the library default: vec 1, temporal depth 4, 64-step
+    trace chunks, one wave per workgroup, ``launch`` 1 (one kernel per step block over
+    every tile kind) where the shape allows it (vec 1 with band 8, or band 16 in
+    float), else 3 (one kernel per tile kind on three streams), and 16-row tiles for
+    float plans of launch 1 with at least 8 such tiles per CU, else 8-row tiles."""
 
     band_rows: int = 0
     vec: int = 0
@@ -162,6 +163,11 @@ class FusedPlan:
     def set_lane(self, lane: int) -> None:
         """Issue this plan's work on stream lane 0 or 1 (see etl_lane_wait)."""
         _lib.call("etl_plan_set_lane", self.handle, int(lane))
+
+    def prepare(self) -> None:
+        """Build the layout (tile work-list, monitor slots) now rather than at the
+        first table upload or advance."""
+        _lib.call("etl_plan_prepare", self.handle)
 
     def resolved_tune(self) -> Tune:
         """The tuning in effect (library defaults filled in)."""

@@ -11,6 +11,7 @@ does, then cast once to the run's precision.
 
 from __future__ import annotations
 
+import functools
 from dataclasses import dataclass
 
 import numpy as np
@@ -48,35 +49,55 @@ class PointInjection:
         return self.coefficient * pulse_signal(self.pulse, self.sample_time(n, dt))
 
     def delta_table(self, n0: int, rows: int, dt: float) -> np.ndarray:
-        """delta(n) for n in [n0, n0 + rows), shape (rows, nodes), float64.
-
-        Evaluated over the whole time column at once, with the operations of
-        ``delta(n)`` per step.  Per step, ``s`` is a NumPy float64 scalar and ``s**2``
-        is the scalar power (C ``pow``), which can differ by one ulp from the array
-        square, so the squares are taken per element the same way.  Whether exp and
-        sin give the same bits for an array as for one element depends on the NumPy
-        build and the CPU, so sampled rows are checked against the per-step
-        evaluation and any difference switches to one evaluation per step.
-        """
-        steps = np.arange(n0, n0 + rows, dtype=np.float64)
-        t = steps * dt if self.is_magnetic else (steps + 0.5) * dt
-        p = self.pulse
-        s = t - p.delay
-        s2 = np.fromiter((x**2 for x in s), dtype=np.float64, count=s.size)
-        g = p.amplitude * np.exp(-s2 / (2.0 * p.width**2)) * np.sin(2.0 * np.pi * p.frequency * s)
-        out = self.coefficient[None, :] * g[:, None]
-        if rows and not _matches_per_step(self, out, n0, dt):
-            for r in range(rows):
-                out[r] = self.delta(n0 + r, dt)
-        return out
+        """delta(n) for n in [n0, n0 + rows), shape (rows, nodes), float64: the
+        coefficients times the pulse column ``signal_column`` (the same product of
+        the same float64 numbers as ``delta(n)``)."""
+        g = signal_column(self.pulse, n0, rows, dt, self.is_magnetic)
+        return self.coefficient[None, :] * g[:, None]
 
 
-def _matches_per_step(inj: PointInjection, table: np.ndarray, n0: int, dt: float) -> bool:
-    rows = table.shape[0]
+def _pulse_key(pulse) -> tuple:
+    return (float(pulse.amplitude), float(pulse.delay), float(pulse.width),
+            float(pulse.frequency))
+
+
+@functools.lru_cache(maxsize=32)
+def _signal_column_cached(key: tuple, n0: int, rows: int, dt: float, magnetic: bool) -> np.ndarray:
+    amplitude, delay, width, frequency = key
+    pulse = _Pulse(amplitude, delay, width, frequency)
+    steps = np.arange(n0, n0 + rows, dtype=np.float64)
+    t = steps * dt if magnetic else (steps + 0.5) * dt
+    s = t - delay
+    # Per step, ``s`` is a NumPy float64 scalar and ``s**2`` is the scalar power (C
+    # ``pow``), which can differ by one ulp from the array square, so the squares are
+    # taken per element the same way.
+    s2 = np.fromiter((x**2 for x in s), dtype=np.float64, count=s.size)
+    g = amplitude * np.exp(-s2 / (2.0 * width**2)) * np.sin(2.0 * np.pi * frequency * s)
+    # Whether exp and sin give the same bits for an array as for one element depends
+    # on the NumPy build and the CPU: sampled rows are checked against the per-step
+    # evaluation and any difference switches to one evaluation per step.
     sample = sorted(set(range(min(rows, 16))) | set(range(0, rows, max(1, rows // 64)))
-                    | {rows - 1})
-    return all(np.array_equal(table[r].view(np.int64), inj.delta(n0 + r, dt).view(np.int64))
-               for r in sample)
+                    | ({rows - 1} if rows else set()))
+    if not all(g[r].view(np.int64) == pulse_signal(pulse, t[r]).view(np.int64) for r in sample):
+        g = np.array([pulse_signal(pulse, t[r]) for r in range(rows)], dtype=np.float64)
+    g.setflags(write=False)
+    return g
+
+
+@dataclass(frozen=True)
+class _Pulse:
+    amplitude: float
+    delay: float
+    width: float
+    frequency: float
+
+
+def signal_column(pulse, n0: int, rows: int, dt: float, magnetic: bool) -> np.ndarray:
+    """g(t_n) for n in [n0, n0 + rows) at the injection times of an H (n Δt) or E
+    ((n + ½) Δt) source, float64, read-only; equal bit for bit to ``pulse_signal``
+    at each single time.  Cached: the simulations of a batch that share a pulse and
+    Δt share the column."""
+    return _signal_column_cached(_pulse_key(pulse), int(n0), int(rows), float(dt), bool(magnetic))
 
 
 def point_injection(source, grid: YeeGrid2D, eps_r: dict) -> PointInjection:

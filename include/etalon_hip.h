@@ -133,7 +133,8 @@ typedef struct EtlTeDesc {
  * 16, or vec 2 with band 8, at temporal depth 1, 2 or 4; float64 vec 1 with band 8
  * (depth 1, 2, 4) or band 16 (depth 1, 2). */
 typedef struct EtlTune {
-    int band_rows;     /* output rows of one wave's tile */
+    int band_rows;     /* output rows of one wave's tile; 0: 16 for float plans of launch
+                        * mode 1 with at least 8 such tiles per CU, else 8 */
     int vec;           /* REAL elements per lane along y */
     int use_graph;     /* replay steps from a hipGraph (1) or launch directly (0) */
     int trace_chunk;   /* steps per DFT flush: a multiple of 2 * temporal */
@@ -141,8 +142,8 @@ typedef struct EtlTune {
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
                         * band 8, or 16 in float; one wave per workgroup); 3: one kernel
-                        * per tile kind on three streams; 0: 1 for float where the shape
-                        * allows it, else 3 */
+                        * per tile kind on three streams; 0: 1 where the shape allows
+                        * it, else 3 */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
@@ -193,6 +194,10 @@ int etl_plan_add_probe_sim(etl_plan_t plan, int sim, int comp, int n_nodes,
 int etl_plan_share_phasors(etl_plan_t plan, int sim, int id, int owner);
 /* The tuning the plan resolved (library defaults filled in). */
 int etl_plan_tune(etl_plan_t plan, EtlTune* out);
+/* Build the plan's layout now (tile work-list, monitor and probe slots, device
+ * tables) instead of at the first table or advance call; a no-op when it is up to
+ * date.  No reference counterpart (the reference has no plan). */
+int etl_plan_prepare(etl_plan_t plan);
 /* Advance steps [n, n + count); the tables must cover them.  On return *parity
  * tells which buffer of each pair holds the state (0: field[k][0]). */
 int etl_plan_advance(etl_plan_t plan, int64_t n, int64_t count, int* parity);
