@@ -430,13 +430,42 @@ def run_batched_adjoint(args) -> dict:
     batch = BatchAdjointProblem(problems)
     print(f"batched adjoint: {len(sims)} candidates set up in "
           f"{time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
-    batch.run(eps)  # warm: plans, graphs, allocator
+    # the first gradient of a fresh problem: the batch's states and forward plan are
+    # built outside the timed region (as for the forward run below), its checkpoint
+    # storage, reverse plans, co-plan and graphs inside
     tape = batch.forward(eps)
     t0 = time.perf_counter()
-    grads = batch.backward(tape)
-    t_grad = tape.forward_s + time.perf_counter() - t0
-    J32 = [float(j) for j in tape.J]
+    batch.backward(tape)
+    t_cold = tape.forward_s + time.perf_counter() - t0
     del tape
+    # a design loop: the session of the first gradient kept (best of 3)
+    t_grad, grads, J32 = float("inf"), None, None
+    for _ in range(3):
+        tape = batch.forward(eps)
+        t0 = time.perf_counter()
+        g = batch.backward(tape)
+        t = tape.forward_s + time.perf_counter() - t0
+        if t < t_grad:
+            t_grad, grads, J32 = t, g, [float(j) for j in tape.J]
+        del tape
+    batch.close()
+    # store mode: the forward pass keeps every design box's Hz of every step in HBM and
+    # the reverse steps read it, no recompute (first gradient warms, best of 3 after)
+    stored = BatchAdjointProblem([AdjointProblem(sim, names, mode="store") for sim in sims])
+    t_store, g_store = float("inf"), None
+    for rep in range(4):
+        tape = stored.forward(eps)
+        t0 = time.perf_counter()
+        g = stored.backward(tape)
+        t = tape.forward_s + time.perf_counter() - t0
+        hist_bytes = sum(h.nbytes for h in tape.histories)
+        if rep > 0 and t < t_store:
+            t_store, g_store = t, g
+        del tape
+    stored.close()
+    # the store sweep groups the reverse steps into other temporal blocks than the
+    # checkpointed one: the float64 gradients agree to rounding, not bit for bit
+    store_rel = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(g_store, grads))
     design_sims = [p.with_permittivity(e) for p, e in zip(problems, eps)]
     BatchSolver(design_sims).run()  # warm
     fwd = BatchSolver(design_sims)
@@ -483,7 +512,8 @@ def run_batched_adjoint(args) -> dict:
     adj.gradient(short, set(names), short_design)
     cpu_wall = time.perf_counter() - t0
     n_cells = sum(p.grid.nx * p.grid.ny for p in problems)
-    return {"t_grad": t_grad, "t_fwd": t_fwd, "J": J32,
+    return {"t_grad": t_grad, "t_cold": t_cold, "t_fwd": t_fwd, "J": J32, "t_store": t_store,
+            "store_rel": store_rel, "hist_bytes": hist_bytes,
             "J64_0": J64, "grad_rel_f64": float(np.abs(grads[0] - g64).max() / scale),
             "fd_rel": fd_rel, "cpu_wall": cpu_wall, "cells": n_cells, "n": len(problems),
             "steps": problems[0].n_steps, "K": problems[0].K, "pixels": eps[0].shape,
@@ -497,7 +527,8 @@ def main_batched_adjoint(args) -> int:
     line = {
         "metric": "batched adjoint grad time vs batched fwd",
         "value": round(r["t_grad"] / r["t_fwd"], 3),
-        "unit": "x (wall of J and dJ/deps of every candidate / wall of one batched plain "
+        "unit": "x (wall of J and dJ/deps of every candidate in a design loop: batch, plans "
+                "and graphs kept between gradients, best of 3 / wall of one batched plain "
                 "forward run of them)",
         "n_gpus": 1,
         "steps": r["steps"],
@@ -520,6 +551,17 @@ def main_batched_adjoint(args) -> int:
                    "cells_total": r["cells"]},
         "t_grad_s": round(r["t_grad"], 3),
         "t_fwd_s": round(r["t_fwd"], 3),
+        "cold": {"t_grad_s": round(r["t_cold"], 3), "ratio": round(r["t_cold"] / r["t_fwd"], 3),
+                 "note": "first gradient of a fresh problem (checkpoint storage, reverse "
+                         "plans, co-plan and graphs built inside the timed region)"},
+        "store_history": {"t_grad_s": round(r["t_store"], 3),
+                          "ratio": round(r["t_store"] / r["t_fwd"], 3),
+                          "candidates_per_s": round(r["n"] / r["t_store"], 2),
+                          "history_bytes": int(r["hist_bytes"]),
+                          "vs_checkpoint_max_rel": float(f"{r['store_rel']:.3g}"),
+                          "note": "design loop, best of 3: the forward pass keeps each design "
+                                  "box's Hz of every step in HBM, the reverse steps read it "
+                                  "(no recompute)"},
         "candidates_per_s": round(r["n"] / r["t_grad"], 2),
         "gcells_grad": round(3 * r["cells"] * r["steps"] / r["t_grad"] / 1e9, 1),
         "gradient_check": {"candidate": 0,

@@ -248,21 +248,10 @@ class AdjointProblem:
         permittivity (the PixelGrid rule, the last structure wins there), cb_ex and
         cb_ey are recomputed and uploaded, fields and monitors go back to zero, and the
         same plan, tables and checkpoint storage run again."""
-        eps = np.asarray(eps, dtype=np.float64)
-        if eps.shape != self.shape:
-            raise ValueError(f"design permittivity must be {self.shape}, got {eps.shape}")
         ses = self._session
         solver = ses.solver
         t0 = time.perf_counter()
-        flat = eps.ravel()
-        for c in ("Ex", "Ey"):
-            ii, jj, p = self.pixmap[c]
-            solver.eps_r[c][ii, jj] = flat[p]
-        cb = cb_arrays(solver.grid, solver.eps_r)
-        solver.state.cb_ex.set(cb["cb_ex"].astype(solver.real))
-        solver.state.cb_ey.set(cb["cb_ey"].astype(solver.real))
-        solver.sim = self.with_permittivity(eps)
-        solver.reset()
+        self.set_design(solver, eps)
         plan = solver._ensure_plan()
         plan.advance_checkpointed(0, self.n_steps, self.K)
         plan.bind()
@@ -272,6 +261,25 @@ class AdjointProblem:
         tape.forward_s = time.perf_counter() - t0
         tape.J = self._objective(solver)
         return tape
+
+    def set_design(self, solver, eps: np.ndarray) -> None:
+        """Give ``solver`` (a solver of this problem's geometry) the design permittivity
+        ``eps`` and take it back to step 0: the design's nodes take the new values (the
+        PixelGrid rule, the last structure wins there), cb_ex and cb_ey are recomputed
+        from the whole permittivity map and uploaded in place (their device addresses,
+        which plans hold, stay), and fields and monitors go back to zero."""
+        eps = np.asarray(eps, dtype=np.float64)
+        if eps.shape != self.shape:
+            raise ValueError(f"design permittivity must be {self.shape}, got {eps.shape}")
+        flat = eps.ravel()
+        for c in ("Ex", "Ey"):
+            ii, jj, p = self.pixmap[c]
+            solver.eps_r[c][ii, jj] = flat[p]
+        cb = cb_arrays(solver.grid, solver.eps_r)
+        solver.state.cb_ex.set(cb["cb_ex"].astype(solver.real))
+        solver.state.cb_ey.set(cb["cb_ey"].astype(solver.real))
+        solver.sim = self.with_permittivity(eps)
+        solver.reset()
 
     def close(self) -> None:
         """Release the session (plans, graphs, buffers) of a design loop."""

@@ -14,7 +14,10 @@ adjoint as ``AdjointProblem``, batched:
    and the reverse steps of segment k of every candidate.
 
 Each candidate's gradient is bit-identical to its own ``AdjointProblem`` (tile
-kernels, same tuning).  No reference counterpart (the reference has no adjoint).
+kernels, same tuning).  In a design loop (``reuse``, the default) the batch, plans,
+graphs and buffers of the first gradient are kept: later gradients give each
+candidate its new permittivity in place and run the same plans again.  No reference
+counterpart (the reference has no adjoint).
 """
 
 from __future__ import annotations
@@ -38,28 +41,79 @@ class BatchTape:
     batch: BatchSolver
     J: list = field(default_factory=list)
     checkpoints: list = field(default_factory=list)  # per candidate: field -> (n_ck, size)
+    histories: list = field(default_factory=list)    # store mode: per candidate (N, box nodes)
     forward_s: float = 0.0
 
 
+@dataclass
+class _BatchSession:
+    """What a design loop keeps between batched gradients of one set of geometries:
+    the batch (states, forward plan, tables, checkpoint storage) and, after the first
+    backward pass, the reverse plan, the recompute plan and their co-plan."""
+
+    batch: BatchSolver
+    checkpoints: list
+    histories: list
+    bufs: list = field(default_factory=list)
+    rev: Optional[_Reverse] = None
+    rplan: Optional[object] = None
+    co: C.c_void_p = field(default_factory=C.c_void_p)
+
+    def close(self) -> None:
+        if self.co:
+            _lib.call("etl_coplan_destroy", self.co)
+            self.co = C.c_void_p()
+        if self.rplan is not None:
+            self.rplan.close()
+            self.rplan = None
+        if self.rev is not None:
+            self.rev.close()
+            self.rev = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BatchAdjointProblem:
-    def __init__(self, problems: Sequence[AdjointProblem], tune=None) -> None:
+    def __init__(self, problems: Sequence[AdjointProblem], tune=None, reuse: bool = True) -> None:
         if not problems:
             raise ValueError("BatchAdjointProblem needs at least one candidate")
         p0 = problems[0]
         for p in problems:
-            if p.kernel != "tile" or p.mode != "checkpoint":
-                raise ValueError("batched candidates use the tile kernels in checkpoint mode")
-            if p.n_steps != p0.n_steps or p.K != p0.K or p.sim.precision != p0.sim.precision:
+            if p.kernel != "tile":
+                raise ValueError("batched candidates use the tile kernels")
+            if (p.n_steps != p0.n_steps or p.K != p0.K or p.sim.precision != p0.sim.precision
+                    or p.mode != p0.mode):
                 raise ValueError("batched candidates share the step count, the checkpoint "
-                                 "interval and the precision")
+                                 "interval, the precision and the mode")
         self.problems = list(problems)
+        # "checkpoint": segments recomputed beside the reverse steps (co-plan);
+        # "store": the forward pass keeps every step's Hz on each design box and the
+        # reverse steps read it (no recompute; N x box nodes values per candidate)
+        self.mode = p0.mode
         self.tune = tune
         self.n_steps, self.K = p0.n_steps, p0.K
+        # a design loop's gradients keep the batch, plans, graphs and buffers of the
+        # first one: new permittivities only replace each candidate's cb_ex and cb_ey.
+        # A tape is then valid until the next forward.
+        self.reuse = bool(reuse)
+        self._session: Optional[_BatchSession] = None
+
+    def close(self) -> None:
+        """Release the session (plans, graphs, buffers) of a design loop."""
+        if self._session is not None:
+            self._session.close()
+            self._session = None
 
     def forward(self, eps_list: Sequence[np.ndarray]) -> BatchTape:
         """Run every candidate with its design permittivity; J per candidate and the tape."""
         if len(eps_list) != len(self.problems):
             raise ValueError("one permittivity array per candidate")
+        if self.reuse and self._session is not None:
+            return self._forward_again(eps_list)
         sims = [p.with_permittivity(e) for p, e in zip(self.problems, eps_list)]
         batch = BatchSolver(sims, tune=self.tune)
         for p, s in zip(self.problems, batch.solvers):
@@ -69,34 +123,120 @@ class BatchAdjointProblem:
         plan = batch._ensure_plan()
         tape = BatchTape(batch=batch)
         n_ck = (N + K - 1) // K
-        for k, (s, ids) in enumerate(zip(batch.solvers, batch._ids)):
+        for k, (p, s, ids) in enumerate(zip(self.problems, batch.solvers, batch._ids)):
             delta, phasors = s._tables(0, N)
             if delta is not None:
                 plan.source_table(0, delta, sim=k)
             for mon_id, table in zip(ids, phasors):
                 if table is not None:
                     plan.phasor_table(mon_id, 0, table, sim=k)
-            ck = {name: zeros((n_ck, getattr(s.state, name).size), s.real) for name in FIELDS}
-            plan.set_checkpoints([ck[name] for name in FIELDS], n_ck, sim=k)
-            tape.checkpoints.append(ck)
-        plan.advance_checkpointed(0, N, K)  # slot k = every candidate's state at step k K
-        plan.bind()
+            if self.mode == "store":
+                nodes = p._box_hz_nodes()
+                hist = zeros((N, nodes.size), s.real)
+                plan.add_probe("Hz", nodes, hist, sim=k)
+                tape.histories.append(hist)
+            else:
+                ck = {name: zeros((n_ck, getattr(s.state, name).size), s.real)
+                      for name in FIELDS}
+                plan.set_checkpoints([ck[name] for name in FIELDS], n_ck, sim=k)
+                tape.checkpoints.append(ck)
+        self._advance_forward(plan)
         for s in batch.solvers:
             s.n = N
         batch.n = N
         batch.backend.synchronize()
         tape.forward_s = time.perf_counter() - t0
         tape.J = [p._objective(s) for p, s in zip(self.problems, batch.solvers)]
+        if self.reuse:
+            self._session = _BatchSession(batch=batch, checkpoints=tape.checkpoints,
+                                          histories=tape.histories)
+        return tape
+
+    def _advance_forward(self, plan) -> None:
+        if self.mode == "store":
+            plan.advance(0, self.n_steps)  # probe row n = every candidate's box at step n
+        else:
+            # slot k = every candidate's state at step k K
+            plan.advance_checkpointed(0, self.n_steps, self.K)
+        plan.bind()
+
+    def _forward_again(self, eps_list: Sequence[np.ndarray]) -> BatchTape:
+        """The forward pass on the session's batch: every candidate takes its new design
+        permittivity (``AdjointProblem.set_design``), and the same plan, tables and
+        checkpoint storage run again."""
+        ses = self._session
+        batch = ses.batch
+        t0 = time.perf_counter()
+        for p, s, e in zip(self.problems, batch.solvers, eps_list):
+            p.set_design(s, e)
+        self._advance_forward(batch._ensure_plan())
+        for s in batch.solvers:
+            s.n = self.n_steps
+        batch.n = self.n_steps
+        batch.backend.synchronize()
+        tape = BatchTape(batch=batch, checkpoints=ses.checkpoints, histories=ses.histories)
+        tape.forward_s = time.perf_counter() - t0
+        tape.J = [p._objective(s) for p, s in zip(self.problems, batch.solvers)]
         return tape
 
     def backward(self, tape: BatchTape) -> list:
         """∂J/∂ε per design pixel (float64) of every candidate."""
+        ses = self._session
+        if self.reuse and ses is not None and tape.batch is ses.batch:
+            return self._backward_session(tape)
+        solvers = tape.batch.solvers
+        rev, rplan, co, bufs = self._reverse_plans(tape)
+        try:
+            self._sweep(rev, co)
+            tape.batch.backend.synchronize()
+        finally:
+            if co:
+                _lib.call("etl_coplan_destroy", co)
+            if rplan is not None:
+                rplan.close()
+            rev.close()
+        return [p._pixel_gradient(s, b) for p, s, b in zip(self.problems, solvers, bufs)]
+
+    def _sweep(self, rev, co) -> None:
+        """Every reverse step: from the stored histories, or with the recompute of
+        segment k - 1 beside the reverse steps of segment k (co-plan sweep)."""
+        if self.mode == "store":
+            rev.run(0, self.n_steps)
+        else:
+            _lib.call("etl_coplan_sweep", co, self.n_steps, self.K)
+
+    def _backward_session(self, tape: BatchTape) -> list:
+        """The backward pass on the session's plans: built on the first gradient, reset
+        (λ and ∂J/∂cb to zero, the new adjoint-source tables in place) on the next."""
+        ses = self._session
+        solvers = tape.batch.solvers
+        if ses.rev is None:
+            ses.rev, ses.rplan, ses.co, ses.bufs = self._reverse_plans(tape)
+        else:
+            _lib.call("etl_adjplan_reset", ses.rev.handle)
+            for k, (p, s, b) in enumerate(zip(self.problems, solvers, ses.bufs)):
+                b.g_ex.fill_zero()
+                b.g_ey.fill_zero()
+                comp, flat, table = p._sources(s)
+                ses.rev.sources(comp, flat, table, sim=k)
+        self._sweep(ses.rev, ses.co)
+        tape.batch.backend.synchronize()
+        return [p._pixel_gradient(s, b) for p, s, b in zip(self.problems, solvers, ses.bufs)]
+
+    def _reverse_plans(self, tape: BatchTape) -> tuple:
+        """The adjoint plan over every candidate (λ, ∂J/∂cb, adjoint sources) and, in
+        checkpoint mode, the recompute plan over every candidate (sources, the Hz probe
+        of each design box into its history, the tape's checkpoints) and their co-plan.
+        Returns (adjoint plan, recompute plan or None, co-plan or null, buffers)."""
         from etalon_amd.hip.plan import FusedPlan
 
         solvers = tape.batch.solvers
-        N, K = self.n_steps, self.K
-        hists = [zeros((2 * K, p._box_hz_nodes().size), s.real)
-                 for p, s in zip(self.problems, solvers)]
+        K = self.K
+        if self.mode == "store":
+            hists = tape.histories
+        else:
+            hists = [zeros((2 * K, p._box_hz_nodes().size), s.real)
+                     for p, s in zip(self.problems, solvers)]
         bufs = [p._adjoint_buffers(s, h) for p, s, h in zip(self.problems, solvers, hists)]
         rev = _Reverse("tile", [b.desc for b in bufs], self.tune)
         plan: Optional[FusedPlan] = None
@@ -105,21 +245,20 @@ class BatchAdjointProblem:
             for k, (p, s) in enumerate(zip(self.problems, solvers)):
                 comp, flat, table = p._sources(s)
                 rev.sources(comp, flat, table, sim=k)
+            if self.mode == "store":
+                return rev, None, co, bufs
             plan = FusedPlan([s.state for s in solvers], recompute_tune(self.tune))
             for k, (p, s, h) in enumerate(zip(self.problems, solvers, hists)):
                 p._configure_recompute(plan, s, k, h)
                 ck = tape.checkpoints[k]
                 plan.set_checkpoints([ck[name] for name in FIELDS], ck[FIELDS[0]].shape[0], sim=k)
             _lib.call("etl_coplan_create", plan.handle, rev.handle, C.byref(co))
-            _lib.call("etl_coplan_sweep", co, N, K)
-            tape.batch.backend.synchronize()
-        finally:
-            if co:
-                _lib.call("etl_coplan_destroy", co)
+        except BaseException:
             if plan is not None:
                 plan.close()
             rev.close()
-        return [p._pixel_gradient(s, b) for p, s, b in zip(self.problems, solvers, bufs)]
+            raise
+        return rev, plan, co, bufs
 
     def run(self, eps_list: Sequence[np.ndarray]) -> tuple[list, list]:
         """(J per candidate, ∂J/∂ε per candidate)."""

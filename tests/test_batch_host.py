@@ -80,3 +80,14 @@ def test_cylinder_lattice_painting_equals_the_oracle():
     for comp in ("Ex", "Ey"):
         assert np.array_equal(mine[comp], theirs[comp]), comp
         assert (mine[comp] == 1.0).sum() > 100, "holes painted"
+
+
+def test_batch_adjoint_needs_one_mode():
+    from etalon_amd.adjoint import AdjointProblem
+    from etalon_amd.batch_adjoint import BatchAdjointProblem
+    from tests.adjoint_sims import design_problem
+
+    a, _ = design_problem("single", steps=200)
+    with pytest.raises(ValueError, match="mode"):
+        BatchAdjointProblem([AdjointProblem(a, ["out"]), AdjointProblem(a, ["out"], mode="store")])
+    assert BatchAdjointProblem([AdjointProblem(a, ["out"], mode="store")]).mode == "store"
