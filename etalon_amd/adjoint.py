@@ -29,7 +29,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import time
-from dataclasses import dataclass, field, replace
+from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 import numpy as np
@@ -48,16 +48,6 @@ _COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 def checkpoint_interval(n_steps: int) -> int:
     """K ≈ √N, a multiple of 64 (the plan's graph chunk) and at least 64."""
     return max(64, 64 * round(math.sqrt(n_steps) / 64))
-
-
-def recompute_tune(tune):
-    """The tune of a forward plan that runs beside reverse steps in a co-plan: the
-    co-plan's joint kernel takes 8-row forward tiles, so an unset band is 8."""
-    from etalon_amd.hip.plan import Tune
-
-    if tune is None:
-        return Tune(band_rows=8)
-    return tune if tune.band_rows else replace(tune, band_rows=8)
 
 
 def _weights(sampling) -> np.ndarray:
@@ -364,7 +354,7 @@ class AdjointProblem:
             ses.bufs = self._adjoint_buffers(solver, hist)
             ses.rev = _Reverse("tile", ses.bufs.desc, self.tune)
             ses.rev.sources(comp, flat, table)
-            ses.rplan = FusedPlan(solver.state, recompute_tune(self.tune))
+            ses.rplan = FusedPlan(solver.state, self.tune)
             self._configure_recompute(ses.rplan, solver, 0, hist)
             ses.rplan.set_checkpoints([ses.checkpoints[name] for name in FIELDS],
                                       len(ses.starts))
@@ -436,7 +426,7 @@ class AdjointProblem:
 
         solver = tape.solver
         N = self.n_steps
-        plan = FusedPlan(solver.state, recompute_tune(self.tune))
+        plan = FusedPlan(solver.state, self.tune)
         try:
             self._configure_recompute(plan, solver, 0, hist)
             # Schedules: "joint" — the recompute of segment k - 1 and the reverse steps

@@ -29,7 +29,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from etalon_amd.adjoint import AdjointProblem, _Reverse, recompute_tune
+from etalon_amd.adjoint import AdjointProblem, _Reverse
 from etalon_amd.batch import BatchSolver
 from etalon_amd.hip import _lib
 from etalon_amd.hip.backend import zeros
@@ -247,7 +247,7 @@ class BatchAdjointProblem:
                 rev.sources(comp, flat, table, sim=k)
             if self.mode == "store":
                 return rev, None, co, bufs
-            plan = FusedPlan([s.state for s in solvers], recompute_tune(self.tune))
+            plan = FusedPlan([s.state for s in solvers], self.tune)
             for k, (p, s, h) in enumerate(zip(self.problems, solvers, hists)):
                 p._configure_recompute(plan, s, k, h)
                 ck = tape.checkpoints[k]

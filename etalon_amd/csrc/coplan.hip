@@ -88,7 +88,7 @@ int co_enqueue(EtlCoPlan* c, const FwdInfo& fi, int count_f, const AdjInfo& ai, 
         AdjSet<R> as{};
         if (do_f)
             fs = fwd_set<R>(static_cast<const FusedArgs<R>*>(fi.args), fi.tl, fi.counter,
-                            fb[i].kstep0, fi.parity ^ (nf & 1));
+                            fb[i].kstep0, fi.parity ^ (nf & 1), fi.band);
         if (do_a)
             as = adj_set<R>(static_cast<const AdjTileArgs<R>*>(ai.args), ai.tl, ai.step, ab[j].y,
                             ai.parity ^ (na & 1));
@@ -186,9 +186,9 @@ int etl_coplan_create(etl_plan_t f, etl_adjplan_t a, etl_coplan_t* out) {
     fwd_info(f, &fi);
     adj_info(a, &ai);
     ETL_REQUIRE(fi.precision == ai.precision, "etl_coplan_create: precisions differ");
-    ETL_REQUIRE(fi.vec == 1 && fi.band == 8,
-                "etl_coplan_create: the forward plan needs vec 1 and band 8 (got %d, %d)",
-                fi.vec, fi.band);
+    ETL_REQUIRE(fi.vec == 1 && (fi.band == 8 || (fi.band == 16 && fi.precision == ETL_F32)),
+                "etl_coplan_create: the forward plan needs vec 1 and band 8 (or 16 in float; "
+                "got %d, %d)", fi.vec, fi.band);
     EtlCoPlan* c = new EtlCoPlan();
     c->f = f;
     c->a = a;

@@ -127,7 +127,7 @@ typedef struct EtlTeDesc {
     const void *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
 } EtlTeDesc;
 
-/* Shape of the fused kernel (0 = library default: vec 1, band 8, temporal 4,
+/* Shape of the fused kernel (0 = library default: vec 1, band 8 or 16, temporal 4,
  * graph on, 64-step trace chunks, one wave per workgroup).  (vec, temporal,
  * band_rows) must be a compiled register-tile shape: float32 vec 1 with band 8 or
  * 16, or vec 2 with band 8, at temporal depth 1, 2 or 4; float64 vec 1 with band 8
@@ -298,7 +298,8 @@ int etl_adjplan_reset(etl_adjplan_t plan);
  * n_first (history rows from hist_row0), step block i of each in one kernel launch,
  * e.g. the recompute of checkpoint segment k - 1 beside the reverse steps of segment
  * k.  Bit-identical to etl_plan_advance_rows then etl_adjplan_run_rows.  The forward
- * plan needs vec 1 and band 8; both plans on one lane; the plans outlive the co-plan. */
+ * plan needs vec 1 and band 8 (or 16 in float); both plans on one lane; the plans
+ * outlive the co-plan. */
 typedef struct EtlCoPlan* etl_coplan_t;
 int etl_coplan_create(etl_plan_t fwd, etl_adjplan_t adj, etl_coplan_t* out);
 int etl_coplan_destroy(etl_coplan_t co);

@@ -110,3 +110,15 @@ def test_batch_store_mode_equals_checkpoint_mode(hip):
         scale = np.abs(gc[k]).max()
         assert np.abs(grads[k] - gc[k]).max() <= 1e-12 * scale, f"candidate {k}: vs checkpoint"
     loop.close()
+
+
+def test_batch_with_16_row_forward_tiles(hip):
+    """16-row forward tiles in the forward plan and in the recompute half of the
+    co-plan's joint launches: the same bits as each candidate alone at the default
+    8-row shape (tile shapes do not change the arithmetic)."""
+    problems, eps = candidates("single", steps=203)
+    print(f"{len(problems)} candidates, band 16", flush=True)
+    Js, grads = BatchAdjointProblem(problems, tune=Tune(band_rows=16), reuse=False).run(eps)
+    for k, (p, e) in enumerate(zip(problems, eps)):
+        J1, g1 = p.run(e)
+        assert Js[k] == J1 and np.array_equal(grads[k], g1), f"candidate {k}"
