@@ -27,6 +27,7 @@ NumPy adjoint in the tests and central differences pin this one.
 from __future__ import annotations
 
 import ctypes as C
+import functools
 import math
 import time
 from dataclasses import dataclass, field
@@ -38,8 +39,7 @@ from etalon_amd import models as et
 from etalon_amd.grid import build_yee_grid, pixel_node_map
 from etalon_amd.hip import _lib
 from etalon_amd.hip.backend import zeros
-from etalon_amd.sampling import dft_phasors
-from etalon_amd.solver import FDTDSolver
+from etalon_amd.solver import FDTDSolver, _phasor_table
 from etalon_amd.state import FIELDS, PROFILES, cb_arrays
 
 _COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
@@ -48,6 +48,16 @@ _COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 def checkpoint_interval(n_steps: int) -> int:
     """K ≈ √N, a multiple of 64 (the plan's graph chunk) and at least 64."""
     return max(64, 64 * round(math.sqrt(n_steps) / 64))
+
+
+@functools.lru_cache(maxsize=8)
+def _forward_phasors(n_steps: int, dt: float, magnetic: bool, freqs: bytes,
+                     dtype: str) -> np.ndarray:
+    """The forward pass's phasor table of steps [0, n_steps) in its precision, widened
+    to complex128; shared by the candidates of one Δt and frequency list; read-only."""
+    P = _phasor_table(0, n_steps, dt, magnetic, freqs, dtype).astype(np.complex128)
+    P.setflags(write=False)
+    return P
 
 
 def _weights(sampling) -> np.ndarray:
@@ -265,9 +275,12 @@ class AdjointProblem:
         for c in ("Ex", "Ey"):
             ii, jj, p = self.pixmap[c]
             solver.eps_r[c][ii, jj] = flat[p]
-        cb = cb_arrays(solver.grid, solver.eps_r)
-        solver.state.cb_ex.set(cb["cb_ex"].astype(solver.real))
-        solver.state.cb_ey.set(cb["cb_ey"].astype(solver.real))
+        # cb is elementwise in εr, and εr changed only on the design box's rows
+        i0, _, ni, _ = self.box
+        rows = {c: solver.eps_r[c][i0:i0 + ni] for c in ("Ex", "Ey")}
+        cb = cb_arrays(solver.grid, rows)
+        solver.state.cb_ex.rows(i0, i0 + ni).set(cb["cb_ex"].astype(solver.real))
+        solver.state.cb_ey.rows(i0, i0 + ni).set(cb["cb_ey"].astype(solver.real))
         solver.sim = self.with_permittivity(eps)
         solver.reset()
 
@@ -289,7 +302,6 @@ class AdjointProblem:
     def _sources(self, solver):
         """Distinct adjoint-source nodes (comp, flat) and the table (N, n) of dJ/dv."""
         N, dt = self.n_steps, self.grid.dt
-        steps = np.arange(N, dtype=np.float64)
         cols: dict = {}
         for mon in solver.monitors:
             smp = mon.sampling
@@ -298,9 +310,10 @@ class AdjointProblem:
             W = _weights(smp)
             acc = mon.acc.get().astype(np.complex128)        # (n_f, n_nodes)
             A = (acc @ W.T) @ W                              # (n_f, n_nodes)
-            t = (steps + 0.5) * dt if smp.is_magnetic else (steps + 1.0) * dt
-            P = dft_phasors(smp.frequencies, t, dt)          # (N, n_f), what the forward used
-            P = P.astype(solver.complex).astype(np.complex128)
+            # (N, n_f): the phasors the forward pass used, in its precision
+            P = _forward_phasors(N, dt, smp.is_magnetic,
+                                 np.asarray(smp.frequencies, np.float64).tobytes(),
+                                 np.dtype(solver.complex).str)
             S = 2.0 * np.real(P @ np.conj(A))                # (N, n_nodes)
             for k, flat in enumerate(smp.flat_index):
                 key = (_COMP[smp.component], int(flat))
@@ -392,15 +405,16 @@ class AdjointProblem:
 
     def _pixel_gradient(self, solver, bufs) -> np.ndarray:
         """∂J/∂εr = −(cb/εr) ∂J/∂cb at the nodes, summed per pixel."""
-        i0, j0, _, _ = self.box
+        i0, j0, ni, _ = self.box
         state = solver.state
         eps = solver.eps_r
-        cb = {"Ex": state.cb_ex.get(), "Ey": state.cb_ey.get()}
+        # the design box's rows of cb (every design node lies in them)
+        cb = {"Ex": state.cb_ex.rows(i0, i0 + ni).get(), "Ey": state.cb_ey.rows(i0, i0 + ni).get()}
         gcb = {"Ex": bufs.g_ex.get(), "Ey": bufs.g_ey.get()}
         out = np.zeros(self.shape[0] * self.shape[1])
         for c in ("Ex", "Ey"):
             ii, jj, p = self.pixmap[c]
-            g = -(cb[c][ii, jj].astype(np.float64) / eps[c][ii, jj]) * gcb[c][ii - i0, jj - j0]
+            g = -(cb[c][ii - i0, jj].astype(np.float64) / eps[c][ii, jj]) * gcb[c][ii - i0, jj - j0]
             np.add.at(out, p, g)
         return out.reshape(self.shape)
 

@@ -101,6 +101,14 @@ class DeviceArray:
             _lib.call("etl_memcpy_d2h", out.ctypes.data, C.c_void_p(self.ptr), self.nbytes)
         return out
 
+    def rows(self, r0: int, r1: int) -> "DeviceArray":
+        """A view of rows [r0, r1) along the first axis (contiguous in C order)."""
+        if self.ndim < 1 or not 0 <= r0 <= r1 <= self.shape[0]:
+            raise IndexError(f"rows [{r0}, {r1}) out of range for shape {self.shape}")
+        stride = int(np.prod(self.shape[1:], dtype=np.int64)) * self.dtype.itemsize
+        return DeviceArray((r1 - r0, *self.shape[1:]), self.dtype, self._buf,
+                           self._offset + r0 * stride)
+
     def set(self, host: Any) -> None:
         host = np.ascontiguousarray(host, dtype=self.dtype)
         if host.shape != self.shape:
