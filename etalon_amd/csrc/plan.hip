@@ -76,6 +76,15 @@ __global__ void psi_check_kernel(const R* __restrict__ psi, int rows, int cols, 
     if ((!in_strip || on_edge) && psi[k] != R(0)) flag[0] = 1;
 }
 
+// flag[1] = 1 if any of the n values is -0 (the tile kernels need fields without -0).
+template <typename R>
+__global__ void negzero_check_kernel(const R* __restrict__ f, int64_t n, int* __restrict__ flag) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const R v = f[k];
+    if (v == R(0) && signbit(v)) flag[1] = 1;
+}
+
 }  // namespace
 }  // namespace etl
 
@@ -820,10 +829,10 @@ int init_sim(EtlPlan* p, const EtlTeDesc* desc, EtlPlan::Sim& sim) {
     };
     int rc = f32 ? bounds(float(0), sim.strips) : bounds(double(0), sim.strips);
     if (rc) return rc;
-    // psi must be +0 wherever the fused kernel does not touch it.
+    // psi must be +0 wherever the fused kernel does not touch it, and no field -0.
     int* flag = nullptr;
     if ((rc = etl_malloc((void**)&flag, 256))) return rc;
-    rc = zero_async(flag, 4);
+    rc = zero_async(flag, 8);
     struct {
         int field, rows, cols, lo_r, hi_r, lo_c, hi_c, er, ec;
     } chk[4] = {
@@ -846,12 +855,28 @@ int init_sim(EtlPlan* p, const EtlTeDesc* desc, EtlPlan::Sim& sim) {
                                q.hi_r, q.lo_c, q.hi_c, q.er, q.ec, flag);
         if (hipGetLastError() != hipSuccess) rc = ETL_EHIP;
     }
-    int host_flag = 0;
-    if (rc == ETL_OK) rc = etl_memcpy_d2h(&host_flag, flag, 4);
+    const int64_t n_field[3] = {int64_t(nx) * (ny + 1), int64_t(nx + 1) * ny, int64_t(nx) * ny};
+    for (int k = 0; k < 3 && rc == ETL_OK; ++k) {
+        const dim3 grid(unsigned((n_field[k] + 255) / 256));
+        if (f32)
+            hipLaunchKernelGGL(negzero_check_kernel<float>, grid, dim3(256), 0, ctx().stream,
+                               (const float*)desc->field[k][0], n_field[k], flag);
+        else
+            hipLaunchKernelGGL(negzero_check_kernel<double>, grid, dim3(256), 0, ctx().stream,
+                               (const double*)desc->field[k][0], n_field[k], flag);
+        if (hipGetLastError() != hipSuccess) rc = ETL_EHIP;
+    }
+    int host_flag[2] = {0, 0};
+    if (rc == ETL_OK) rc = etl_memcpy_d2h(host_flag, flag, 8);
     etl_free(flag);
-    if (rc == ETL_OK && host_flag) {
+    if (rc == ETL_OK && host_flag[0]) {
         set_error("etl_plan_create: psi is non-zero outside the CPML strips; the fused "
                   "path needs psi = +0 there (use the per-half-step primitives)");
+        rc = ETL_ESTATE;
+    }
+    if (rc == ETL_OK && host_flag[1]) {
+        set_error("etl_plan_create: a field holds -0; the fused path needs fields without "
+                  "-0 (use the per-half-step primitives)");
         rc = ETL_ESTATE;
     }
     return rc;

@@ -29,9 +29,13 @@
 // Variants.  Strip tiles (S != 0) touch a CPML strip: they load, carry and store the
 // psi arrays of the strips they meet (S bit 0: the y strips, psi_hz_y and psi_ex_y;
 // bit 1: the x strips, psi_hz_x and psi_ey_x) where the 1D profiles differ from
-// (b, c, 1/kappa) = (1, 0, 1); elsewhere psi is +0 and the update reduces to
-// d*1 + (+0), exactly the reference's arithmetic (the +0 fixes the sign of a zero
-// difference).  EXTRA tiles also inject sources and trace monitors.
+// (b, c, 1/kappa) = (1, 0, 1).  Elsewhere psi is +0 and the reference's term
+// d*(1/kappa) + psi is d*1 + (+0), which equals d bit for bit unless d is -0; d is a
+// difference of two field values times a positive constant, and a - b is -0 only
+// when a is -0.  The plan checks at creation that no Ex, Ey or Hz value is -0, and
+// the reference's updates never make one from values that are not (x - y and x + y
+// are -0 only when x is; the psi terms they add are never -0), so the term is d.
+// EXTRA tiles also inject sources and trace monitors.
 #pragma once
 
 #include "common.h"
@@ -269,7 +273,6 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     ldv<R, V>(ey[RW], eyA, min(max(g0 + RW, 0), nx), ny, c, ny);
 
     const R db = a.db, idx = a.inv_dx, idy = a.inv_dy;
-    const R zero = R(0);
 
     // y-strip membership and profiles of this lane's columns
     bool mH[V], ysh[V], yse[V];
@@ -371,14 +374,14 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                     phy[SY ? l : 0][k] = py;
                     ty = d * ikyh[k] + py;
                 } else {
-                    ty = d + zero;
+                    ty = d;
                 }
                 if (SX && xh) {
                     const R pxv = bx * phx[SX ? l : 0][k] + cxp * e;
                     phx[SX ? l : 0][k] = pxv;
                     tx = e * ikx + pxv;
                 } else {
-                    tx = e + zero;
+                    tx = e;
                 }
                 hz[l][k] = hz[l][k] + db * (ty - tx);
             }
@@ -419,7 +422,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                         pey[SY ? l : 0][k] = p;
                         tt = d * ikye[k] + p;
                     } else {
-                        tt = d + zero;
+                        tt = d;
                     }
                     const R exn = ex[l][k] + cx[l][k] * tt;
                     ex[l][k] = upd ? exn : ex[l][k];
@@ -432,7 +435,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                         pex[SX ? l : 0][k] = p;
                         tt = d * ikxe + p;
                     } else {
-                        tt = d + zero;
+                        tt = d;
                     }
                     ey[l][k] = ey[l][k] - cy[l][k] * tt;
                 }

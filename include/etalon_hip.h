@@ -111,7 +111,11 @@ typedef struct EtlPlan* etl_plan_t;
 
 /* Every field and cb array of a plan must come from etl_malloc: the fused kernel
  * issues whole vector loads that may start a few elements before a row and end a
- * few past it, which etl_malloc's guard bands make safe. */
+ * few past it, which etl_malloc's guard bands make safe.  At creation the state must
+ * hold psi = +0 outside the CPML strips and no -0 in Ex, Ey, Hz (the reference's
+ * updates keep both true, so a state from zeros, from step() or from a plan always
+ * qualifies); otherwise etl_plan_create returns ETL_ESTATE and the caller steps with
+ * the per-half-step primitives. */
 typedef struct EtlTeDesc {
     int precision;
     int nx, ny;

@@ -124,6 +124,32 @@ def test_fused_and_step_paths_interleave(hip, precision, dtype):
 
 
 @pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_negative_zero_state_steps_instead(hip, precision, dtype):
+    """The tile kernels take d*1 + (+0) as d, which needs fields without -0: a state
+    holding -0 makes plan creation return ETL_ESTATE, and advance() steps with the
+    per-half-step kernels instead — bit for bit the oracle's result, -0 included."""
+    from etalon_amd.hip._lib import HipError
+    from etalon_amd.hip.plan import FusedPlan
+    from etalon_amd.solver import FDTDSolver
+
+    sim = small_sim(precision=precision)
+    gpu, cpu = FDTDSolver(sim), ref.Solver(sim)
+    for name in ("hz", "ey"):  # -0 on interior nodes away from the source
+        host = np.zeros(getattr(cpu.s, name).shape, dtype)
+        host[5:9, 20:26] = -0.0
+        setattr(cpu.s, name, host.copy())
+        getattr(gpu.state, name).set(host)
+    with pytest.raises(HipError) as err:
+        FusedPlan(gpu.state)
+    assert err.value.code == -3 and "-0" in str(err.value)
+    gpu.advance(40)
+    cpu.run(40)
+    _compare_solver(gpu, cpu)
+    zeros = gpu.state.hz.get()[5:9, 20:26]
+    print(f"{precision}: -0 kept at {int(np.signbit(zeros).sum())} Hz nodes", flush=True)
+
+
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
 def test_vacuum_pulse_2000_steps_bitwise(hip, precision, dtype):
     """184² grid (8 µm + 12-layer CPML), courant 0.5, 4 point monitors × 11 λ, 2000 steps."""
     from etalon_amd.solver import FDTDSolver
