@@ -346,123 +346,107 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
             }
         }
     }
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-        // ---- H half-step, window rows [t, RW - t) ----
-#pragma unroll
-        for (int l = t; l < RW - t; ++l) {
-            const int g = g0 + l;
-            bool xh = false;
-            R bx = R(1), cxp = R(0), ikx = R(1);
-            if constexpr (SX) {
-                xh = g < xh_lo || g >= xh_hi;
-                if (xh) {
-                    bx = lane_value(pbxh, l);
-                    cxp = lane_value(pcxh, l);
-                    ikx = lane_value(pikxh, l);
-                }
-            }
-            const R exr = from_next_lane(ex[l][0]);  // lane 63: outside the window
-#pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const R exn = (k + 1 < V) ? ex[l][k + 1] : exr;
-                const R d = (exn - ex[l][k]) * idy;
-                const R e = (ey[l + 1][k] - ey[l][k]) * idx;
-                R ty, tx;
-                if (SY && ysh[k]) {
-                    const R py = byh[k] * phy[SY ? l : 0][k] + cyh[k] * d;
-                    phy[SY ? l : 0][k] = py;
-                    ty = d * ikyh[k] + py;
-                } else {
-                    ty = d;
-                }
-                if (SX && xh) {
-                    const R pxv = bx * phx[SX ? l : 0][k] + cxp * e;
-                    phx[SX ? l : 0][k] = pxv;
-                    tx = e * ikx + pxv;
-                } else {
-                    tx = e;
-                }
-                hz[l][k] = hz[l][k] + db * (ty - tx);
-            }
-            if constexpr (EXTRA) {
-                if ((src.rows[2] >> l) & 1) inject_row<R, V, T>(src, t, 2, g, c, hz[l]);
+    const int lo = max(o0, 0), hi = o0 + wo;
+    // The H half-step of step t on window row l (rows [t, RW - t)), in place.
+    auto h_row = [&](int t, int l) __attribute__((always_inline)) {
+        const int g = g0 + l;
+        bool xh = false;
+        R bx = R(1), cxp = R(0), ikx = R(1);
+        if constexpr (SX) {
+            xh = g < xh_lo || g >= xh_hi;
+            if (xh) {
+                bx = lane_value(pbxh, l);
+                cxp = lane_value(pcxh, l);
+                ikx = lane_value(pikxh, l);
             }
         }
-        // ---- E half-step: Ex rows [t, RW - t), Ey rows [t + 1, RW - t) ----
+        const R exr = from_next_lane(ex[l][0]);  // lane 63: outside the window
 #pragma unroll
-        for (int l = t; l < RW - t; ++l) {
-            const int g = g0 + l;
-            const R hl = from_prev_lane(hz[l][V - 1]);  // lane 0: outside the window
-            // Without SX the window lies inside rows [1, nx - 1], without SY inside
-            // columns [1, ny - 1].
-            const bool ex_row = !SX || (g >= 0 && g <= nx - 1);
-            const bool ey_row = (l >= t + 1) && (!SX || (g >= 1 && g <= nx - 1));
-            bool xe = false;
-            R bxe = R(1), cxe = R(0), ikxe = R(1);
-            if constexpr (SX) {
-                xe = ey_row && (g < xe_lo || g >= xe_hi);
-                if (xe) {
-                    bxe = lane_value(pbxe, l);
-                    cxe = lane_value(pcxe, l);
-                    ikxe = lane_value(pikxe, l);
-                }
+        for (int k = 0; k < V; ++k) {
+            const R exn = (k + 1 < V) ? ex[l][k + 1] : exr;
+            const R d = (exn - ex[l][k]) * idy;
+            const R e = (ey[l + 1][k] - ey[l][k]) * idx;
+            R ty, tx;
+            if (SY && ysh[k]) {
+                const R py = byh[k] * phy[SY ? l : 0][k] + cyh[k] * d;
+                phy[SY ? l : 0][k] = py;
+                ty = d * ikyh[k] + py;
+            } else {
+                ty = d;
             }
+            if (SX && xh) {
+                const R pxv = bx * phx[SX ? l : 0][k] + cxp * e;
+                phx[SX ? l : 0][k] = pxv;
+                tx = e * ikx + pxv;
+            } else {
+                tx = e;
+            }
+            hz[l][k] = hz[l][k] + db * (ty - tx);
+        }
+        if constexpr (EXTRA) {
+            if ((src.rows[2] >> l) & 1) inject_row<R, V, T>(src, t, 2, g, c, hz[l]);
+        }
+    };
+    // The E half-step of step t on window row l: Ex rows [t, RW - t), Ey rows
+    // [t + 1, RW - t), in place.
+    auto e_row = [&](int t, int l) __attribute__((always_inline)) {
+        const int g = g0 + l;
+        const R hl = from_prev_lane(hz[l][V - 1]);  // lane 0: outside the window
+        // Without SX the window lies inside rows [1, nx - 1], without SY inside
+        // columns [1, ny - 1].
+        const bool ex_row = !SX || (g >= 0 && g <= nx - 1);
+        const bool ey_row = (l >= t + 1) && (!SX || (g >= 1 && g <= nx - 1));
+        bool xe = false;
+        R bxe = R(1), cxe = R(0), ikxe = R(1);
+        if constexpr (SX) {
+            xe = ey_row && (g < xe_lo || g >= xe_hi);
+            if (xe) {
+                bxe = lane_value(pbxe, l);
+                cxe = lane_value(pcxe, l);
+                ikxe = lane_value(pikxe, l);
+            }
+        }
 #pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const int j = c + k;
-                // Ex(g, j); j = 0, ny stay unchanged (a select, not a branch)
-                if (ex_row) {
-                    const bool upd = !SY || (j >= 1 && j <= ny - 1);
-                    const R hleft = (k == 0) ? hl : hz[l][k - 1];
-                    const R d = (hz[l][k] - hleft) * idy;
-                    R tt;
-                    if (SY && yse[k]) {
-                        const R p = bye[k] * pey[SY ? l : 0][k] + cye[k] * d;
-                        pey[SY ? l : 0][k] = p;
-                        tt = d * ikye[k] + p;
-                    } else {
-                        tt = d;
-                    }
-                    const R exn = ex[l][k] + cx[l][k] * tt;
-                    ex[l][k] = upd ? exn : ex[l][k];
+        for (int k = 0; k < V; ++k) {
+            const int j = c + k;
+            // Ex(g, j); j = 0, ny stay unchanged (a select, not a branch)
+            if (ex_row) {
+                const bool upd = !SY || (j >= 1 && j <= ny - 1);
+                const R hleft = (k == 0) ? hl : hz[l][k - 1];
+                const R d = (hz[l][k] - hleft) * idy;
+                R tt;
+                if (SY && yse[k]) {
+                    const R p = bye[k] * pey[SY ? l : 0][k] + cye[k] * d;
+                    pey[SY ? l : 0][k] = p;
+                    tt = d * ikye[k] + p;
+                } else {
+                    tt = d;
                 }
-                if (ey_row) {  // Ey(g, j), g in [1, nx - 1]; rows 0 and nx stay unchanged
-                    const R d = (hz[l][k] - hz[l > 0 ? l - 1 : 0][k]) * idx;
-                    R tt;
-                    if (SX && xe) {
-                        const R p = bxe * pex[SX ? l : 0][k] + cxe * d;
-                        pex[SX ? l : 0][k] = p;
-                        tt = d * ikxe + p;
-                    } else {
-                        tt = d;
-                    }
-                    ey[l][k] = ey[l][k] - cy[l][k] * tt;
-                }
+                const R exn = ex[l][k] + cx[l][k] * tt;
+                ex[l][k] = upd ? exn : ex[l][k];
             }
-            if constexpr (EXTRA) {
-                if ((src.rows[0] >> l) & 1) inject_row<R, V, T>(src, t, 0, g, c, ex[l]);
-                if ((src.rows[1] >> l) & 1) inject_row<R, V, T>(src, t, 1, g, c, ey[l]);
+            if (ey_row) {  // Ey(g, j), g in [1, nx - 1]; rows 0 and nx stay unchanged
+                const R d = (hz[l][k] - hz[l > 0 ? l - 1 : 0][k]) * idx;
+                R tt;
+                if (SX && xe) {
+                    const R p = bxe * pex[SX ? l : 0][k] + cxe * d;
+                    pex[SX ? l : 0][k] = p;
+                    tt = d * ikxe + p;
+                } else {
+                    tt = d;
+                }
+                ey[l][k] = ey[l][k] - cy[l][k] * tt;
             }
         }
         if constexpr (EXTRA) {
-            G* trow = trace_base + int64_t(kstep0 + t) * n_slots;
-            if (mon_lo[2] < mon_hi[2])
-                trace_tile<R, V, RW, T, BR>(ent[2], mon_lo[2], mon_hi[2], lds, hz, i0, cs, trow);
-            if (mon_lo[0] < mon_hi[0])
-                trace_tile<R, V, RW, T, BR>(ent[0], mon_lo[0], mon_hi[0], lds, ex, i0, cs, trow);
-            if (mon_lo[1] < mon_hi[1])
-                trace_tile<R, V, RW + 1, T, BR>(ent[1], mon_lo[1], mon_hi[1], lds, ey, i0, cs,
-                                                trow);
+            if ((src.rows[0] >> l) & 1) inject_row<R, V, T>(src, t, 0, g, c, ex[l]);
+            if ((src.rows[1] >> l) & 1) inject_row<R, V, T>(src, t, 1, g, c, ey[l]);
         }
-    }
-
-    // ---- store the owned rows [i0, i0 + BR) and columns [o0, o0 + WO) ----
-    const int lo = max(o0, 0), hi = o0 + wo;
-#pragma unroll
-    for (int l = T; l < T + BR; ++l) {
+    };
+    // Store owned row l (window rows [T, T + BR), columns [o0, o0 + WO)).
+    auto store_owned = [&](int l) __attribute__((always_inline)) {
         const int g = g0 + l;
-        if (g >= nx) break;
+        if (g >= nx) return;
         store_row<R, V>(hzB + int64_t(g) * ny, hz[l], c, lo, min(hi, ny));
         store_row<R, V>(exB + int64_t(g) * px, ex[l], c, lo, min(hi, px));
         store_row<R, V>(eyB + int64_t(g) * ny, ey[l], c, lo, min(hi, ny));
@@ -486,7 +470,46 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
             if (g >= 1 && (g < xe_lo || g >= xe_hi))
                 store_masked<R, V>(peyB + int64_t(g) * ny, pex[SX ? l : 0], c, mo);
         }
+    };
+    if constexpr (EXTRA) {
+        // Step-major: the monitors' trace rows are written per step.
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+#pragma unroll
+            for (int l = t; l < RW - t; ++l) h_row(t, l);
+#pragma unroll
+            for (int l = t; l < RW - t; ++l) e_row(t, l);
+            G* trow = trace_base + int64_t(kstep0 + t) * n_slots;
+            if (mon_lo[2] < mon_hi[2])
+                trace_tile<R, V, RW, T, BR>(ent[2], mon_lo[2], mon_hi[2], lds, hz, i0, cs, trow);
+            if (mon_lo[0] < mon_hi[0])
+                trace_tile<R, V, RW, T, BR>(ent[0], mon_lo[0], mon_hi[0], lds, ex, i0, cs, trow);
+            if (mon_lo[1] < mon_hi[1])
+                trace_tile<R, V, RW + 1, T, BR>(ent[1], mon_lo[1], mon_hi[1], lds, ey, i0, cs,
+                                                trow);
+        }
+#pragma unroll
+        for (int l = T; l < T + BR; ++l) store_owned(l);
+    } else {
+        // Time-skewed: at front f, step t advances window row f - t.  Row l + 1 is at
+        // step t - 1 when step t reads its Ey, row l - 1 already at step t when the E
+        // half-step reads its Hz, and row l's own values at step t - 1 — the values the
+        // step-major order reads.  Front f needs window rows up to f + 1 only, so the
+        // first rows compute while the later ones are still loading, and each owned row
+        // is stored as soon as its last step is done.
+#pragma unroll
+        for (int f = 0; f < RW; ++f) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int l = f - t;
+                if (l < t || l >= RW - t) continue;
+                h_row(t, l);
+                e_row(t, l);
+                if (t == T - 1 && l >= T && l < T + BR) store_owned(l);
+            }
+        }
     }
+
     // the last band copies Ey row nx (outer tangential E)
     if (i0 + BR >= nx) {
         R eyl[V];
