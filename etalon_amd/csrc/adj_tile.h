@@ -238,8 +238,33 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
                 }
     }
 
+    // Per step: ld, le of the rows A has done for B (shared by the steps: a row's
+    // values are read by B of the same step before A of the next step rewrites them),
+    // and Lb of the row A does next.
+    R ld[RW][V], le[RW][V], lbc[T][V];
+    // Lb of window Ey row l.
+    auto lb_row = [&](int l, R (&out)[V]) __attribute__((always_inline)) {
+        const int g = g0 + l;
+        const bool row_ok = g >= 1 && g <= nx - 1;
+        bool xe = false;
+        R ikx = R(1), cxv = R(0);
+        if constexpr (SX) {
+            xe = row_ok && (g < xe_lo || g >= xe_hi);
+            if (xe) {
+                ikx = lane_value(pikxe, l);
+                cxv = lane_value(pcxe, l);
+            }
+        }
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
+        for (int k = 0; k < V; ++k) {
+            const R lw = -cy[l][k] * ey[l][k];
+            R v = lw;
+            if (SX && xe) v = ikx * lw + cxv * (pex[SX ? l : 0][k] + lw);
+            out[k] = row_ok ? v : R(0);
+        }
+    };
+    // The adjoint sources of step t (EXTRA tiles).
+    auto add_sources = [&](int t) __attribute__((always_inline)) {
         const int r = rtop - t;  // table row n_first + r, history row hrow0 + r of this step
         if constexpr (EXTRA) {
 #pragma unroll
@@ -261,126 +286,100 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
                 else add_run<R, V, RW>(hz, lr, c, ru.y, ru.z, vals);
             }
         }
-        // ---- A: Hz rows [t, RW - t); Ld, Le of those rows for B
-        R ld[RW][V], le[RW][V];
-        auto lb_row = [&](int l, R (&out)[V]) {  // Lb of window Ey row l
-            const int g = g0 + l;
-            const bool row_ok = g >= 1 && g <= nx - 1;
-            bool xe = false;
-            R ikx = R(1), cxv = R(0);
-            if constexpr (SX) {
-                xe = row_ok && (g < xe_lo || g >= xe_hi);
-                if (xe) {
-                    ikx = lane_value(pikxe, l);
-                    cxv = lane_value(pcxe, l);
-                }
-            }
+    };
+    // A of step t on window row l (Hz rows [t, RW - t)): lambda_Hz, and Ld, Le for B.
+    auto a_row = [&](int t, int l) __attribute__((always_inline)) {
+        R la[V], lb_next[V];
 #pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const R lw = -cy[l][k] * ey[l][k];
-                R v = lw;
-                if (SX && xe) v = ikx * lw + cxv * (pex[SX ? l : 0][k] + lw);
-                out[k] = row_ok ? v : R(0);
-            }
-        };
-        R lb_cur[V];
-        lb_row(t, lb_cur);
-#pragma unroll
-        for (int l = t; l < RW - t; ++l) {
-            R la[V], lb_next[V];
-#pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const R lu = cx[l][k] * ex[l][k];
-                R v = lu;
-                if (SY && yse[k]) v = ikye[k] * lu + cye[k] * (pey[SY ? l : 0][k] + lu);
-                la[k] = mEx[k] ? v : R(0);
-            }
-            lb_row(l + 1, lb_next);
-            const R la_r = from_next_lane(la[0]);
-            const int g = g0 + l;
-            bool xh = false;
-            R bx = R(1), cxh = R(0), ikx = R(1);
-            if constexpr (SX) {
-                xh = g < xh_lo || g >= xh_hi;
-                if (xh) {
-                    bx = lane_value(pbxh, l);
-                    cxh = lane_value(pcxh, l);
-                    ikx = lane_value(pikxh, l);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const R la_n = (k + 1 < V) ? la[k + 1] : la_r;
-                const R lh = hz[l][k] + idy * (la[k] - la_n) + idx * (lb_cur[k] - lb_next[k]);
-                hz[l][k] = lh;
-                if (SY && ysh[k]) {
-                    const R lpy = phy[SY ? l : 0][k] + db * lh;
-                    ld[l][k] = db * ikyh[k] * lh + cyh[k] * lpy;
-                    phy[SY ? l : 0][k] = byh[k] * lpy;
-                } else {
-                    ld[l][k] = db * lh;
-                }
-                if (SX && xh) {
-                    const R lpx = phx[SX ? l : 0][k] - db * lh;
-                    le[l][k] = -db * ikx * lh + cxh * lpx;
-                    phx[SX ? l : 0][k] = bx * lpx;
-                } else {
-                    le[l][k] = -db * lh;
-                }
-                lb_cur[k] = lb_next[k];
+        for (int k = 0; k < V; ++k) {
+            const R lu = cx[l][k] * ex[l][k];
+            R v = lu;
+            if (SY && yse[k]) v = ikye[k] * lu + cye[k] * (pey[SY ? l : 0][k] + lu);
+            la[k] = mEx[k] ? v : R(0);
+        }
+        lb_row(l + 1, lb_next);
+        const R la_r = from_next_lane(la[0]);
+        const int g = g0 + l;
+        bool xh = false;
+        R bx = R(1), cxh = R(0), ikx = R(1);
+        if constexpr (SX) {
+            xh = g < xh_lo || g >= xh_hi;
+            if (xh) {
+                bx = lane_value(pbxh, l);
+                cxh = lane_value(pcxh, l);
+                ikx = lane_value(pikxh, l);
             }
         }
-        // ---- B: Ex rows [t, RW - t), Ey rows [t + 1, RW - t)
 #pragma unroll
-        for (int l = t; l < RW - t; ++l) {
-            const int g = g0 + l;
-            const R ld_l = from_prev_lane(ld[l][V - 1]);
-            const bool ey_row = l >= t + 1;
-            const bool xe = SX && ey_row && g >= 1 && g <= nx - 1 && (g < xe_lo || g >= xe_hi);
-            R bxe = R(1);
-            if (SX && xe) bxe = lane_value(pbxe, l);
-            const bool owned = EXTRA && l >= T && l < T + BR;
-            // history of this row (m) and the one above (m - 1), and column j - 1
-            const int m = owned ? l - T + 1 : 1;
-            R h_left = R(0);
-            if (owned && boxed) h_left = from_prev_lane(hv[EXTRA ? t : 0][EXTRA ? m : 0][V - 1]);
+        for (int k = 0; k < V; ++k) {
+            const R la_n = (k + 1 < V) ? la[k + 1] : la_r;
+            const R lh = hz[l][k] + idy * (la[k] - la_n) + idx * (lbc[t][k] - lb_next[k]);
+            hz[l][k] = lh;
+            if (SY && ysh[k]) {
+                const R lpy = phy[SY ? l : 0][k] + db * lh;
+                ld[l][k] = db * ikyh[k] * lh + cyh[k] * lpy;
+                phy[SY ? l : 0][k] = byh[k] * lpy;
+            } else {
+                ld[l][k] = db * lh;
+            }
+            if (SX && xh) {
+                const R lpx = phx[SX ? l : 0][k] - db * lh;
+                le[l][k] = -db * ikx * lh + cxh * lpx;
+                phx[SX ? l : 0][k] = bx * lpx;
+            } else {
+                le[l][k] = -db * lh;
+            }
+            lbc[t][k] = lb_next[k];
+        }
+    };
+    // B of step t on window row l: Ex rows [t, RW - t), Ey rows [t + 1, RW - t).
+    auto b_row = [&](int t, int l) __attribute__((always_inline)) {
+        const int g = g0 + l;
+        const R ld_l = from_prev_lane(ld[l][V - 1]);
+        const bool ey_row = l >= t + 1;
+        const bool xe = SX && ey_row && g >= 1 && g <= nx - 1 && (g < xe_lo || g >= xe_hi);
+        R bxe = R(1);
+        if (SX && xe) bxe = lane_value(pbxe, l);
+        const bool owned = EXTRA && l >= T && l < T + BR;
+        // history of this row (m) and the one above (m - 1), and column j - 1
+        const int m = owned ? l - T + 1 : 1;
+        R h_left = R(0);
+        if (owned && boxed) h_left = from_prev_lane(hv[EXTRA ? t : 0][EXTRA ? m : 0][V - 1]);
 #pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const int j = c + k;
-                const R lex = ex[l][k];
-                if (SY && yse[k]) pey[SY ? l : 0][k] = bye[k] * (pey[SY ? l : 0][k] + cx[l][k] * lex);
-                if (owned && boxed && mEx[k] && g >= a.bi0 && g < a.bi0 + a.bni && j >= a.bj0 &&
-                    j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
-                    const R h0 = (k > 0) ? hv[EXTRA ? t : 0][EXTRA ? m : 0][k > 0 ? k - 1 : 0]
-                                         : h_left;
-                    const double u =
-                        (double(hv[EXTRA ? t : 0][EXTRA ? m : 0][k]) - double(h0)) * double(idy);
-                    gxa[owned ? l - T : 0][k] += double(lex) * u;
+        for (int k = 0; k < V; ++k) {
+            const int j = c + k;
+            const R lex = ex[l][k];
+            if (SY && yse[k]) pey[SY ? l : 0][k] = bye[k] * (pey[SY ? l : 0][k] + cx[l][k] * lex);
+            if (owned && boxed && mEx[k] && g >= a.bi0 && g < a.bi0 + a.bni && j >= a.bj0 &&
+                j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
+                const R h0 = (k > 0) ? hv[EXTRA ? t : 0][EXTRA ? m : 0][k > 0 ? k - 1 : 0]
+                                     : h_left;
+                const double u =
+                    (double(hv[EXTRA ? t : 0][EXTRA ? m : 0][k]) - double(h0)) * double(idy);
+                gxa[owned ? l - T : 0][k] += double(lex) * u;
+            }
+            const R ldp = (k > 0) ? ld[l][k - 1] : ld_l;
+            ex[l][k] = lex + idy * (ldp - ld[l][k]);
+            if (ey_row) {
+                const R ley = ey[l][k];
+                if (SX && xe) pex[SX ? l : 0][k] = bxe * (pex[SX ? l : 0][k] - cy[l][k] * ley);
+                if (owned && boxed && g >= a.bi0 && g < a.bi0 + a.bni && g >= 1 && g <= nx - 1 &&
+                    j >= a.bj0 && j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
+                    const double w = (double(hv[EXTRA ? t : 0][EXTRA ? m : 0][k]) -
+                                      double(hv[EXTRA ? t : 0][EXTRA ? m - 1 : 0][k])) *
+                                     double(idx);
+                    gya[owned ? l - T : 0][k] -= double(ley) * w;
                 }
-                const R ldp = (k > 0) ? ld[l][k - 1] : ld_l;
-                ex[l][k] = lex + idy * (ldp - ld[l][k]);
-                if (ey_row) {
-                    const R ley = ey[l][k];
-                    if (SX && xe) pex[SX ? l : 0][k] = bxe * (pex[SX ? l : 0][k] - cy[l][k] * ley);
-                    if (owned && boxed && g >= a.bi0 && g < a.bi0 + a.bni && g >= 1 && g <= nx - 1 &&
-                        j >= a.bj0 && j < a.bj0 + a.bnj && j >= o0 && j < o0 + wo) {
-                        const double w = (double(hv[EXTRA ? t : 0][EXTRA ? m : 0][k]) -
-                                          double(hv[EXTRA ? t : 0][EXTRA ? m - 1 : 0][k])) *
-                                         double(idx);
-                        gya[owned ? l - T : 0][k] -= double(ley) * w;
-                    }
-                    ey[l][k] = ley + idx * (le[l - 1][k] - le[l][k]);
-                }
+                ey[l][k] = ley + idx * (le[l - 1][k] - le[l][k]);
             }
         }
-    }
-
+    };
     // ---- store the owned rows [i0, i0 + BR) and columns [o0, o0 + WO)
     const int lo = max(o0, 0), hi = o0 + wo;
-#pragma unroll
-    for (int l = T; l < T + BR; ++l) {
+    // Store owned row l (window rows [T, T + BR)).
+    auto store_owned = [&](int l) __attribute__((always_inline)) {
         const int g = g0 + l;
-        if (g >= nx) break;
+        if (g >= nx) return;
         store_row<R, V>(hzB + int64_t(g) * ny, hz[l], c, lo, min(hi, ny));
         store_row<R, V>(exB + int64_t(g) * px, ex[l], c, lo, min(hi, px));
         store_row<R, V>(eyB + int64_t(g) * ny, ey[l], c, lo, min(hi, ny));
@@ -416,6 +415,35 @@ __device__ __forceinline__ void adj_tile_body(const AdjTileArgs<R>* __restrict__
                         to_global(a.gy)[q] += gya[l - T][k];
                     }
                 }
+            }
+        }
+    };
+    if constexpr (EXTRA) {
+        // Step-major: sources and the forward history enter per step.
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            add_sources(t);
+            lb_row(t, lbc[t]);
+#pragma unroll
+            for (int l = t; l < RW - t; ++l) a_row(t, l);
+#pragma unroll
+            for (int l = t; l < RW - t; ++l) b_row(t, l);
+        }
+#pragma unroll
+        for (int l = T; l < T + BR; ++l) store_owned(l);
+    } else {
+        // Time-skewed, as the forward tiles: at front f, step t does row f - t (A then
+        // B), each owned row stored when its last step is done.
+#pragma unroll
+        for (int f = 0; f < RW; ++f) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int l = f - t;
+                if (l < t || l >= RW - t) continue;
+                if (l == t) lb_row(t, lbc[t]);
+                a_row(t, l);
+                b_row(t, l);
+                if (t == T - 1 && l >= T && l < T + BR) store_owned(l);
             }
         }
     }

@@ -42,9 +42,11 @@ struct CkEntry {
 // word moves otherwise and for the tail.
 __global__ void ckpt_copy_kernel(const CkEntry* __restrict__ tab, int seg, int parity,
                                  int restore) {
-    const CkEntry e = tab[blockIdx.y];
+    const CkEntry& e = tab[blockIdx.y];
+    // a select, not e.f[parity]: a dynamically indexed local copy of the entry would
+    // be promoted to LDS (32 KB per workgroup)
+    uint32_t* f = parity ? e.f[1] : e.f[0];
     uint32_t* ck = e.ck + seg * e.words;
-    uint32_t* f = e.f[parity];
     const uint32_t* src = restore ? ck : f;
     uint32_t* dst = restore ? f : ck;
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
