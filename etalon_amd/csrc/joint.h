@@ -195,19 +195,24 @@ __device__ __forceinline__ void joint_body(const FwdSet<R>& f, const AdjSet<R>& 
     }
 }
 
-// kJointWaves: the waves per SIMD the register allocation of a float joint kernel must
-// allow.  Its heaviest tiles (strip, source/monitor) need more registers than the
-// interior tiles; capped, they spill a little, and the interior tiles, which are most
-// of the work and bandwidth-bound, keep enough waves in flight.  Double tiles need
-// twice the registers: capped they would spill in every tile, so they are not.
+// kJointWaves, kJointWavesF64: the waves per SIMD the register allocation of a float
+// (double) joint kernel must allow.  Its heaviest tiles (strip, source/monitor) need
+// more registers than the interior tiles; capped, they spill a little, and the
+// interior tiles, which are most of the work and bandwidth-bound, keep enough waves in
+// flight.  Measured at 2024²: float 3 (the best of 2, 3, 4); double 2 (265 Gcell/s,
+// against 217 uncapped — one wave per SIMD — and 179 at 3, where every tile spills).
 #ifndef ETL_JOINT_WAVES
 #define ETL_JOINT_WAVES 3
 #endif
+#ifndef ETL_JOINT_WAVES_F64
+#define ETL_JOINT_WAVES_F64 2
+#endif
 constexpr int kJointWaves = ETL_JOINT_WAVES;
+constexpr int kJointWavesF64 = ETL_JOINT_WAVES_F64;
 
 template <typename R, int TF, int TA, int PART, int JB>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? kJointWaves : 1)))
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? kJointWaves : kJointWavesF64)))
 void joint_kernel(FwdSet<R> f, AdjSet<R> a) {
     joint_body<R, TF, TA, PART, JB>(f, a);
 }
