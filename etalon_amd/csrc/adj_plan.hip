@@ -485,7 +485,7 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
         if (rc == ETL_OK)
             rc = precision == ETL_F32 ? adj_strips<float>(d, s.strips) : adj_strips<double>(d, s.strips);
     }
-    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = precision == ETL_F32 ? 1 : 3;
+    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = 1;  // every tile kind in one kernel
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
