@@ -7,8 +7,9 @@ reference has no adjoint); it is pinned by the transpose identity
 finite differences of the oracle's own forward run, and it in turn pins the HIP
 adjoint.
 
-Forward step n (te_numpy.update_h / update_e), linear in the
-state x = (Ex, Ey, Hz, ψ_Ex,y, ψ_Ey,x, ψ_Hz,x, ψ_Hz,y):
+Forward step n (te_numpy.update_h / update_e after
+etalon/fdtd/backends/numpy_backend.py:41-61), linear in the state
+x = (Ex, Ey, Hz, ψ_Ex,y, ψ_Ey,x, ψ_Hz,x, ψ_Hz,y):
 
     H:  d = Dy Ex,  e = Dx Ey,  ψy' = by ψy + cy d,  ψx' = bx ψx + cx e,
         Hz' = Hz + db (d iky + ψy' − e ikx − ψx')
