@@ -30,7 +30,7 @@ import ctypes as C
 import functools
 import math
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import Optional, Sequence
 
 import numpy as np
@@ -58,6 +58,18 @@ def _forward_phasors(n_steps: int, dt: float, magnetic: bool, freqs: bytes,
     P = _phasor_table(0, n_steps, dt, magnetic, freqs, dtype).astype(np.complex128)
     P.setflags(write=False)
     return P
+
+
+def adjoint_tune(tune):
+    """The tune of an adjoint problem's recompute and reverse plans: an unset temporal
+    depth is 4, the depth of the reverse kernels, so that a co-plan pairs the
+    recompute's step blocks with the reverse steps' (large float forward plans would
+    default to depth 6)."""
+    from etalon_amd.hip.plan import Tune
+
+    if tune is None:
+        return Tune(temporal=4)
+    return tune if tune.temporal else replace(tune, temporal=4)
 
 
 def _weights(sampling) -> np.ndarray:
@@ -152,7 +164,10 @@ class AdjointProblem:
         self.sim = sim
         self.objective = tuple(objective)
         self.mode = mode
-        self.tune = tune
+        self.tune = tune  # the forward pass
+        # the recompute and reverse plans: depth 4 unless set, so that a co-plan pairs
+        # their step blocks
+        self.rtune = adjoint_tune(tune)
         grid = build_yee_grid(sim)
         self.grid = grid
         self.n_steps = grid.n_steps(sim.run_time)
@@ -339,7 +354,7 @@ class AdjointProblem:
             rows = 2 * K if self.schedule != "sequential" else K
             hist = zeros((rows, self._box_hz_nodes().size), solver.real)
         bufs = self._adjoint_buffers(solver, hist)
-        rev = _Reverse(self.kernel, bufs.desc, self.tune)
+        rev = _Reverse(self.kernel, bufs.desc, self.rtune)
         try:
             comp, flat, table = self._sources(solver)
             rev.sources(comp, flat, table)
@@ -365,9 +380,9 @@ class AdjointProblem:
         if ses.rev is None:
             hist = zeros((2 * K, self._box_hz_nodes().size), solver.real)
             ses.bufs = self._adjoint_buffers(solver, hist)
-            ses.rev = _Reverse("tile", ses.bufs.desc, self.tune)
+            ses.rev = _Reverse("tile", ses.bufs.desc, self.rtune)
             ses.rev.sources(comp, flat, table)
-            ses.rplan = FusedPlan(solver.state, self.tune)
+            ses.rplan = FusedPlan(solver.state, self.rtune)
             self._configure_recompute(ses.rplan, solver, 0, hist)
             ses.rplan.set_checkpoints([ses.checkpoints[name] for name in FIELDS],
                                       len(ses.starts))
@@ -440,7 +455,7 @@ class AdjointProblem:
 
         solver = tape.solver
         N = self.n_steps
-        plan = FusedPlan(solver.state, self.tune)
+        plan = FusedPlan(solver.state, self.rtune)
         try:
             self._configure_recompute(plan, solver, 0, hist)
             # Schedules: "joint" — the recompute of segment k - 1 and the reverse steps

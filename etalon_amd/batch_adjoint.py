@@ -29,7 +29,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from etalon_amd.adjoint import AdjointProblem, _Reverse
+from etalon_amd.adjoint import AdjointProblem, _Reverse, adjoint_tune
 from etalon_amd.batch import BatchSolver
 from etalon_amd.hip import _lib
 from etalon_amd.hip.backend import zeros
@@ -94,7 +94,8 @@ class BatchAdjointProblem:
         # "store": the forward pass keeps every step's Hz on each design box and the
         # reverse steps read it (no recompute; N x box nodes values per candidate)
         self.mode = p0.mode
-        self.tune = tune
+        self.tune = tune  # the forward pass
+        self.rtune = adjoint_tune(tune)  # the recompute and reverse plans
         self.n_steps, self.K = p0.n_steps, p0.K
         # a design loop's gradients keep the batch, plans, graphs and buffers of the
         # first one: new permittivities only replace each candidate's cb_ex and cb_ey.
@@ -238,7 +239,7 @@ class BatchAdjointProblem:
             hists = [zeros((2 * K, p._box_hz_nodes().size), s.real)
                      for p, s in zip(self.problems, solvers)]
         bufs = [p._adjoint_buffers(s, h) for p, s, h in zip(self.problems, solvers, hists)]
-        rev = _Reverse("tile", [b.desc for b in bufs], self.tune)
+        rev = _Reverse("tile", [b.desc for b in bufs], self.rtune)
         plan: Optional[FusedPlan] = None
         co = C.c_void_p()
         try:
@@ -247,7 +248,7 @@ class BatchAdjointProblem:
                 rev.sources(comp, flat, table, sim=k)
             if self.mode == "store":
                 return rev, None, co, bufs
-            plan = FusedPlan([s.state for s in solvers], self.tune)
+            plan = FusedPlan([s.state for s in solvers], self.rtune)
             for k, (p, s, h) in enumerate(zip(self.problems, solvers, hists)):
                 p._configure_recompute(plan, s, k, h)
                 ck = tape.checkpoints[k]

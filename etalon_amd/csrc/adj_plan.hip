@@ -286,7 +286,7 @@ int enqueue_adj(EtlAdjPlan* p, int count, int parity, int* launches) {
     const int T = p->tune.temporal;
     int rtop = count - 1, n = 0;
     while (rtop >= 0) {
-        const int depth = (rtop + 1 >= T) ? T : 1;
+        const int depth = next_depth(rtop + 1, T, p->precision, 1, p->tune.band_rows);
         if (p->tune.launch == 1) {  // every tile kind in one kernel
             const AdjSet<R> as = adj_set<R>(args, tl, p->step, rtop, parity ^ (n & 1));
             if (int rc = launch_joint<R>(nullptr, 0, &as, depth, s)) return rc;
@@ -410,6 +410,7 @@ void adj_info(const EtlAdjPlan* p, AdjInfo* out) {
     out->precision = p->precision;
     out->temporal = p->tune.temporal;
     out->launch = p->tune.launch;
+    out->band = p->tune.band_rows;
     out->lane = p->lane;
     out->args = p->args_dev;
     out->tl = make_tile_list(t, p->n_interior, p->n_strip, p->n_strip_y, p->n_strip_x, p->n_extra,

@@ -20,7 +20,7 @@ struct FwdInfo {
 };
 
 struct AdjInfo {
-    int precision, temporal, launch, lane;
+    int precision, temporal, launch, band, lane;
     const void* args;  // AdjTileArgs<R>[n_sims] on the device
     TileList tl;
     int* step;

@@ -63,7 +63,7 @@ int co_enqueue(EtlCoPlan* c, const FwdInfo& fi, int count_f, const AdjInfo& ai, 
     for (int done = 0; done < count_f;) {
         const int blk = std::min(fi.trace_chunk, count_f - done);
         for (int k = 0; k < blk;) {
-            const int d = (blk - k >= fi.temporal) ? fi.temporal : 1;
+            const int d = next_depth(blk - k, fi.temporal, fi.precision, fi.vec, fi.band);
             fb.push_back({d, k, 0});
             k += d;
         }
@@ -72,7 +72,7 @@ int co_enqueue(EtlCoPlan* c, const FwdInfo& fi, int count_f, const AdjInfo& ai, 
     }
     std::vector<int2> ab;  // (depth, rtop)
     for (int rtop = count_a - 1; rtop >= 0;) {
-        const int d = (rtop + 1 >= ai.temporal) ? ai.temporal : 1;
+        const int d = next_depth(rtop + 1, ai.temporal, ai.precision, 1, ai.band);
         ab.push_back(make_int2(d, rtop));
         rtop -= d;
     }

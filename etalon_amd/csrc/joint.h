@@ -90,7 +90,7 @@ inline AdjSet<R> adj_set(const AdjTileArgs<R>* args, const TileList& tl, const i
 // Depths launch_joint accepts for (forward, adjoint): (T, 0), (0, T), (T, T), T in 1, 2, 4.
 inline bool joint_depths_supported(int tf, int ta) {
     auto ok = [](int t) { return t == 1 || t == 2 || t == 4; };
-    return (ok(tf) && ta == 0) || (tf == 0 && ok(ta)) || (ok(tf) && tf == ta);
+    return ((ok(tf) || tf == 6) && ta == 0) || (tf == 0 && ok(ta)) || (ok(tf) && tf == ta);
 }
 
 // Instantiations, compiled in separate files: forward only, adjoint only, both.

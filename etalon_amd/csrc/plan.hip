@@ -506,7 +506,8 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
     const int T = p->tune.temporal;
     int k = 0, n = 0;
     while (k < count) {
-        const int depth = (count - k >= T) ? T : 1;
+        const int depth =
+            next_depth(count - k, T, p->precision, p->tune.vec, p->tune.band_rows);
         if (p->tune.launch == 1) {  // every tile kind in one kernel
             if (events) {
                 ETL_HIP(hipEventRecord(events[4 * n], s));
@@ -952,20 +953,17 @@ extern "C" {
 
 namespace {
 
-// Default tile rows: 16 for float plans of one launch per step block with enough
-// 16-row tiles to give every CU of the device 8 of them (taller tiles re-read fewer
-// halo rows per step block), else 8 (more, smaller tiles to fill the device).
-int default_band(const EtlTeDesc* descs, int n_sims, int precision, const EtlTune& t) {
-    const int vec = t.vec == 0 ? 1 : t.vec;
-    const int depth = t.temporal == 0 ? 4 : t.temporal;
-    if (precision != ETL_F32 || vec != 1 || t.launch == 3 ||
-        !tile_shape_supported(precision, 1, depth, 16) || !tile_shape_supported(precision, 1, 1, 16))
-        return 8;
-    const int64_t cols = 64 - 2 * depth;
+// A large float plan: vec 1, not launch mode 3, and at least 8 tiles of 16 x 56 per
+// CU of the device.  Its defaults are 16-row tiles (taller tiles re-read fewer halo
+// rows per block) and temporal depth 6 (a launch moves about the same bytes as at
+// depth 4 for 6 steps instead of 4); smaller plans keep 8-row tiles and depth 4,
+// which fill the device with more, shorter waves.
+bool large_float_plan(const EtlTeDesc* descs, int n_sims, int precision, const EtlTune& t) {
+    if (precision != ETL_F32 || (t.vec != 0 && t.vec != 1) || t.launch == 3) return false;
     int64_t tiles = 0;
     for (int k = 0; k < n_sims; ++k)
-        tiles += int64_t((descs[k].nx + 15) / 16) * ((descs[k].ny + cols - 1) / cols);
-    return tiles >= int64_t(8) * std::max(ctx().cu_count, 1) ? 16 : 8;
+        tiles += int64_t((descs[k].nx + 15) / 16) * ((descs[k].ny + 55) / 56);
+    return tiles >= int64_t(8) * std::max(ctx().cu_count, 1);
 }
 
 }  // namespace
@@ -981,9 +979,10 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     EtlTune t{};
     if (tune) t = *tune;
     if (t.vec == 0) t.vec = 1;
-    if (t.temporal == 0) t.temporal = 4;
-    if (t.band_rows == 0) t.band_rows = default_band(descs, n_sims, precision, t);
-    if (t.trace_chunk == 0) t.trace_chunk = 64;
+    const bool large = large_float_plan(descs, n_sims, precision, t);
+    if (t.temporal == 0) t.temporal = large && (t.band_rows == 0 || t.band_rows == 16) ? 6 : 4;
+    if (t.band_rows == 0) t.band_rows = large ? 16 : 8;
+    if (t.trace_chunk == 0) t.trace_chunk = t.temporal == 6 ? 96 : 64;
     if (t.waves == 0) t.waves = 1;
     if (tune == nullptr) t.use_graph = 1;
     ETL_REQUIRE(tile_shape_supported(precision, t.vec, t.temporal, t.band_rows) &&
@@ -997,6 +996,8 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
                 "etl_plan_create: waves per workgroup must be 1, 2 or 4 (got %d)", t.waves);
     ETL_REQUIRE(t.launch == 0 || t.launch == 1 || t.launch == 3,
                 "etl_plan_create: launch must be 0, 1 or 3 (got %d)", t.launch);
+    ETL_REQUIRE(t.temporal != 6 || t.launch != 3,
+                "etl_plan_create: depth 6 runs with one launch per step block (launch 1)");
     ETL_REQUIRE(t.launch != 1 || (t.vec == 1 && (t.band_rows == 8 ||
                                                  (t.band_rows == 16 && precision == ETL_F32))),
                 "etl_plan_create: one launch per step block needs vec 1 and band 8 (or 16 in "

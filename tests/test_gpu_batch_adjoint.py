@@ -40,7 +40,7 @@ def test_batch_at_other_depths(hip, depth):
     problems, eps = candidates("single", steps=203)
     tune = Tune(temporal=depth)
     for p in problems:
-        p.tune = tune
+        p.tune = p.rtune = tune
     print(f"{len(problems)} candidates, depth {depth}", flush=True)
     Js, grads = BatchAdjointProblem(problems, tune=tune).run(eps)
     for k, (p, e) in enumerate(zip(problems, eps)):
