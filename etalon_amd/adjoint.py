@@ -30,7 +30,7 @@ import ctypes as C
 import functools
 import math
 import time
-from dataclasses import dataclass, field, replace
+from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 import numpy as np
@@ -58,18 +58,6 @@ def _forward_phasors(n_steps: int, dt: float, magnetic: bool, freqs: bytes,
     P = _phasor_table(0, n_steps, dt, magnetic, freqs, dtype).astype(np.complex128)
     P.setflags(write=False)
     return P
-
-
-def adjoint_tune(tune):
-    """The tune of an adjoint problem's recompute and reverse plans: an unset temporal
-    depth is 4, the depth of the reverse kernels, so that a co-plan pairs the
-    recompute's step blocks with the reverse steps' (large float forward plans would
-    default to depth 6)."""
-    from etalon_amd.hip.plan import Tune
-
-    if tune is None:
-        return Tune(temporal=4)
-    return tune if tune.temporal else replace(tune, temporal=4)
 
 
 def _weights(sampling) -> np.ndarray:
@@ -164,10 +152,9 @@ class AdjointProblem:
         self.sim = sim
         self.objective = tuple(objective)
         self.mode = mode
-        self.tune = tune  # the forward pass
-        # the recompute and reverse plans: depth 4 unless set, so that a co-plan pairs
-        # their step blocks
-        self.rtune = adjoint_tune(tune)
+        # the plans' tune (forward, recompute and reverse); by default every plan over
+        # this grid picks the same temporal depth, so a co-plan pairs its step blocks
+        self.tune = self.rtune = tune
         grid = build_yee_grid(sim)
         self.grid = grid
         self.n_steps = grid.n_steps(sim.run_time)
@@ -529,7 +516,9 @@ class _Reverse:
         if kernel == "tile":
             t = None
             if tune is not None:
-                t = _lib.EtlTune(0, 1, 1, 0, tune.temporal, tune.waves, tune.launch)
+                # the forward band: the plan's default depth follows the forward rule
+                t = _lib.EtlTune(tune.band_rows, 1, 1, 0, tune.temporal, tune.waves,
+                                 tune.launch)
             arr = (_lib.EtlAdjointDesc * len(descs))(*descs)
             _lib.call("etl_adjplan_create", arr, len(descs), C.byref(t) if t else None,
                       C.byref(self.handle))

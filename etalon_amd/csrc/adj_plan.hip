@@ -435,12 +435,22 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
                 "etl_adjplan_create: unknown precision %d", precision);
     EtlTune t{};
     if (tune) t = *tune;
-    if (t.temporal == 0) t.temporal = 4;
+    if (t.launch == 0) t.launch = 1;  // every tile kind in one kernel
+    // the forward plans' rule (fused.h: large_grids), seeing the forward band the
+    // caller asks for, so that a co-plan's recompute and reverse plans agree
+    if (t.temporal == 0)
+        t.temporal = precision == ETL_F32 && t.launch == 1 &&
+                             (t.band_rows == 0 || t.band_rows == 16) &&
+                             large_grids(descs, n_sims, ctx().cu_count)
+                         ? 6
+                         : 4;
     if (t.waves == 0) t.waves = 1;
     t.vec = 1;
     t.band_rows = kAdjBand;
-    ETL_REQUIRE(t.temporal == 1 || t.temporal == 2 || t.temporal == 4,
-                "etl_adjplan_create: temporal depth %d (1, 2 or 4)", t.temporal);
+    ETL_REQUIRE(t.temporal == 1 || t.temporal == 2 || t.temporal == 4 ||
+                    (t.temporal == 6 && precision == ETL_F32 && t.launch != 3),
+                "etl_adjplan_create: temporal depth %d (1, 2 or 4; 6 in float with one launch "
+                "per step block)", t.temporal);
     ETL_REQUIRE(t.waves == 1 || t.waves == 2 || t.waves == 4,
                 "etl_adjplan_create: waves per workgroup must be 1, 2 or 4");
     ETL_REQUIRE(t.launch == 0 || t.launch == 1 || t.launch == 3,
@@ -486,7 +496,6 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
         if (rc == ETL_OK)
             rc = precision == ETL_F32 ? adj_strips<float>(d, s.strips) : adj_strips<double>(d, s.strips);
     }
-    if (rc == ETL_OK && p->tune.launch == 0) p->tune.launch = 1;  // every tile kind in one kernel
     if (rc == ETL_OK) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||

@@ -126,6 +126,20 @@ inline bool tile_shape_supported(int precision, int vec, int depth, int band) {
     return false;
 }
 
+// Whether float plans over these grids count as large: at least 8 tiles of 16 x 56
+// per CU of the device.  Large float plans of one launch per step block default to
+// 16-row tiles and temporal depth 6 (a launch moves about the same bytes as at depth
+// 4 for 6 steps instead of 4), smaller ones to 8-row tiles and depth 4, which fill
+// the device with more, shorter waves.  Forward and adjoint plans use the same rule,
+// so that the two plans of a co-plan over the same grids agree on the depth.
+template <typename Desc>
+inline bool large_grids(const Desc* descs, int n_sims, int cu_count) {
+    int64_t tiles = 0;
+    for (int k = 0; k < n_sims; ++k)
+        tiles += int64_t((descs[k].nx + 15) / 16) * ((descs[k].ny + 55) / 56);
+    return tiles >= int64_t(8) * (cu_count > 0 ? cu_count : 1);
+}
+
 // Depth of the next launch with r >= 1 steps left at plan depth T: T, or for a tail
 // of fewer than T steps the largest compiled smaller depth that fits (4, 2, then 1)
 // — a launch moves the bytes of a whole window whatever its depth, so a tail of

@@ -953,17 +953,10 @@ extern "C" {
 
 namespace {
 
-// A large float plan: vec 1, not launch mode 3, and at least 8 tiles of 16 x 56 per
-// CU of the device.  Its defaults are 16-row tiles (taller tiles re-read fewer halo
-// rows per block) and temporal depth 6 (a launch moves about the same bytes as at
-// depth 4 for 6 steps instead of 4); smaller plans keep 8-row tiles and depth 4,
-// which fill the device with more, shorter waves.
+// Large float plans (the rule in fused.h): vec 1, not launch mode 3.
 bool large_float_plan(const EtlTeDesc* descs, int n_sims, int precision, const EtlTune& t) {
-    if (precision != ETL_F32 || (t.vec != 0 && t.vec != 1) || t.launch == 3) return false;
-    int64_t tiles = 0;
-    for (int k = 0; k < n_sims; ++k)
-        tiles += int64_t((descs[k].nx + 15) / 16) * ((descs[k].ny + 55) / 56);
-    return tiles >= int64_t(8) * std::max(ctx().cu_count, 1);
+    return precision == ETL_F32 && (t.vec == 0 || t.vec == 1) && t.launch != 3 &&
+           large_grids(descs, n_sims, ctx().cu_count);
 }
 
 }  // namespace

@@ -91,7 +91,7 @@ def test_single_gradient_against_float64(hip, oracle_gradient):
     assert abs(J - J_ref) / J_ref < 1e-4 and err < 1e-3
 
 
-@pytest.mark.parametrize("depth", [1, 2, 4])
+@pytest.mark.parametrize("depth", [1, 2, 4, 6])
 def test_tile_and_plain_adjoint_agree(hip, depth):
     """The temporally blocked reverse steps equal the per-half-step kernels (signed
     zeros aside), with sources, design box and CPML, in float32 and any depth."""
@@ -131,7 +131,7 @@ def test_checkpoint_and_store_modes_agree_bitwise(hip):
     assert np.array_equal(g1, g3)
 
 
-@pytest.mark.parametrize("depth", [1, 4])
+@pytest.mark.parametrize("depth", [1, 4, 6])
 def test_segment_schedules_agree_bitwise(hip, depth):
     """The recompute of segment k - 1 beside the reverse steps of segment k — in one
     launch per step block (joint) or on two streams (lanes) — against one after the

@@ -35,7 +35,7 @@ def test_batch_gradients_equal_each_candidate_alone(hip, precision):
         assert np.array_equal(grads[k], g1), f"candidate {k}: gradient"
 
 
-@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("depth", [1, 2, 6])
 def test_batch_at_other_depths(hip, depth):
     problems, eps = candidates("single", steps=203)
     tune = Tune(temporal=depth)
