@@ -30,7 +30,7 @@ import ctypes as C
 import functools
 import math
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import Optional, Sequence
 
 import numpy as np
@@ -58,6 +58,17 @@ def _forward_phasors(n_steps: int, dt: float, magnetic: bool, freqs: bytes,
     P = _phasor_table(0, n_steps, dt, magnetic, freqs, dtype).astype(np.complex128)
     P.setflags(write=False)
     return P
+
+
+def recompute_tune(tune, precision: str):
+    """The tune of an adjoint problem's recompute and reverse plans: the forward tune,
+    with an unset temporal depth 4 in double (large double forward plans default to
+    depth 6; the double reverse kernels are compiled up to depth 4)."""
+    from etalon_amd.hip.plan import Tune
+
+    if precision != "double" or (tune is not None and tune.temporal):
+        return tune
+    return Tune(temporal=4) if tune is None else replace(tune, temporal=4)
 
 
 def _weights(sampling) -> np.ndarray:
@@ -152,9 +163,12 @@ class AdjointProblem:
         self.sim = sim
         self.objective = tuple(objective)
         self.mode = mode
-        # the plans' tune (forward, recompute and reverse); by default every plan over
-        # this grid picks the same temporal depth, so a co-plan pairs its step blocks
-        self.tune = self.rtune = tune
+        # the plans' tune (forward, recompute and reverse).  By default every float
+        # plan over this grid picks the same temporal depth, so a co-plan pairs its step
+        # blocks; double reverse kernels run at depth 4 at most, and the recompute
+        # follows them.
+        self.tune = tune
+        self.rtune = recompute_tune(tune, sim.precision)
         grid = build_yee_grid(sim)
         self.grid = grid
         self.n_steps = grid.n_steps(sim.run_time)

@@ -113,11 +113,12 @@ inline int strip_kind(int r0, int r1, int cs, int T, int W, int x_lo, int x_hi, 
 struct TileShape {
     int precision, vec, depth, band;
 };
-// Depth 6 (float, vec 1) is compiled for the one-launch joint kernels only.
+// Depth 6 (vec 1; 16-row tiles in float only) is compiled for the one-launch joint
+// kernels only.
 constexpr TileShape kTileShapes[] = {
     {0, 1, 1, 8},  {0, 1, 2, 8},  {0, 1, 4, 8},  {0, 1, 6, 8},  {0, 1, 1, 16}, {0, 1, 2, 16},
     {0, 1, 4, 16}, {0, 1, 6, 16}, {0, 2, 1, 8},  {0, 2, 2, 8},  {0, 2, 4, 8},  {1, 1, 1, 8},
-    {1, 1, 2, 8},  {1, 1, 4, 8},  {1, 1, 1, 16}, {1, 1, 2, 16},
+    {1, 1, 2, 8},  {1, 1, 4, 8},  {1, 1, 6, 8},  {1, 1, 1, 16}, {1, 1, 2, 16},
 };
 inline bool tile_shape_supported(int precision, int vec, int depth, int band) {
     for (const TileShape& s : kTileShapes)
@@ -126,12 +127,12 @@ inline bool tile_shape_supported(int precision, int vec, int depth, int band) {
     return false;
 }
 
-// Whether float plans over these grids count as large: at least 8 tiles of 16 x 56
-// per CU of the device.  Large float plans of one launch per step block default to
-// 16-row tiles and temporal depth 6 (a launch moves about the same bytes as at depth
-// 4 for 6 steps instead of 4), smaller ones to 8-row tiles and depth 4, which fill
-// the device with more, shorter waves.  Forward and adjoint plans use the same rule,
-// so that the two plans of a co-plan over the same grids agree on the depth.
+// Whether plans over these grids count as large: at least 8 tiles of 16 x 56 per CU
+// of the device.  Large plans of one launch per step block default to temporal depth
+// 6 (a launch moves about the same bytes as at depth 4 for 6 steps instead of 4) and,
+// in float, 16-row tiles; smaller ones to 8-row tiles and depth 4, which fill the
+// device with more, shorter waves.  Float adjoint plans use the same rule, so that
+// the two plans of a co-plan over the same grids agree on the depth.
 template <typename Desc>
 inline bool large_grids(const Desc* descs, int n_sims, int cu_count) {
     int64_t tiles = 0;

@@ -1,4 +1,4 @@
-// One-launch forward step blocks (joint.h), float and double, depths 1, 2, 4.
+// One-launch forward step blocks (joint.h), float and double, depths 1, 2, 4, 6.
 #include "joint.h"
 
 namespace etl {
@@ -11,9 +11,7 @@ int launch_joint_fwd_any(const FwdSet<R>* f, int tf, int part, hipStream_t s) {
             case 2: return launch_joint_part<R, 2, 0>(f, nullptr, part, s);
             case 4: return launch_joint_part<R, 4, 0>(f, nullptr, part, s);
         }
-        if constexpr (sizeof(R) == 4) {  // depth 6: float only
-            if (tf == 6) return launch_joint_part<R, 6, 0>(f, nullptr, part, s);
-        }
+        if (tf == 6) return launch_joint_part<R, 6, 0>(f, nullptr, part, s);
     } else if constexpr (sizeof(R) == 4) {  // 16-row interior tiles: float only
         if (f->band == 16) switch (tf) {
                 case 1: return launch_joint_part<R, 1, 0, 16>(f, nullptr, part, s);

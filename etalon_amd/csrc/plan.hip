@@ -953,9 +953,9 @@ extern "C" {
 
 namespace {
 
-// Large float plans (the rule in fused.h): vec 1, not launch mode 3.
-bool large_float_plan(const EtlTeDesc* descs, int n_sims, int precision, const EtlTune& t) {
-    return precision == ETL_F32 && (t.vec == 0 || t.vec == 1) && t.launch != 3 &&
+// Large plans (the rule in fused.h): vec 1, not launch mode 3.
+bool large_plan(const EtlTeDesc* descs, int n_sims, const EtlTune& t) {
+    return (t.vec == 0 || t.vec == 1) && t.launch != 3 &&
            large_grids(descs, n_sims, ctx().cu_count);
 }
 
@@ -972,9 +972,11 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     EtlTune t{};
     if (tune) t = *tune;
     if (t.vec == 0) t.vec = 1;
-    const bool large = large_float_plan(descs, n_sims, precision, t);
-    if (t.temporal == 0) t.temporal = large && (t.band_rows == 0 || t.band_rows == 16) ? 6 : 4;
-    if (t.band_rows == 0) t.band_rows = large ? 16 : 8;
+    const bool large = large_plan(descs, n_sims, t);
+    const int large_band = precision == ETL_F32 ? 16 : 8;
+    if (t.temporal == 0)
+        t.temporal = large && (t.band_rows == 0 || t.band_rows == large_band) ? 6 : 4;
+    if (t.band_rows == 0) t.band_rows = large ? large_band : 8;
     if (t.trace_chunk == 0) t.trace_chunk = t.temporal == 6 ? 96 : 64;
     if (t.waves == 0) t.waves = 1;
     if (tune == nullptr) t.use_graph = 1;

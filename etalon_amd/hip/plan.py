@@ -27,9 +27,9 @@ class Tune:
     """Fused-kernel shape; 0 = the library default: vec 1, one wave per workgroup,
     ``launch`` 1 (one kernel per step block over every tile kind) where the shape
     allows it (vec 1 with band 8, or band 16 in float), else 3 (one kernel per tile
-    kind on three streams); for large float plans (at least 8 tiles of 16 x 56 per
-    CU) 16-row tiles and temporal depth 6 with 96-step trace chunks, else 8-row tiles
-    and depth 4 with 64-step chunks.  Depth 6 is float with launch 1 only."""
+    kind on three streams); for large plans (at least 8 tiles of 16 x 56 per CU)
+    temporal depth 6 with 96-step trace chunks and, in float, 16-row tiles; else
+    8-row tiles and depth 4 with 64-step chunks.  Depth 6 runs with launch 1 only."""
 
     band_rows: int = 0
     vec: int = 0

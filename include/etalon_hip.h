@@ -135,18 +135,18 @@ typedef struct EtlTeDesc {
  * 6, graph on, 64- or 96-step trace chunks, one wave per workgroup).  (vec, temporal,
  * band_rows) must be a compiled register-tile shape: float32 vec 1 with band 8 or
  * 16, or vec 2 with band 8, at temporal depth 1, 2 or 4, and float32 vec 1 at depth 6
- * (band 8 or 16, launch 1); float64 vec 1 with band 8 (depth 1, 2, 4) or band 16
- * (depth 1, 2). */
+ * (band 8 or 16, launch 1); float64 vec 1 with band 8 (depth 1, 2, 4, and 6 with
+ * launch 1) or band 16 (depth 1, 2). */
 typedef struct EtlTune {
     int band_rows;     /* output rows of one wave's tile; 0: 16 for large float plans
                         * (vec 1, not launch 3, at least 8 tiles of 16 x 56 per CU),
-                        * else 8 */
+                        * else 8 (large: the rule of the temporal default too) */
     int vec;           /* REAL elements per lane along y */
     int use_graph;     /* replay steps from a hipGraph (1) or launch directly (0) */
     int trace_chunk;   /* steps per DFT flush: a multiple of 2 * temporal (0: 64, or 96
                         * at depth 6) */
-    int temporal;      /* steps fused per launch: 1 (none), 2, 4, or 6 in float with
-                        * launch 1; 0: 6 for large float plans, else 4 */
+    int temporal;      /* steps fused per launch: 1 (none), 2, 4, or 6 with launch 1
+                        * (band 8, or 16 in float); 0: 6 for large plans, else 4 */
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
                         * band 8, or 16 in float; one wave per workgroup); 3: one kernel

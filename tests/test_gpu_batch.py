@@ -46,8 +46,8 @@ def assert_same(a, b, what):
                                    (1, 4, 16, 1, 1), (1, 6, 16, 1, 1), (1, 6, 8, 1, 1)])
 def test_batch_equals_each_simulation_alone(hip, precision, shape):
     vec, depth, band, waves, launch = shape
-    if precision == "double" and ((band == 16 and launch == 1) or depth == 6):
-        pytest.skip("16-row tiles in one launch and depth 6 are compiled for float only")
+    if precision == "double" and band == 16 and launch == 1:
+        pytest.skip("16-row interior tiles in one launch are compiled for float only")
     sims = small_batch(precision)
     print(f"batch of {len(sims)} {precision}, shape vec={vec} T={depth} band={band} waves={waves} "
           f"launch={launch}", flush=True)

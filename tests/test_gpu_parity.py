@@ -186,8 +186,7 @@ TUNES_F64 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16)]
 SHAPES = [("single", np.float32, *s, 3) for s in TUNES_F32] + \
     [("double", np.float64, *s, 3) for s in TUNES_F64] + \
     [(p, d, 1, t, 8, 1) for p, d in (("single", np.float32), ("double", np.float64))
-     for t in (1, 2, 4)] + [("single", np.float32, 1, t, 16, 1) for t in (1, 2, 4, 6)] + \
-    [("single", np.float32, 1, 6, 8, 1)]
+     for t in (1, 2, 4, 6)] + [("single", np.float32, 1, t, 16, 1) for t in (1, 2, 4, 6)]
 
 
 @pytest.mark.parametrize(("precision", "dtype", "vec", "depth", "band", "launch"), SHAPES)
