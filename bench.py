@@ -81,6 +81,7 @@ REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
 # 256 MiB copy gives them (FETCH_SIZE counts half the bytes read).
 PMC_KIB = {("single", 1, 6, 16, 1): (47527.0, 49859.5),
            ("single", 1, 4, 16, 1): (45949.8, 49675.3),
+           ("double", 1, 6, 8, 1): (103206.7, 110419.9),
            ("double", 1, 4, 8, 1): (90073.0, 98812.3)}
 FETCH_SCALE, WRITE_SCALE = 1.9998, 1.0
 
