@@ -1,7 +1,7 @@
 """Throughput benchmark of the fused 2D TE FDTD path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision single|double]
-                    [--no-cpu-baseline]
+                    [--no-cpu-baseline] [--hbm-steps S] [--interior N]
     python bench.py --workload batched [--batch N] [--batch-steps S] [--seed K]
     python bench.py --workload adjoint [--seed K] [--fd-pixels P]
     python bench.py --workload batched-adjoint [--candidates N] [--phc-steps S] [--fd-pixels P]
@@ -17,14 +17,15 @@ the guide at x = +45 µm (4 µm long, Hz, 21 wavelengths 1.50–1.60 µm), coura
 (fused kernel + monitor DFT), bracketed by device synchronisation.
 
 One JSON line on stdout: value = cell-updates/s over the whole grid (PML
-included) ÷ 1e9; a roofline object for the dominant kernel — in float the one
-kernel per 4-step block that advances every tile (joint_kernel), in double the
-interior-tile kernel of the three-stream launch — (algorithmic bytes: 32 B per
-cell-update in float32 — read and write Ex, Ey, Hz, read cb_ex, cb_ey — times the
-cells it advances times the steps fused per launch, over its average duration from
-HIP events on the stream it runs on, against 8 TB/s; "effective", since temporal
-blocking moves fewer bytes than that, with the measured `traffic` beside it); and
-the NumPy oracle on the host (one core) on a bounded sample of the same grid.
+included) ÷ 1e9; a roofline object for the dominant kernel — the one kernel per
+step block (6 steps on this grid) that advances every tile (joint_kernel) —
+(algorithmic bytes: 32 B per cell-update in float32, 64 B in float64 — read and
+write Ex, Ey, Hz, read cb_ex, cb_ey — times the cells it advances times the steps
+fused per launch, over its average duration from HIP events on the stream it runs
+on, against 8 TB/s; "effective", since temporal blocking moves fewer bytes than
+that, with the measured `traffic` beside it); `hbm_resident`, the same workload at
+8000 x 8000 interior cells, whose fields do not fit the GPU's caches; and the
+NumPy oracle on the host (one core) on a bounded sample of the same grid.
 
 ``--workload batched``: N (64) simulations of uneven sizes (interior 200–1500
 cells per side, seeded bends and rings, 1–8 monitors each, 4000 steps) advanced
@@ -84,6 +85,10 @@ PMC_KIB = {("single", 1, 6, 16, 1): (47527.0, 49859.5),
            ("double", 1, 6, 8, 1): (103206.7, 110419.9),
            ("double", 1, 4, 8, 1): (90073.0, 98812.3)}
 FETCH_SCALE, WRITE_SCALE = 1.9998, 1.0
+# The same for the one-launch kernel at 8024 x 8024, keyed by (precision, temporal,
+# band_rows).
+PMC_KIB_8024 = {("single", 6, 16): (744470.3, 795740.1),
+                ("double", 6, 8): (2222508.3, 1602349.5)}
 
 
 from etalon_amd.workloads import adjoint_design_sim, throughput_sim, uneven_batch  # noqa: E402
@@ -122,10 +127,10 @@ def _cpu_model() -> str:
     return "unknown CPU"
 
 
-def run(precision: str, steps: int, warmup: int) -> dict:
+def run(precision: str, steps: int, warmup: int, interior: int = 2000) -> dict:
     from etalon_amd.solver import FDTDSolver
 
-    sim = throughput_sim(precision)
+    sim = throughput_sim(precision, interior)
     solver = FDTDSolver(sim)
     nx, ny = solver.grid.nx, solver.grid.ny
     solver.advance(warmup)
@@ -163,6 +168,33 @@ def run(precision: str, steps: int, warmup: int) -> dict:
         "tune": plan.resolved_tune().__dict__,
         "device": solver.backend.device,
     }
+
+
+def hbm_resident(precision: str, steps: int) -> dict:
+    """The same workload at 8000 x 8000 interior cells (8024 x 8024): its fields (1.3 GB
+    in float32) exceed the 256 MB Infinity Cache, so every launch streams them from
+    HBM.  96 warm-up steps, then ``steps`` timed ones."""
+    r = run(precision, steps, 96, interior=8000)
+    tm = r["timing"]
+    nx, ny = r["cells"]
+    out = {
+        "grid": [nx, ny],
+        "value": round(r["value"], 3),
+        "unit": "Gcell-updates/s",
+        "steps": steps,
+        "ms_per_step": round(r["ms_per_step"], 5),
+        "avg_ms_per_launch": round(tm["kernel_ms"], 5),
+        "steps_per_launch": tm["depth"],
+        "effective_gbs": round(r["achieved_gbs"], 1),
+        "tune": r["tune"],
+    }
+    pmc = PMC_KIB_8024.get((precision, r["tune"]["temporal"], r["tune"]["band_rows"]))
+    if pmc is not None and (nx, ny) == (8024, 8024):
+        traffic = round((pmc[0] * FETCH_SCALE + pmc[1] * WRITE_SCALE) * 1024)
+        out["traffic"] = traffic
+        out["traffic_gbs"] = round(traffic / (tm["kernel_ms"] * 1e-3) / 1e9, 1)
+        out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    return out
 
 
 def _fingerprint(solver) -> str:
@@ -615,6 +647,10 @@ def main() -> int:
     ap.add_argument("--precision", choices=("single", "double"), default="single")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--interior", type=int, default=2000,
+                    help="interior cells per side of the throughput grid (profiling runs)")
+    ap.add_argument("--hbm-steps", type=int, default=1920,
+                    help="steps of the 8024x8024 companion run (0: skip it)")
     ap.add_argument("--workload", choices=("throughput", "batched", "adjoint", "batched-adjoint"),
                     default="throughput")
     ap.add_argument("--fd-pixels", type=int, default=16)
@@ -643,7 +679,7 @@ def main() -> int:
     if args.workload == "batched-adjoint":
         return main_batched_adjoint(args)
 
-    main_run = run(args.precision, args.steps, args.warmup)
+    main_run = run(args.precision, args.steps, args.warmup, args.interior)
     nx, ny = main_run["cells"]
     tune = main_run["tune"]
     pmc = PMC_KIB.get((args.precision, tune["vec"], tune["temporal"], tune["band_rows"],
@@ -676,11 +712,13 @@ def main() -> int:
         "ms_per_step": round(main_run["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(main_run["value"] / REFERENCE_GCELLS[args.precision], 2),
+        "vs_baseline": (round(main_run["value"] / REFERENCE_GCELLS[args.precision], 2)
+                        if args.interior == 2000 else None),
         "dtype": "f32" if args.precision == "single" else "f64",
         "data": "synthetic: strip waveguide geometry, zero initial fields, seeded-free",
         "config": {
-            "workload": f"throughput grid: 2000x2000 interior + 12-layer CPML = {nx}x{ny} cells, "
+            "workload": f"throughput grid: {args.interior}x{args.interior} interior + 12-layer "
+                        f"CPML = {nx}x{ny} cells, "
                         "Si strip eps 12.11 w 0.5 um in oxide eps 2.085, Hz Gaussian source "
                         "1.55 um, line DFT monitor Hz at 21 wavelengths, courant 0.99",
             "grid": [nx, ny],
@@ -704,6 +742,8 @@ def main() -> int:
                          "this grid (RTX 4060 Ti: 2.94 Gcell/s f32, 1.5 f64)",
         "device": main_run["device"],
     }
+    if args.hbm_steps > 0:
+        line["hbm_resident"] = hbm_resident(args.precision, args.hbm_steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.precision, args.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -15,23 +15,26 @@ GAP = 0.2     # ring-to-bus gap, µm
 DL = 0.05     # grid step, µm
 
 
-def throughput_sim(precision: str = "single") -> et.Simulation:
-    """2000 x 2000 interior cells at dl = 0.05 µm + 12-layer CPML (2024 x 2024): a
+def throughput_sim(precision: str = "single", interior: int = 2000) -> et.Simulation:
+    """``interior`` x ``interior`` cells at dl = 0.05 µm + 12-layer CPML (2000: 2024 x
+    2024 cells; 8000: 8024 x 8024, whose fields no longer fit the GPU's caches): a
     silicon strip (ε 12.11, 0.5 µm) along x through y = 0 into the PML, oxide cladding
     (ε 2.085), an Hz Gaussian source (1.55 µm, bandwidth 0.1 µm) 2 µm from the left
-    edge in the strip, a 4 µm Hz line monitor across the guide at x = +45 µm at 21
-    wavelengths 1.50–1.60 µm, courant 0.99."""
+    edge in the strip, a 4 µm Hz line monitor across the guide 5 µm from the right edge
+    (x = +45 µm at 2000) at 21 wavelengths 1.50–1.60 µm, courant 0.99."""
     oxide = et.Medium(n=float(np.sqrt(OXIDE)))
     silicon = et.Medium(n=float(np.sqrt(SILICON)))
+    size = round(interior * DL, 9)
     return et.Simulation(
-        size=(100.0, 100.0, 0.0),
+        size=(size, size, 0.0),
         grid=et.UniformGrid(dl=DL),
         medium=oxide,
-        structures=[et.Box(center=(0.0, 0.0, 0.0), size=(200.0, WIDTH, 0.0), medium=silicon)],
+        structures=[et.Box(center=(0.0, 0.0, 0.0), size=(2 * size, WIDTH, 0.0), medium=silicon)],
         boundary=et.PML(layers=12),
-        sources=[et.PointSource(center=(-48.0, 0.0, 0.0), component="Hz",
+        sources=[et.PointSource(center=(-size / 2 + 2.0, 0.0, 0.0), component="Hz",
                                 pulse=et.GaussianPulse(wavelength=1.55, bandwidth=0.1))],
-        monitors=[et.DFTMonitor(name="line", center=(45.0, 0.0, 0.0), size=(0.0, 4.0, 0.0),
+        monitors=[et.DFTMonitor(name="line", center=(size / 2 - 5.0, 0.0, 0.0),
+                                size=(0.0, 4.0, 0.0),
                                 wavelengths=np.linspace(1.50, 1.60, 21), components=("Hz",))],
         run_time=1.0,
         courant=0.99,

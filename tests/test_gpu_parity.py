@@ -168,6 +168,40 @@ def test_vacuum_pulse_2000_steps_bitwise(hip, precision, dtype):
             assert bitwise_equal(result[mon.name][comp].values, values[(mon.name, comp)])
 
 
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_largest_grid_bitwise(hip, precision, dtype):
+    """8024 x 8024 cells (64.4 M; the fields exceed the GPU's caches): the throughput
+    workload at 8000 x 8000 interior cells from random Ex, Ey, Hz (zero psi), 10 steps of
+    the default plan in two advances (depths 6, 1, then 2, 1), every field and the
+    monitor against the oracle, bit for bit."""
+    from etalon_amd.solver import FDTDSolver
+    from etalon_amd.workloads import throughput_sim
+
+    sim = throughput_sim(precision, 8000)
+    gpu, cpu = FDTDSolver(sim), ref.Solver(sim)
+    assert (gpu.grid.nx, gpu.grid.ny) == (8024, 8024)
+    rng = np.random.default_rng(5)
+    for name in ("ex", "ey", "hz"):
+        host = rng.standard_normal(getattr(cpu.s, name).shape, dtype=dtype)
+        if name == "hz":
+            host *= dtype(1 / 377.0)
+        setattr(cpu.s, name, host)
+        getattr(gpu.state, name).set(host)
+    gpu.advance(7)
+    gpu.advance(3)
+    cpu.run(10)
+    assert gpu.n == cpu.n == 10
+    print(f"{precision} 8024^2: tune {gpu._plan.resolved_tune().__dict__}", flush=True)
+    for name in FIELDS:
+        value = getattr(cpu.s, name)
+        assert np.abs(value).max() > 0, name
+        assert bitwise_equal(getattr(gpu.state, name).get(), value), name
+    assert len(gpu.monitors) == len(cpu.acc) == 1
+    for mg, acc in zip(gpu.monitors, cpu.acc):
+        assert np.abs(acc).max() > 0
+        assert bitwise_equal(mg.acc.get(), acc), mg.sampling.component
+
+
 _ORACLE_CACHE = {}
 
 
