@@ -22,7 +22,7 @@ from typing import Optional, Sequence
 
 from etalon_amd.backends import get_backend
 from etalon_amd.grid import build_yee_grid
-from etalon_amd.solver import _PLAN_TABLE_ROWS, FDTDSolver
+from etalon_amd.solver import _FIRST_TABLE_ROWS, _PLAN_TABLE_ROWS, FDTDSolver
 
 
 class BatchSolver:
@@ -79,8 +79,10 @@ class BatchSolver:
             return
         plan = self._ensure_plan()
         done = 0
+        grow = _FIRST_TABLE_ROWS  # tables of the next rows built while the GPU runs these
         while done < count:
-            rows = min(count - done, _PLAN_TABLE_ROWS)
+            rows = min(count - done, grow)
+            grow = min(4 * grow, _PLAN_TABLE_ROWS)
             for k, (s, ids) in enumerate(zip(self.solvers, self._ids)):
                 delta, phasors = s._tables(self.n, rows)
                 if delta is not None:

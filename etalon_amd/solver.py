@@ -34,8 +34,14 @@ from etalon_amd.state import FIELDS, build_te_state
 
 # Upper bound on the phasor table held at once (time steps × frequencies).
 _PHASOR_TABLE_ENTRIES = 1 << 22
-# Steps per fused-plan table refill (source deltas and phasors).
-_PLAN_TABLE_ROWS = 1 << 14
+# Steps per fused-plan table refill (source deltas and phasors): the first refill of an
+# advance covers _FIRST_TABLE_ROWS steps and each next one 4x more, up to
+# _PLAN_TABLE_ROWS.  The plan's launches are asynchronous, so the host builds the next
+# tables while the GPU runs the steps of the current ones, and only the first (small)
+# table is built with the GPU idle.  Both are multiples of every temporal depth (1, 2,
+# 4, 6) and trace chunk (64, 96), so a refill splits no step block.
+_FIRST_TABLE_ROWS = 192
+_PLAN_TABLE_ROWS = 24576
 
 
 @functools.lru_cache(maxsize=16)
@@ -239,12 +245,16 @@ class FDTDSolver:
                 self.step()
             return
         done = 0
+        grow = _FIRST_TABLE_ROWS
         while done < count:
-            rows = min(count - done, _PLAN_TABLE_ROWS)
             have = self._plan_tables
-            if not (have[0] <= self.n and self.n + rows <= have[1]):
+            if have[0] <= self.n < have[1]:
                 # the tables depend on the step numbers only: a run from step 0 again
                 # (after reset) reuses what the plan holds
+                rows = min(count - done, have[1] - self.n)
+            else:
+                rows = min(count - done, grow)
+                grow = min(4 * grow, _PLAN_TABLE_ROWS)
                 delta, phasors = self._tables(self.n, rows)
                 if delta is not None:
                     plan.source_table(self.n, delta)
