@@ -244,7 +244,7 @@ class FDTDSolver:
             for _ in range(count):
                 self.step()
             return
-        done = 0
+        start, done, refills = self.n, 0, 0
         grow = _FIRST_TABLE_ROWS
         while done < count:
             have = self._plan_tables
@@ -255,17 +255,28 @@ class FDTDSolver:
             else:
                 rows = min(count - done, grow)
                 grow = min(4 * grow, _PLAN_TABLE_ROWS)
-                delta, phasors = self._tables(self.n, rows)
-                if delta is not None:
-                    plan.source_table(self.n, delta)
-                for mon_id, table in zip(self._plan_ids, phasors):
-                    if table is not None:
-                        plan.phasor_table(mon_id, self.n, table)
-                self._plan_tables = (self.n, self.n + rows)
+                self._stage_tables(plan, self.n, rows)
+                refills += 1
             plan.advance(self.n, rows)
             plan.bind()
             self.n += rows
             done += rows
+        if refills > 1 and count <= _PLAN_TABLE_ROWS:
+            # Built while the GPU runs the last piece and uploaded before the plan's next
+            # steps: a run over the same steps again (a design loop's next forward, after
+            # reset) then finds all its tables in one piece.
+            self._stage_tables(plan, start, count)
+
+    def _stage_tables(self, plan, n0: int, rows: int) -> None:
+        """Hand the plan the source and phasor tables of steps [n0, n0 + rows); it
+        uploads them before its next steps."""
+        delta, phasors = self._tables(n0, rows)
+        if delta is not None:
+            plan.source_table(n0, delta)
+        for mon_id, table in zip(self._plan_ids, phasors):
+            if table is not None:
+                plan.phasor_table(mon_id, n0, table)
+        self._plan_tables = (n0, n0 + rows)
 
     # ---------------------------------------------------------------- results
     def fields(self) -> dict[str, np.ndarray]:

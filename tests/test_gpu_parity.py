@@ -212,6 +212,30 @@ def _oracle_full(precision, steps):
     return _ORACLE_CACHE[key]
 
 
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_reset_and_run_again_bitwise(hip, precision, dtype):
+    """A design loop's pattern: advance, reset, advance again.  The first advance
+    refills the step tables in pieces and leaves the plan the table of its whole range;
+    the second run uses that table, a third one other pieces; all three end bit for bit
+    at the oracle's state."""
+    from etalon_amd.solver import FDTDSolver
+
+    sim = full_sim(precision)
+    gpu = FDTDSolver(sim)
+    cpu = _oracle_full(precision, 600)
+    gpu.advance(600)
+    assert gpu._plan_tables == (0, 600)
+    _compare_solver(gpu, cpu)
+    gpu.reset()
+    gpu.advance(600)
+    assert gpu._plan_tables == (0, 600)
+    _compare_solver(gpu, cpu)
+    gpu.reset()
+    for count in (100, 37, 463):
+        gpu.advance(count)
+    _compare_solver(gpu, cpu)
+
+
 # (vec, temporal depth, band rows): every compiled tile shape is its own code path
 TUNES_F32 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16), (1, 4, 16),
              (2, 1, 8), (2, 2, 8), (2, 4, 8)]
