@@ -34,7 +34,8 @@ sims/s, the same simulations run one after another (fused, and with the per-step
 primitives on a bounded sample of steps), a bitwise check of every simulation's
 fields against its sequential run, and the float32 error of every monitor against
 the float64 run of the same batch (float64 on this path is bit-identical to the
-NumPy oracle).
+NumPy oracle); cpu_baseline: the NumPy oracle over every simulation, one process
+per core (at most 16), ``--cpu-batch-steps`` steps each.
 
 ``--workload adjoint``: a 424 x 424 grid (400 x 400 interior + 12 CPML) with a
 2 x 2 µm design region of 40 x 40 pixels between an input and an output strip,
@@ -275,7 +276,53 @@ def run_batched(n: int, steps: int, seed: int, plain_steps: int) -> dict:
             "errors": errors, "tune": tune, "tune_report": report}
 
 
+def _oracle_sample(task) -> tuple:
+    """One simulation on the NumPy oracle: 2 warm-up steps, then ``steps`` timed ones;
+    returns (cell-updates, seconds)."""
+    from oracle import te_numpy as ref
+
+    sim, steps = task
+    solver = ref.Solver(sim)
+    for _ in range(2):
+        solver.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step()
+    return solver.g.nx * solver.g.ny * steps, time.perf_counter() - t0
+
+
+def batched_cpu_baseline(sims: list, steps: int) -> dict:
+    """The NumPy oracle over every simulation of the batch, one process per core (at
+    most 16), a bounded number of steps each.  Runs before anything touches the GPU,
+    so the worker processes are forked from a process without GPU state."""
+    import multiprocessing
+
+    workers = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    with multiprocessing.get_context("fork").Pool(workers) as pool:
+        out = pool.map(_oracle_sample, [(sim, steps) for sim in sims], chunksize=1)
+    wall = time.perf_counter() - t0
+    updates = sum(u for u, _ in out)
+    busy = sum(t for _, t in out)
+    return {
+        "value": updates / (busy / workers) / 1e9,
+        "unit": "Gcell-updates/s",
+        "cores": workers,
+        "kind": "port",
+        "single_core": updates / busy / 1e9,
+        "sample": f"{steps} steps (after 2 warm-up) of each of the {len(sims)} simulations on "
+                  f"the NumPy oracle, one process per core over {workers} cores; value = "
+                  f"cell-updates / (summed step time / cores), single_core = cell-updates / "
+                  f"summed step time; {wall:.1f} s wall with set-up; host {os.cpu_count()} "
+                  f"CPUs, {_cpu_model()}",
+    }
+
+
 def main_batched(args) -> int:
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = batched_cpu_baseline(uneven_batch(n=args.batch, seed=args.seed,
+                                                steps=args.batch_steps), args.cpu_batch_steps)
     r = run_batched(args.batch, args.batch_steps, args.seed, args.plain_steps)
     updates = r["cells"] * r["steps"]
     errors = np.array(r["errors"])
@@ -316,6 +363,8 @@ def main_batched(args) -> int:
                      for t in (r["tune_report"] or {}).get("timings", [])],
         "setup_s": round(r["setup_s"], 2),
     }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
     return 0
 
@@ -659,6 +708,8 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=4000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--cpu-batch-steps", type=int, default=40,
+                    help="batched workload: oracle steps per simulation for cpu_baseline")
     ap.add_argument("--candidates", type=int, default=32)
     ap.add_argument("--phc-steps", type=int, default=8000)
     ap.add_argument("--cpu-phc-steps", type=int, default=20)
