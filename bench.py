@@ -696,6 +696,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=("single", "double"), default="single")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--copy-mb", type=int, default=1024,
+                    help="MiB of the device copy timed for the achievable HBM rate (0: skip)")
     ap.add_argument("--interior", type=int, default=2000,
                     help="interior cells per side of the throughput grid (profiling runs)")
     ap.add_argument("--hbm-steps", type=int, default=1920,
@@ -752,6 +754,16 @@ def main() -> int:
     if traffic is not None:
         roof["traffic_gbs"] = round(traffic / kernel_s / 1e9, 1)
         roof["traffic_frac"] = round(traffic / kernel_s / 1e9 / HBM_PEAK_GBS, 4)
+    if args.copy_mb > 0:
+        from etalon_amd.hip.calibrate import copy_rate
+
+        # an achievable HBM rate: a device copy of buffers well beyond the Infinity Cache
+        copy = copy_rate(args.copy_mb, 10)
+        roof["copy_gbs"] = copy["gbs_read_plus_write"]
+        roof["copy_note"] = (f"the HIP runtime's copy of {args.copy_mb} MiB into another "
+                             "buffer: bytes read + written per second")
+        if traffic is not None:
+            roof["traffic_frac_of_copy"] = round(roof["traffic_gbs"] / copy["gbs_read_plus_write"], 4)
     tm = main_run["timing"]
     line = {
         "metric": "Gcell-updates/s & %HBM roofline",

@@ -23,25 +23,30 @@ from etalon_amd.hip import _lib
 from etalon_amd.hip.backend import HipBackend, zeros
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--mb", type=int, default=256)
-    ap.add_argument("--reps", type=int, default=5)
-    args = ap.parse_args()
+def copy_rate(mb: int, reps: int) -> dict:
+    """Copy an ``mb`` MiB device buffer into another, once untimed and ``reps`` times
+    timed (wall clock between device synchronisations)."""
     backend = HipBackend("single")
-    n = args.mb << 20
+    n = mb << 20
     a = zeros((n // 4,), np.float32)
     b = zeros((n // 4,), np.float32)
     _lib.call("etl_memcpy_d2d", C.c_void_p(b.ptr), C.c_void_p(a.ptr), n)
     backend.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.reps):
+    for _ in range(reps):
         _lib.call("etl_memcpy_d2d", C.c_void_p(b.ptr), C.c_void_p(a.ptr), n)
     backend.synchronize()
-    dt = (time.perf_counter() - t0) / args.reps
-    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "dispatches": args.reps + 1,
-                      "ms_per_copy": round(dt * 1e3, 4),
-                      "gbs_read_plus_write": round(2 * n / dt / 1e9, 1)}), flush=True)
+    dt = (time.perf_counter() - t0) / reps
+    return {"copy_bytes_read": n, "copy_bytes_written": n, "dispatches": reps + 1,
+            "ms_per_copy": round(dt * 1e3, 4), "gbs_read_plus_write": round(2 * n / dt / 1e9, 1)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mb", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args()
+    print(json.dumps(copy_rate(args.mb, args.reps)), flush=True)
     return 0
 
 
