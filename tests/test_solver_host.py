@@ -1,0 +1,82 @@
+"""Host logic of FDTDSolver.advance, without a GPU: the step tables are refilled in
+growing pieces (the host builds the next piece while the GPU runs the current one),
+a multi-piece advance leaves the plan the table of its whole range, and a run over
+the same steps again (after reset) uses it without a refill.  A recording stand-in
+takes the place of the fused plan."""
+
+import numpy as np
+
+from etalon_amd import solver as solver_mod
+from etalon_amd.solver import FDTDSolver
+
+
+class RecordingPlan:
+    def __init__(self):
+        self.calls = []
+
+    def source_table(self, n0, delta):
+        self.calls.append(("delta", n0, len(delta)))
+
+    def phasor_table(self, mon_id, n0, table):
+        self.calls.append(("phasor", mon_id, n0, len(table)))
+
+    def advance(self, n0, rows):
+        self.calls.append(("advance", n0, rows))
+
+    def bind(self):
+        pass
+
+
+def make_solver(plan):
+    s = object.__new__(FDTDSolver)  # no state or device: advance's host logic only
+    s.n = 0
+    s._plan_tables = (0, 0)
+    s._plan_ids = [7]
+    s._ensure_plan = lambda: plan
+    s.fused_supported = lambda: True
+    s._tables = lambda n0, rows: (np.zeros((rows, 1)), [np.zeros((rows, 2))])
+    return s
+
+
+def advances(plan):
+    return [c[1:] for c in plan.calls if c[0] == "advance"]
+
+
+def test_refills_grow_and_stage_the_whole_range():
+    plan = RecordingPlan()
+    s = make_solver(plan)
+    s.advance(10000)
+    assert advances(plan) == [(0, 192), (192, 768), (960, 3072), (4032, 5968)]
+    # each piece's tables are handed over before its steps; the whole range last
+    assert plan.calls[-2:] == [("delta", 0, 10000), ("phasor", 7, 0, 10000)]
+    assert s.n == 10000 and s._plan_tables == (0, 10000)
+    first = solver_mod._FIRST_TABLE_ROWS
+    for n0, rows in advances(plan)[:-1]:
+        assert rows % first == 0  # no refill splits a step block or trace chunk
+
+    plan.calls.clear()
+    s.n = 0  # reset: the same steps again
+    s.advance(10000)
+    assert plan.calls == [("advance", 0, 10000)]
+
+
+def test_one_piece_needs_no_second_table():
+    plan = RecordingPlan()
+    s = make_solver(plan)
+    s.advance(100)
+    assert plan.calls == [("delta", 0, 100), ("phasor", 7, 0, 100), ("advance", 0, 100)]
+    assert s._plan_tables == (0, 100)
+    plan.calls.clear()
+    s.advance(50)  # beyond the held table: a new piece from step 100
+    assert plan.calls == [("delta", 100, 50), ("phasor", 7, 100, 50), ("advance", 100, 50)]
+
+
+def test_long_runs_cap_the_piece_and_stage_nothing_larger():
+    plan = RecordingPlan()
+    s = make_solver(plan)
+    cap = solver_mod._PLAN_TABLE_ROWS
+    s.advance(60000)
+    rows = [r for _, r in advances(plan)]
+    assert rows == [192, 768, 3072, 12288, cap, 60000 - (192 + 768 + 3072 + 12288 + cap)]
+    assert max(c[2] for c in plan.calls if c[0] == "delta") <= cap
+    assert s.n == 60000
