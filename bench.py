@@ -759,11 +759,13 @@ def main() -> int:
 
         # an achievable HBM rate: a device copy of buffers well beyond the Infinity Cache
         copy = copy_rate(args.copy_mb, 10)
-        roof["copy_gbs"] = copy["gbs_read_plus_write"]
-        roof["copy_note"] = (f"the HIP runtime's copy of {args.copy_mb} MiB into another "
-                             "buffer: bytes read + written per second")
+        roof["copy_gbs"] = max(copy["gbs_read_plus_write"], copy["stream_copy_gbs"])
+        roof["copy_note"] = (f"the faster of two copies of {args.copy_mb} MiB into another "
+                             f"buffer, bytes read + written per second: the HIP runtime's "
+                             f"{copy['gbs_read_plus_write']}, a 16-byte streaming kernel "
+                             f"{copy['stream_copy_gbs']}")
         if traffic is not None:
-            roof["traffic_frac_of_copy"] = round(roof["traffic_gbs"] / copy["gbs_read_plus_write"], 4)
+            roof["traffic_frac_of_copy"] = round(roof["traffic_gbs"] / roof["copy_gbs"], 4)
     tm = main_run["timing"]
     line = {
         "metric": "Gcell-updates/s & %HBM roofline",

@@ -6,6 +6,7 @@
 // (hipFree synchronises the device).  Every block carries kGuard bytes of valid
 // device memory before and after the array, so the fused kernels can issue whole
 // vector loads that overhang a row by a few elements without branching.
+#include <algorithm>
 #include <cstdarg>
 #include <map>
 #include <mutex>
@@ -88,6 +89,14 @@ __global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n_words) {
     size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     for (; k < n_words; k += stride) p[k] = 0u;
+}
+
+// Streaming copy for measuring the device's copy rate: 16-byte elements, grid-stride.
+__global__ void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                   size_t n) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride)
+        dst[k] = src[k];
 }
 
 __global__ void fill_zero_bytes_kernel(uint8_t* __restrict__ p, size_t n) {
@@ -264,6 +273,45 @@ int etl_memcpy_d2d(void* dst, const void* src, size_t bytes) {
     if (int rc = wait_other_lanes(0)) return rc;  // after the work of every lane
     ETL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx().stream));
     return ETL_OK;
+}
+
+int etl_copy_rate(size_t bytes, int reps, float* ms_per_copy) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(ms_per_copy != nullptr && reps >= 1 && bytes >= 16 && bytes % 16 == 0,
+                "etl_copy_rate: bytes must be a positive multiple of 16, reps >= 1");
+    void* a = nullptr;
+    void* b = nullptr;
+    int rc = etl_malloc(&a, bytes);
+    if (rc == ETL_OK) rc = etl_malloc(&b, bytes);
+    if (rc == ETL_OK) rc = zero_async(a, bytes);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (rc == ETL_OK && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)) {
+        set_error("etl_copy_rate: event creation failed");
+        rc = ETL_EHIP;
+    }
+    if (rc == ETL_OK) {
+        const size_t n = bytes / 16;
+        hipStream_t s = ctx().stream;
+        const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 8192));
+        for (int r = 0; r <= reps && rc == ETL_OK; ++r) {  // launch 0 untimed
+            if (r == 1 && hipEventRecord(t0, s) != hipSuccess) rc = ETL_EHIP;
+            hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, s,
+                               static_cast<const float4*>(a), static_cast<float4*>(b), n);
+            if (hipGetLastError() != hipSuccess) rc = ETL_EHIP;
+        }
+        float ms = 0.f;
+        if (rc == ETL_OK && (hipEventRecord(t1, s) != hipSuccess ||
+                             hipEventSynchronize(t1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+            rc = ETL_EHIP;
+        if (rc == ETL_OK) *ms_per_copy = ms / float(reps);
+        else set_error("etl_copy_rate: copy or timing failed");
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (a) etl_free(a);
+    if (b) etl_free(b);
+    return rc;
 }
 
 int etl_zero(void* dst, size_t bytes) {

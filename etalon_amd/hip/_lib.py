@@ -101,6 +101,7 @@ _SIGNATURES = {
     "etl_memcpy_d2d_lane": [vp, vp, C.c_size_t, C.c_int],
     "etl_lane_wait": [C.c_int, C.c_int],
     "etl_zero": [vp, C.c_size_t],
+    "etl_copy_rate": [C.c_size_t, C.c_int, C.POINTER(C.c_float)],
     "etl_synchronize": [],
     "etl_update_h_te": [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                         C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],

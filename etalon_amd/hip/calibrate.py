@@ -25,7 +25,8 @@ from etalon_amd.hip.backend import HipBackend, zeros
 
 def copy_rate(mb: int, reps: int) -> dict:
     """Copy an ``mb`` MiB device buffer into another, once untimed and ``reps`` times
-    timed (wall clock between device synchronisations)."""
+    timed: with the HIP runtime's copy (wall clock between device synchronisations)
+    and with the library's 16-byte streaming copy kernel (HIP events)."""
     backend = HipBackend("single")
     n = mb << 20
     a = zeros((n // 4,), np.float32)
@@ -37,8 +38,13 @@ def copy_rate(mb: int, reps: int) -> dict:
         _lib.call("etl_memcpy_d2d", C.c_void_p(b.ptr), C.c_void_p(a.ptr), n)
     backend.synchronize()
     dt = (time.perf_counter() - t0) / reps
+    del a, b
+    ms = C.c_float()
+    _lib.call("etl_copy_rate", n, reps, C.byref(ms))
     return {"copy_bytes_read": n, "copy_bytes_written": n, "dispatches": reps + 1,
-            "ms_per_copy": round(dt * 1e3, 4), "gbs_read_plus_write": round(2 * n / dt / 1e9, 1)}
+            "ms_per_copy": round(dt * 1e3, 4), "gbs_read_plus_write": round(2 * n / dt / 1e9, 1),
+            "stream_copy_ms": round(ms.value, 4),
+            "stream_copy_gbs": round(2 * n / (ms.value * 1e-3) / 1e9, 1)}
 
 
 def main() -> int:

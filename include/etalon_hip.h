@@ -59,6 +59,11 @@ int etl_memcpy_h2d(void* dst, const void* src, size_t bytes);  /* synchronous */
 int etl_memcpy_d2h(void* dst, const void* src, size_t bytes);  /* synchronous */
 int etl_memcpy_d2d(void* dst, const void* src, size_t bytes);  /* stream-ordered */
 int etl_zero(void* dst, size_t bytes);                         /* fill kernel, stream-ordered */
+/* Measurement only, no reference counterpart: copies `bytes` (a multiple of 16) between
+ * two fresh device buffers with a 16-byte streaming kernel, once untimed and `reps`
+ * times timed with HIP events; *ms_per_copy = the average.  The achievable-bandwidth
+ * figure that bench.py reports beside the nominal HBM peak.  Synchronous. */
+int etl_copy_rate(size_t bytes, int reps, float* ms_per_copy);
 int etl_synchronize(void);
 /* Lanes (see above): work issued on lane `waiter` after this call starts once the
  * work issued on lane `signaler` before it has finished.  No reference counterpart. */
