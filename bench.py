@@ -17,15 +17,19 @@ the guide at x = +45 µm (4 µm long, Hz, 21 wavelengths 1.50–1.60 µm), coura
 (fused kernel + monitor DFT), bracketed by device synchronisation.
 
 One JSON line on stdout: value = cell-updates/s over the whole grid (PML
-included) ÷ 1e9; a roofline object for the dominant kernel — the one kernel per
-step block (6 steps on this grid) that advances every tile (joint_kernel) —
-(algorithmic bytes: 32 B per cell-update in float32, 64 B in float64 — read and
-write Ex, Ey, Hz, read cb_ex, cb_ey — times the cells it advances times the steps
-fused per launch, over its average duration from HIP events on the stream it runs
-on, against 8 TB/s; "effective", since temporal blocking moves fewer bytes than
-that, with the measured `traffic` beside it); `hbm_resident`, the same workload at
-8000 x 8000 interior cells, whose fields do not fit the GPU's caches; and the
-NumPy oracle on the host (one core) on a bounded sample of the same grid.
+included) ÷ 1e9, with ``timed_region.host_share`` the part of the timed wall that
+the GPU's own step time does not cover; a roofline object for the dominant kernel —
+the one kernel per step block (6 steps on this grid) that advances every tile
+(joint_kernel) — whose frac is the HBM bytes one launch moves (rocprofv3 PMC counts
+of this build of the kernel; the compulsory bytes, each array read or written once,
+when the table holds none) over its average duration from HIP events on the stream
+it runs on, against 8 TB/s, with the compulsory and the "effective" figure (32 B per
+cell-update in float32, 64 B in float64, times the cells and the steps a launch
+advances; above the peak under temporal blocking) beside it; `hbm_resident`, the
+same workload at 8000 x 8000 interior cells, whose fields do not fit the GPU's
+caches; summaries of the other precision, the batched workload and the adjoint
+workload (``--no-extras`` leaves them out); and the NumPy oracle on the host (one
+core) on a bounded sample of the same grid.
 
 ``--workload batched``: N (64) simulations of uneven sizes (interior 200–1500
 cells per side, seeded bends and rings, 1–8 monitors each, 4000 steps) advanced
@@ -76,20 +80,46 @@ BYTES_PER_CELL = {"single": 32, "double": 64}
 # Reference-published throughput on this grid (2024^2 cells incl. CPML), RTX 4060 Ti,
 # CuPy kernels: README.md:57, bench/results/b2ff355.json:76-87, docs/plan-m0.md:232.
 REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
-# Bytes the timed kernel moves per launch on this grid, from rocprofv3 PMC runs of this
-# script (profiles/fused_2024_pmc_traffic.md): FETCH_SIZE and WRITE_SIZE in KiB per
-# dispatch, keyed by (precision, vec, temporal, band_rows, launch) - launch 1: the one
-# kernel of every tile kind, launch 3: the interior-tile kernel - and the factors that a
-# 256 MiB copy gives them (FETCH_SIZE counts half the bytes read).
+# Bytes the timed kernel moves per launch, from rocprofv3 PMC runs of this script
+# (profiles/fused_2024_pmc_traffic.md, profiles/joint_kernel_8024_hbm.md): FETCH_SIZE and
+# WRITE_SIZE in KiB per dispatch, keyed by (precision, vec, temporal, band_rows, launch)
+# for the 2024 x 2024 grid and by (precision, temporal, band_rows) at 8024 x 8024, and
+# the factors that a 256 MiB copy gives them (FETCH_SIZE counts half the bytes read).
+# The counts belong to one build of the kernels: PMC_BUILD is the fingerprint of the
+# sources they were measured with (kernel_build_id), and a build with another
+# fingerprint reports no measured traffic until the counters are taken again.
+PMC_BUILD = "645ef0bd73e23b1e"
 PMC_KIB = {("single", 1, 6, 16, 1): (47527.0, 49859.5),
            ("single", 1, 4, 16, 1): (45949.8, 49675.3),
            ("double", 1, 6, 8, 1): (103206.7, 110419.9),
            ("double", 1, 4, 8, 1): (90073.0, 98812.3)}
 FETCH_SCALE, WRITE_SCALE = 1.9998, 1.0
-# The same for the one-launch kernel at 8024 x 8024, keyed by (precision, temporal,
-# band_rows).
 PMC_KIB_8024 = {("single", 6, 16): (744470.3, 795740.1),
                 ("double", 6, 8): (2222508.3, 1602349.5)}
+# The sources that define the forward joint kernel and its tile work-list.
+KERNEL_SOURCES = ("common.h", "fused.h", "te_tile.h", "adj_tile.h", "joint.h", "coplan.h",
+                  "joint_fwd.hip", "plan.hip")
+
+
+def kernel_build_id() -> str:
+    """sha256 (16 hex digits) over the sources of the forward joint kernel and its
+    tile layout (KERNEL_SOURCES): equal ids, equal kernels and work-lists."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        h.update(name.encode())
+        h.update((ROOT / "etalon_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(table: dict, key: tuple) -> "int | None":
+    """HBM bytes per launch from the PMC table, or None when the table is of another
+    build of the kernels or holds no entry for this shape."""
+    pmc = table.get(key)
+    if pmc is None or kernel_build_id() != PMC_BUILD:
+        return None
+    return round((pmc[0] * FETCH_SCALE + pmc[1] * WRITE_SCALE) * 1024)
 
 
 from etalon_amd.workloads import adjoint_design_sim, throughput_sim, uneven_batch  # noqa: E402
@@ -157,6 +187,7 @@ def run(precision: str, steps: int, warmup: int, interior: int = 2000) -> dict:
     depth = tm["depth"]
     per_cell = BYTES_PER_CELL[precision]
     kernel_bytes = per_cell * tm["kernel_cells"] * depth  # algorithmic, per launch
+    gpu_ms_per_step = tm["total_ms"] / (tm["launches"] * depth)
     return {
         "cells": (nx, ny),
         "wall": wall,
@@ -164,11 +195,58 @@ def run(precision: str, steps: int, warmup: int, interior: int = 2000) -> dict:
         "ms_per_step": wall / steps * 1e3,
         "timing": tm,
         "kernel_bytes": kernel_bytes,
-        "achieved_gbs": kernel_bytes / (tm["kernel_ms"] * 1e-3) / 1e9,
+        "gpu_ms_per_step": gpu_ms_per_step,
+        # the share of the timed wall that the GPU's own step time does not cover
+        # (launch and table work on the host, the syncs around the region)
+        "host_share": max(0.0, 1.0 - steps * gpu_ms_per_step / (wall * 1e3)),
         "launch_gbs": per_cell * cells * depth / (tm["launch_ms"] * 1e-3) / 1e9,
         "tune": plan.resolved_tune().__dict__,
         "device": solver.backend.device,
     }
+
+
+def roofline(precision: str, r: dict, traffic: "int | None") -> dict:
+    """The roofline object of the dominant kernel (the one joint kernel per step block).
+
+    frac = HBM bytes one launch moves over its event-timed duration, against 8 TB/s.
+    The bytes are the measured ones (rocprofv3 FETCH_SIZE and WRITE_SIZE of this build
+    of the kernels) when the PMC table holds them, else the compulsory ones — every
+    array the launch needs read once and every array it updates written once, 32 B per
+    cell in float32 and 64 in float64, a lower bound on its traffic.  The effective
+    figure (the algorithmic bytes of the T unfused steps a launch advances) is kept
+    beside it: temporal blocking puts it above the peak."""
+    tm = r["timing"]
+    kernel_s = tm["kernel_ms"] * 1e-3
+    compulsory = BYTES_PER_CELL[precision] * tm["kernel_cells"]
+    moved = traffic if traffic is not None else compulsory
+    gbs = moved / kernel_s / 1e9
+    roof = {
+        "bound": "hbm",
+        "achieved": round(gbs, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "basis": ("measured: HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE of "
+                  "this build of the kernel (scaled by a copy kernel's factors), over the "
+                  "launch's event-timed duration" if traffic is not None else
+                  "compulsory: no PMC counts for this build of the kernel, so the bytes are "
+                  "the compulsory ones (each array read or written once per launch), a lower "
+                  "bound on its traffic"),
+        "kernel_build": kernel_build_id(),
+        "compulsory_bytes": compulsory,
+        "compulsory_gbs": round(compulsory / kernel_s / 1e9, 1),
+        "compulsory_frac": round(compulsory / kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+        "effective_gbs": round(r["kernel_bytes"] / kernel_s / 1e9, 1),
+        "effective_frac": round(r["kernel_bytes"] / kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+        "effective_note": f"{BYTES_PER_CELL[precision]} B per cell-update (Ex, Ey, Hz read and "
+                          "written, cb_ex, cb_ey read) x cells x the steps one launch advances, "
+                          "over its duration: above the peak because temporal blocking moves "
+                          "each array once per launch, not once per step",
+    }
+    if traffic is not None:
+        roof["traffic_over_compulsory"] = round(traffic / compulsory, 3)
+    return roof
 
 
 def hbm_resident(precision: str, steps: int) -> dict:
@@ -178,7 +256,11 @@ def hbm_resident(precision: str, steps: int) -> dict:
     r = run(precision, steps, 96, interior=8000)
     tm = r["timing"]
     nx, ny = r["cells"]
-    out = {
+    traffic = None
+    if (nx, ny) == (8024, 8024):
+        traffic = measured_traffic(PMC_KIB_8024, (precision, r["tune"]["temporal"],
+                                                  r["tune"]["band_rows"]))
+    return {
         "grid": [nx, ny],
         "value": round(r["value"], 3),
         "unit": "Gcell-updates/s",
@@ -186,16 +268,9 @@ def hbm_resident(precision: str, steps: int) -> dict:
         "ms_per_step": round(r["ms_per_step"], 5),
         "avg_ms_per_launch": round(tm["kernel_ms"], 5),
         "steps_per_launch": tm["depth"],
-        "effective_gbs": round(r["achieved_gbs"], 1),
         "tune": r["tune"],
+        "roofline": roofline(precision, r, traffic),
     }
-    pmc = PMC_KIB_8024.get((precision, r["tune"]["temporal"], r["tune"]["band_rows"]))
-    if pmc is not None and (nx, ny) == (8024, 8024):
-        traffic = round((pmc[0] * FETCH_SCALE + pmc[1] * WRITE_SCALE) * 1024)
-        out["traffic"] = traffic
-        out["traffic_gbs"] = round(traffic / (tm["kernel_ms"] * 1e-3) / 1e9, 1)
-        out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
-    return out
 
 
 def _fingerprint(solver) -> str:
@@ -369,7 +444,9 @@ def main_batched(args) -> int:
     return 0
 
 
-def run_adjoint(seed: int, fd_pixels: int, cpu_steps: int) -> dict:
+def run_adjoint(seed: int, fd_pixels: int, cpu_steps: int, full: bool = True) -> dict:
+    """The adjoint workload's figures; ``full`` False skips the sequential-schedule and
+    store-mode gradients and the NumPy sample (the summary in the throughput line)."""
     from etalon_amd.adjoint import AdjointProblem
     from etalon_amd.solver import FDTDSolver
 
@@ -410,12 +487,14 @@ def run_adjoint(seed: int, fd_pixels: int, cpu_steps: int) -> dict:
         c.backward(tape)
         t = tape.forward_s + time.perf_counter() - t0
         t_cold = t if t_cold is None else min(t_cold, t)
-    seq = AdjointProblem(sim, names, schedule="sequential")
-    seq.run(eps)
-    t_seq, grad_q = grad_time(seq)
-    store = AdjointProblem(sim, names, mode="store")
-    store.run(eps)
-    t_store, grad_s = grad_time(store)
+    t_seq = t_store = grad_q = grad_s = None
+    if full:
+        seq = AdjointProblem(sim, names, schedule="sequential")
+        seq.run(eps)
+        t_seq, grad_q = grad_time(seq)
+        store = AdjointProblem(sim, names, mode="store")
+        store.run(eps)
+        t_store, grad_s = grad_time(store)
 
     sim64, _, _ = adjoint_design_sim("double", seed=seed)
     prob64 = AdjointProblem(sim64, names)
@@ -433,18 +512,20 @@ def run_adjoint(seed: int, fd_pixels: int, cpu_steps: int) -> dict:
         fd_rel.append(abs(grad[p, q] - fd) / scale)
 
     # NumPy float64 adjoint (the oracle) on the first `cpu_steps` steps of the grid
-    from oracle import adjoint_numpy as adj
-    from etalon_amd.workloads import adjoint_design_sim as make
+    cpu_wall = None
+    if full:
+        from oracle import adjoint_numpy as adj
+        from etalon_amd.workloads import adjoint_design_sim as make
 
-    short, short_design, _ = make("double", seed=seed, steps=cpu_steps)
-    t0 = time.perf_counter()
-    adj.gradient(short, set(names), short_design)
-    cpu_wall = time.perf_counter() - t0
+        short, short_design, _ = make("double", seed=seed, steps=cpu_steps)
+        t0 = time.perf_counter()
+        adj.gradient(short, set(names), short_design)
+        cpu_wall = time.perf_counter() - t0
     return {"t_fwd": t_fwd, "t_grad": t_grad, "t_store": t_store, "t_seq": t_seq,
             "t_fwd_cold": t_fwd_cold, "t_grad_cold": t_cold,
-            "seq_equal": bool(np.array_equal(grad, grad_q)), "J": tape.J, "J64": J64,
+            "seq_equal": full and bool(np.array_equal(grad, grad_q)), "J": tape.J, "J64": J64,
             "grad_rel_f64": float(np.abs(grad - grad64).max() / scale),
-            "store_equal": bool(np.array_equal(grad, grad_s)), "fd_rel": fd_rel,
+            "store_equal": full and bool(np.array_equal(grad, grad_s)), "fd_rel": fd_rel,
             "cpu_wall": cpu_wall, "cpu_steps": cpu_steps, "K": prob.K,
             "cells": prob.grid.nx * prob.grid.ny, "steps": prob.n_steps, "pixels": eps.shape}
 
@@ -695,6 +776,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", choices=("single", "double"), default="single")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="throughput line without the other precision, batched and adjoint "
+                         "summaries (profiling runs)")
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--copy-mb", type=int, default=1024,
                     help="MiB of the device copy timed for the achievable HBM rate (0: skip)")
@@ -735,25 +819,11 @@ def main() -> int:
     main_run = run(args.precision, args.steps, args.warmup, args.interior)
     nx, ny = main_run["cells"]
     tune = main_run["tune"]
-    pmc = PMC_KIB.get((args.precision, tune["vec"], tune["temporal"], tune["band_rows"],
-                       tune["launch"]))
     traffic = None
-    if pmc is not None and (nx, ny) == (2024, 2024):
-        traffic = round((pmc[0] * FETCH_SCALE + pmc[1] * WRITE_SCALE) * 1024)
-    kernel_s = main_run["timing"]["kernel_ms"] * 1e-3
-    roof = {
-        "bound": "hbm",
-        "achieved": round(main_run["achieved_gbs"], 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "note": "effective: algorithmic bytes of T unfused steps per launch; temporal blocking "
-                "moves `traffic` bytes (measured) instead",
-    }
-    if traffic is not None:
-        roof["traffic_gbs"] = round(traffic / kernel_s / 1e9, 1)
-        roof["traffic_frac"] = round(traffic / kernel_s / 1e9 / HBM_PEAK_GBS, 4)
+    if (nx, ny) == (2024, 2024):
+        traffic = measured_traffic(PMC_KIB, (args.precision, tune["vec"], tune["temporal"],
+                                             tune["band_rows"], tune["launch"]))
+    roof = roofline(args.precision, main_run, traffic)
     if args.copy_mb > 0:
         from etalon_amd.hip.calibrate import copy_rate
 
@@ -764,9 +834,7 @@ def main() -> int:
                              f"buffer, bytes read + written per second: the HIP runtime's "
                              f"{copy['gbs_read_plus_write']}, a 16-byte streaming kernel "
                              f"{copy['stream_copy_gbs']}")
-        if traffic is not None:
-            roof["traffic_frac_of_copy"] = round(roof["traffic_gbs"] / roof["copy_gbs"], 4)
-    tm = main_run["timing"]
+        roof["achieved_over_copy"] = round(roof["achieved"] / roof["copy_gbs"], 4)
     line = {
         "metric": "Gcell-updates/s & %HBM roofline",
         "value": round(main_run["value"], 3),
@@ -790,29 +858,101 @@ def main() -> int:
             "precision": args.precision,
         },
         "roofline": roof,
-        "kernel": {
-            "name": ("joint_kernel<R, T, 0, 0> (every tile kind, one launch per step block)"
-                     if tm["launch"] == 1 else "tile_kernel (interior tiles)"),
-            "avg_ms_per_launch": round(tm["kernel_ms"], 5),
-            "steps_per_launch": tm["depth"],
-            "cells_per_step": tm["kernel_cells"],
-            "algorithmic_bytes_per_launch": main_run["kernel_bytes"],
-            "bytes_per_cell_update": BYTES_PER_CELL[args.precision],
-            "tune": main_run["tune"],
-            "fused_launch_ms": round(tm["launch_ms"], 5),
-            "fused_launch_effective_gbs": round(main_run["launch_gbs"], 1),
-            "events_ms_per_step": round(tm["total_ms"] / (tm["launches"] * tm["depth"]), 5),
-        },
+        "timed_region": _timed_region(main_run),
+        "kernel": _kernel_summary(main_run, args.precision),
         "baseline_note": "vs_baseline divides by the reference's published CuPy throughput on "
                          "this grid (RTX 4060 Ti: 2.94 Gcell/s f32, 1.5 f64)",
         "device": main_run["device"],
     }
     if args.hbm_steps > 0:
         line["hbm_resident"] = hbm_resident(args.precision, args.hbm_steps)
+    if not args.no_extras:
+        other = "double" if args.precision == "single" else "single"
+        line[f"{'f64' if other == 'double' else 'f32'}"] = precision_summary(
+            other, args.steps, args.warmup, args.interior)
+        line["batched"] = batched_summary(args)
+        line["adjoint"] = adjoint_summary(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.precision, args.cpu_steps)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _timed_region(r: dict) -> dict:
+    return {"wall_ms": round(r["wall"] * 1e3, 4),
+            "gpu_ms_per_step": round(r["gpu_ms_per_step"], 5),
+            "host_share": round(r["host_share"], 4),
+            "note": "host_share = 1 - steps x the GPU's own time per step (HIP events over "
+                    "direct launches of the same plan) / the timed wall: what the host's "
+                    "launches, table work and the syncs around the region add"}
+
+
+def _kernel_summary(r: dict, precision: str) -> dict:
+    tm = r["timing"]
+    return {
+        "name": ("joint_kernel<R, T, 0, 0, BR> (every tile kind, one launch per step block)"
+                 if tm["launch"] == 1 else "tile_kernel (interior tiles)"),
+        "avg_ms_per_launch": round(tm["kernel_ms"], 5),
+        "steps_per_launch": tm["depth"],
+        "cells_per_step": tm["kernel_cells"],
+        "algorithmic_bytes_per_launch": r["kernel_bytes"],
+        "bytes_per_cell_update": BYTES_PER_CELL[precision],
+        "tune": r["tune"],
+        "fused_launch_ms": round(tm["launch_ms"], 5),
+        "events_ms_per_step": round(r["gpu_ms_per_step"], 5),
+    }
+
+
+def precision_summary(precision: str, steps: int, warmup: int, interior: int) -> dict:
+    """The throughput grid in the other precision: the same steps and warm-up."""
+    r = run(precision, steps, warmup, interior)
+    nx, ny = r["cells"]
+    tune = r["tune"]
+    traffic = None
+    if (nx, ny) == (2024, 2024):
+        traffic = measured_traffic(PMC_KIB, (precision, tune["vec"], tune["temporal"],
+                                             tune["band_rows"], tune["launch"]))
+    return {"value": round(r["value"], 3), "unit": "Gcell-updates/s",
+            "dtype": "f64" if precision == "double" else "f32",
+            "steps": steps, "warmup": warmup, "ms_per_step": round(r["ms_per_step"], 5),
+            "vs_baseline": (round(r["value"] / REFERENCE_GCELLS[precision], 2)
+                            if interior == 2000 else None),
+            "roofline": roofline(precision, r, traffic), "timed_region": _timed_region(r),
+            "kernel": _kernel_summary(r, precision)}
+
+
+def batched_summary(args) -> dict:
+    """The batched workload (64 uneven simulations, 4000 steps) in the throughput line."""
+    r = run_batched(args.batch, args.batch_steps, args.seed, args.plain_steps)
+    updates = r["cells"] * r["steps"]
+    errors = np.array(r["errors"])
+    return {"metric": "batched sims/s (aggregate Gcell-updates/s)",
+            "value": round(updates / r["wall"] / 1e9, 3), "unit": "Gcell-updates/s",
+            "sims_per_s": round(r["n"] / r["wall"], 2), "n_sims": r["n"], "steps": r["steps"],
+            "cells_total": r["cells"], "wall_s": round(r["wall"], 4), "dtype": "f32",
+            "speedup_vs_sequential_plain": round(r["plain_wall"] / r["wall"], 2),
+            "speedup_vs_sequential_fused": round(r["seq_wall"] / r["wall"], 2),
+            "bitwise_equal_to_sequential": f"{r['bitwise_same']}/{r['n']}",
+            "f32_vs_f64_max_rel": float(f"{errors.max():.3g}"),
+            "f32_vs_f64_monitors": int(errors.size), "tune": r["tune"],
+            "note": f"seed {args.seed}; sequential plain = {args.plain_steps} step() calls per "
+                    "simulation scaled to the full step count; full line: --workload batched"}
+
+
+def adjoint_summary(args) -> dict:
+    """The adjoint workload (424 x 424, 40 x 40 pixels, 4000 steps) in the throughput line."""
+    r = run_adjoint(args.seed, args.fd_pixels, args.cpu_adjoint_steps, full=False)
+    fd = np.array(r["fd_rel"])
+    return {"metric": "adjoint grad time vs 1 fwd",
+            "value": round(r["t_grad"] / r["t_fwd"], 3), "unit": "x",
+            "t_grad_s": round(r["t_grad"], 4), "t_fwd_s": round(r["t_fwd"], 4),
+            "cold_ratio": round(r["t_grad_cold"] / r["t_fwd_cold"], 3),
+            "cold_t_grad_s": round(r["t_grad_cold"], 4), "cold_t_fwd_s": round(r["t_fwd_cold"], 4),
+            "steps": r["steps"], "K": r["K"], "pixels": list(r["pixels"]), "dtype": "f32",
+            "fd_pixels": int(fd.size), "vs_f64_central_fd_max_rel": float(f"{fd.max():.3g}"),
+            "vs_f64_adjoint_max_rel": float(f"{r['grad_rel_f64']:.3g}"),
+            "note": "value: design loop (solver, plans and graphs kept), best of 3; cold: fresh "
+                    "solver / problem per run; full line: --workload adjoint"}
 
 
 if __name__ == "__main__":

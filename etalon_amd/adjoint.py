@@ -90,15 +90,20 @@ class Tape:
     starts: list = field(default_factory=list)       # segment starts
     history: Optional[object] = None                 # store mode: (N, box) probe output
     forward_s: float = 0.0
+    eps: Optional[np.ndarray] = None                 # the design permittivity it ran
+    generation: int = 0                              # the session forward that made it
 
 
 @dataclass
 class _Session:
-    """What a design loop keeps between gradients of one geometry."""
+    """What a design loop keeps between gradients of one geometry.  Every forward on
+    it overwrites the solver's state, monitors and checkpoints; ``generation`` counts
+    those forwards, so a tape of an earlier one is known to be stale."""
 
     solver: FDTDSolver
     checkpoints: dict
     starts: list
+    generation: int = 1
     bufs: Optional["_AdjointBuffers"] = None
     rev: Optional["_Reverse"] = None
     rplan: Optional[object] = None
@@ -221,6 +226,10 @@ class AdjointProblem:
         """Run the simulation with design permittivity ``eps``; J and the tape."""
         if self.reuse and self._session is not None:
             return self._forward_again(eps)
+        return self._forward_new(eps, session=self.reuse)
+
+    def _forward_new(self, eps: np.ndarray, session: bool) -> Tape:
+        """The forward pass on a new solver; ``session``: keep it for a design loop."""
         sim = self.with_permittivity(eps)
         solver = FDTDSolver(sim, tune=self.tune)
         self._check_box_off_cpml(solver)
@@ -235,7 +244,7 @@ class AdjointProblem:
             if table is not None:
                 plan.phasor_table(mon_id, 0, table)
         solver._plan_tables = (0, N)
-        tape = Tape(solver=solver, J=0.0)
+        tape = Tape(solver=solver, J=0.0, eps=np.array(eps, dtype=np.float64))
         if self.mode == "store":
             tape.history = zeros((N, self._box_hz_nodes().size), solver.real)
             plan.add_probe("Hz", self._box_hz_nodes(), tape.history)
@@ -250,9 +259,10 @@ class AdjointProblem:
             plan.set_checkpoints([tape.checkpoints[name] for name in FIELDS], n_ck)
             plan.advance_checkpointed(0, N, K)  # slot k = the state at step k K
             plan.bind()
-            if self.reuse:
+            if session:
                 self._session = _Session(solver=solver, checkpoints=tape.checkpoints,
                                          starts=tape.starts)
+                tape.generation = self._session.generation
         solver.n = N
         backend.synchronize()
         tape.forward_s = time.perf_counter() - t0
@@ -268,12 +278,14 @@ class AdjointProblem:
         solver = ses.solver
         t0 = time.perf_counter()
         self.set_design(solver, eps)
+        ses.generation += 1
         plan = solver._ensure_plan()
         plan.advance_checkpointed(0, self.n_steps, self.K)
         plan.bind()
         solver.n = self.n_steps
         solver.backend.synchronize()
-        tape = Tape(solver=solver, J=0.0, checkpoints=ses.checkpoints, starts=ses.starts)
+        tape = Tape(solver=solver, J=0.0, checkpoints=ses.checkpoints, starts=ses.starts,
+                    eps=np.array(eps, dtype=np.float64), generation=ses.generation)
         tape.forward_s = time.perf_counter() - t0
         tape.J = self._objective(solver)
         return tape
@@ -343,8 +355,16 @@ class AdjointProblem:
 
     # ------------------------------------------------------------------ backward
     def backward(self, tape: Tape) -> np.ndarray:
-        """∂J/∂ε per design pixel (float64), from the forward tape."""
-        if self.reuse and self._session is not None and tape.solver is self._session.solver:
+        """∂J/∂ε per design pixel (float64), from the forward tape.
+
+        A tape of the design-loop session whose state a later forward has overwritten
+        (J1 = objective(a); J2 = objective(b); (J1 - J2).backward(), a line search
+        between forward and backward) is replayed: its forward runs again at its own
+        permittivity on a new solver, and the backward pass runs on that."""
+        ses = self._session
+        if self.reuse and ses is not None and tape.solver is ses.solver:
+            if tape.generation != ses.generation:
+                return self.backward(self._forward_new(tape.eps, session=False))
             return self._backward_session(tape)
         solver = tape.solver
         N, K = self.n_steps, self.K

@@ -43,17 +43,22 @@ class BatchTape:
     checkpoints: list = field(default_factory=list)  # per candidate: field -> (n_ck, size)
     histories: list = field(default_factory=list)    # store mode: per candidate (N, box nodes)
     forward_s: float = 0.0
+    eps: list = field(default_factory=list)          # the design permittivities it ran
+    generation: int = 0                              # the session forward that made it
 
 
 @dataclass
 class _BatchSession:
     """What a design loop keeps between batched gradients of one set of geometries:
     the batch (states, forward plan, tables, checkpoint storage) and, after the first
-    backward pass, the reverse plan, the recompute plan and their co-plan."""
+    backward pass, the reverse plan, the recompute plan and their co-plan.  Every
+    forward on it overwrites the batch's state; ``generation`` counts those forwards,
+    so a tape of an earlier one is known to be stale."""
 
     batch: BatchSolver
     checkpoints: list
     histories: list
+    generation: int = 1
     bufs: list = field(default_factory=list)
     rev: Optional[_Reverse] = None
     rplan: Optional[object] = None
@@ -115,6 +120,10 @@ class BatchAdjointProblem:
             raise ValueError("one permittivity array per candidate")
         if self.reuse and self._session is not None:
             return self._forward_again(eps_list)
+        return self._forward_new(eps_list, session=self.reuse)
+
+    def _forward_new(self, eps_list: Sequence[np.ndarray], session: bool) -> BatchTape:
+        """The forward pass on a new batch; ``session``: keep it for a design loop."""
         sims = [p.with_permittivity(e) for p, e in zip(self.problems, eps_list)]
         batch = BatchSolver(sims, tune=self.tune)
         for p, s in zip(self.problems, batch.solvers):
@@ -122,7 +131,7 @@ class BatchAdjointProblem:
         N, K = self.n_steps, self.K
         t0 = time.perf_counter()
         plan = batch._ensure_plan()
-        tape = BatchTape(batch=batch)
+        tape = BatchTape(batch=batch, eps=[np.array(e, dtype=np.float64) for e in eps_list])
         n_ck = (N + K - 1) // K
         for k, (p, s, ids) in enumerate(zip(self.problems, batch.solvers, batch._ids)):
             delta, phasors = s._tables(0, N)
@@ -148,9 +157,10 @@ class BatchAdjointProblem:
         batch.backend.synchronize()
         tape.forward_s = time.perf_counter() - t0
         tape.J = [p._objective(s) for p, s in zip(self.problems, batch.solvers)]
-        if self.reuse:
+        if session:
             self._session = _BatchSession(batch=batch, checkpoints=tape.checkpoints,
                                           histories=tape.histories)
+            tape.generation = self._session.generation
         return tape
 
     def _advance_forward(self, plan) -> None:
@@ -170,20 +180,29 @@ class BatchAdjointProblem:
         t0 = time.perf_counter()
         for p, s, e in zip(self.problems, batch.solvers, eps_list):
             p.set_design(s, e)
+        ses.generation += 1
         self._advance_forward(batch._ensure_plan())
         for s in batch.solvers:
             s.n = self.n_steps
         batch.n = self.n_steps
         batch.backend.synchronize()
-        tape = BatchTape(batch=batch, checkpoints=ses.checkpoints, histories=ses.histories)
+        tape = BatchTape(batch=batch, checkpoints=ses.checkpoints, histories=ses.histories,
+                         eps=[np.array(e, dtype=np.float64) for e in eps_list],
+                         generation=ses.generation)
         tape.forward_s = time.perf_counter() - t0
         tape.J = [p._objective(s) for p, s in zip(self.problems, batch.solvers)]
         return tape
 
     def backward(self, tape: BatchTape) -> list:
-        """∂J/∂ε per design pixel (float64) of every candidate."""
+        """∂J/∂ε per design pixel (float64) of every candidate.
+
+        A tape of the design-loop session whose state a later forward has overwritten
+        is replayed: its forward runs again at its own permittivities on a new batch,
+        and the backward pass runs on that."""
         ses = self._session
         if self.reuse and ses is not None and tape.batch is ses.batch:
+            if tape.generation != ses.generation:
+                return self.backward(self._forward_new(tape.eps, session=False))
             return self._backward_session(tape)
         solvers = tape.batch.solvers
         rev, rplan, co, bufs = self._reverse_plans(tape)

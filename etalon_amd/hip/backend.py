@@ -41,9 +41,13 @@ def _precision_of(dtype: np.dtype) -> int:
 
 
 class _Buffer:
-    """One device allocation; freed when the last array using it goes away."""
+    """One device allocation; freed when the last array using it goes away.
 
-    __slots__ = ("ptr", "nbytes", "__weakref__")
+    ``writes`` counts host writes into it (``DeviceArray.set`` and item assignment,
+    through any view): a fused plan records the counts of its state buffers and
+    re-checks its preconditions when one has changed."""
+
+    __slots__ = ("ptr", "nbytes", "writes", "__weakref__")
 
     def __init__(self, nbytes: int):
         lib = _lib.init(_lib._device or 0)
@@ -51,6 +55,7 @@ class _Buffer:
         _lib.call("etl_malloc", C.byref(p), max(int(nbytes), 1))
         self.ptr = p.value
         self.nbytes = int(nbytes)
+        self.writes = 0
         weakref.finalize(self, _free, lib, self.ptr)
 
 
@@ -109,10 +114,16 @@ class DeviceArray:
         return DeviceArray((r1 - r0, *self.shape[1:]), self.dtype, self._buf,
                            self._offset + r0 * stride)
 
+    @property
+    def writes(self) -> int:
+        """Host writes into this array's buffer so far (any view of it)."""
+        return self._buf.writes
+
     def set(self, host: Any) -> None:
         host = np.ascontiguousarray(host, dtype=self.dtype)
         if host.shape != self.shape:
             host = np.ascontiguousarray(np.broadcast_to(host, self.shape))
+        self._buf.writes += 1
         if self.nbytes:
             _lib.call("etl_memcpy_h2d", C.c_void_p(self.ptr), host.ctypes.data, self.nbytes)
 

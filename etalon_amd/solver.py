@@ -100,6 +100,7 @@ class FDTDSolver:
         self.tune = tune
         self._plan = None
         self._plan_ids: list[int] = []
+        self._plan_marks: tuple = ()
         self._plan_tables = (0, 0)
 
     # ---------------------------------------------------------------- one step
@@ -150,14 +151,27 @@ class FDTDSolver:
         from etalon_amd.hip.plan import FusedPlan
 
         plan = self._plan
-        if plan is not None and all(getattr(self.state, name) is plan.pairs[name][plan.parity]
-                                    for name in FIELDS):
+        marks = self._write_marks()
+        if (plan is not None and marks == self._plan_marks
+                and all(getattr(self.state, name) is plan.pairs[name][plan.parity]
+                        for name in FIELDS)):
             return plan
+        # New state arrays, or values written into them from the host since the plan
+        # checked them: a new plan checks again that psi is +0 outside the CPML strips
+        # and that no field holds -0 (else ETL_ESTATE, and advance() steps instead).
+        self._plan = None
+        if plan is not None:
+            plan.close()
         plan = FusedPlan(self.state, self.tune)
         self._plan_ids = self.register(plan, 0)
         self._plan = plan
+        self._plan_marks = marks
         self._plan_tables = (0, 0)  # steps [a, b) the plan's tables cover
         return plan
+
+    def _write_marks(self) -> tuple:
+        """Host-write counts of the state's field buffers (see DeviceArray.writes)."""
+        return tuple(getattr(getattr(self.state, name), "writes", 0) for name in FIELDS)
 
     def register(self, plan, sim: int) -> list[int]:
         """Give simulation ``sim`` of a fused plan this solver's sources and monitors;
@@ -253,9 +267,13 @@ class FDTDSolver:
                 # (after reset) reuses what the plan holds
                 rows = min(count - done, have[1] - self.n)
             else:
-                rows = min(count - done, grow)
+                # at least _FIRST_TABLE_ROWS rows: short advances that follow (a step
+                # loop of advance(1), a timed region after a warm-up) then find their
+                # tables on the device and upload nothing
+                staged = min(grow, max(count - done, _FIRST_TABLE_ROWS))
+                rows = min(count - done, staged)
                 grow = min(4 * grow, _PLAN_TABLE_ROWS)
-                self._stage_tables(plan, self.n, rows)
+                self._stage_tables(plan, self.n, staged)
                 refills += 1
             plan.advance(self.n, rows)
             plan.bind()

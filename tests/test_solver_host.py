@@ -60,15 +60,33 @@ def test_refills_grow_and_stage_the_whole_range():
     assert plan.calls == [("advance", 0, 10000)]
 
 
-def test_one_piece_needs_no_second_table():
+def test_short_advances_stage_ahead():
+    """A refill covers at least _FIRST_TABLE_ROWS steps, so the short advances that
+    follow (a step loop of small advances, a timed region after a warm-up) upload
+    nothing until they pass its end."""
     plan = RecordingPlan()
     s = make_solver(plan)
+    first = solver_mod._FIRST_TABLE_ROWS
     s.advance(100)
-    assert plan.calls == [("delta", 0, 100), ("phasor", 7, 0, 100), ("advance", 0, 100)]
-    assert s._plan_tables == (0, 100)
+    assert plan.calls == [("delta", 0, first), ("phasor", 7, 0, first), ("advance", 0, 100)]
+    assert s._plan_tables == (0, first)
     plan.calls.clear()
-    s.advance(50)  # beyond the held table: a new piece from step 100
-    assert plan.calls == [("delta", 100, 50), ("phasor", 7, 100, 50), ("advance", 100, 50)]
+    s.advance(50)  # within the held table: no refill
+    assert plan.calls == [("advance", 100, 50)]
+    plan.calls.clear()
+    s.advance(100)  # the rest of the held table, then a new piece from its end
+    assert plan.calls == [("advance", 150, first - 150), ("delta", first, first),
+                          ("phasor", 7, first, first), ("advance", first, 250 - first)]
+
+
+def test_warm_up_then_timed_steps_upload_nothing():
+    """bench.py's pattern: W warm-up steps, then K timed ones within the first table."""
+    plan = RecordingPlan()
+    s = make_solver(plan)
+    s.advance(5)
+    plan.calls.clear()
+    s.advance(20)
+    assert plan.calls == [("advance", 5, 20)]
 
 
 def test_long_runs_cap_the_piece_and_stage_nothing_larger():
