@@ -131,6 +131,12 @@ class DeviceArray:
         a = self.get()
         return a if dtype is None else a.astype(dtype)
 
+    def astype(self, dtype) -> np.ndarray:
+        """A host copy in ``dtype`` (``xp`` is NumPy: host code that does arithmetic on
+        device arrays, such as etalon/studio/runner.py:258-280, works on the copy;
+        etalon_amd.observe reduces on the device instead)."""
+        return self.get().astype(dtype)
+
     def copy(self) -> "DeviceArray":
         out = DeviceArray(self.shape, self.dtype)
         _lib.call("etl_memcpy_d2d", C.c_void_p(out.ptr), C.c_void_p(self.ptr), self.nbytes)

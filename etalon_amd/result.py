@@ -116,3 +116,79 @@ def monitor_data(sampling, acc: np.ndarray, z: float) -> MonitorData:
         "length_units": "um",
         "time_convention": "exp(-i omega t)",
     })
+
+
+def monitor_geometry(size) -> str:
+    """How many axes of a monitor have size > 0: point, line, plane or box
+    (etalon/studio/runner.py:359-362)."""
+    return ("point", "line", "plane", "box")[sum(1 for s in size if s > 0.0)]
+
+
+def monitor_layout(sim, result: SimulationResult) -> tuple[dict[str, np.ndarray], list[dict]]:
+    """The arrays of ``result.npz`` and the monitor list of ``meta.json``
+    (etalon/studio/runner.py:314-356).
+
+    Keys: ``m<i>.<C>`` complex data shaped (n_x, n_y, n_wavelengths), with n = 1 along
+    an interpolated axis; ``m<i>.<C>.x`` and ``.y`` its node coordinates (µm, float64);
+    ``m<i>.wavelengths`` (µm).  ``i`` is the monitor's index in ``sim.monitors``.  Each
+    list entry: name, index, geometry, components, wavelengths_um, and for lines and
+    planes the first component's (n_x, n_y) shape and [x0, x1, y0, y1] extent.
+    """
+    arrays: dict[str, np.ndarray] = {}
+    monitors = []
+    for i, monitor in enumerate(sim.monitors):
+        components = result[monitor.name]
+        first = None
+        for component, data in components.items():
+            values = np.asarray(data.values)
+            if "x" not in data.dims:
+                values = values[np.newaxis, ...]
+            if "y" not in data.dims:
+                values = values[:, np.newaxis, ...]
+            x = np.atleast_1d(np.asarray(data.x, dtype=np.float64))
+            y = np.atleast_1d(np.asarray(data.y, dtype=np.float64))
+            arrays[f"m{i}.{component}"] = values
+            arrays[f"m{i}.{component}.x"] = x
+            arrays[f"m{i}.{component}.y"] = y
+            if first is None:
+                first = ([int(values.shape[0]), int(values.shape[1])],
+                         [float(x[0]), float(x[-1]), float(y[0]), float(y[-1])])
+        arrays[f"m{i}.wavelengths"] = np.asarray(monitor.wavelengths, dtype=np.float64)
+        geometry = monitor_geometry(monitor.size)
+        point = geometry == "point"
+        monitors.append({
+            "name": monitor.name,
+            "index": i,
+            "geometry": geometry,
+            "components": list(components),
+            "wavelengths_um": [float(w) for w in monitor.wavelengths],
+            "shape": None if point or first is None else first[0],
+            "extent": None if point or first is None else first[1],
+        })
+    return arrays, monitors
+
+
+def save_result(folder, sim, result: SimulationResult) -> dict:
+    """Write ``result.npz`` (the arrays of ``monitor_layout``, replaced atomically) and
+    ``meta.json`` (steps, wall time, cell-updates/s, the monitor list) into ``folder``,
+    as a finished run's folder holds them (etalon/studio/runner.py:555-570); returns
+    the metadata written."""
+    import json
+    import os
+    from pathlib import Path
+
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    arrays, monitors = monitor_layout(sim, result)
+    tmp = folder / "result.tmp.npz"
+    np.savez(tmp, **arrays)
+    os.replace(tmp, folder / "result.npz")
+    m = result.metadata
+    wall = m["wall_time_s"]
+    meta = {"steps": int(m["n_steps"]),
+            "wall_time_s": float(wall) if wall is not None and np.isfinite(wall) else None,
+            "cells_per_s": m["cell_updates_per_s"], "backend": m["backend"],
+            "device": m["device"], "precision": m["precision"], "monitors": monitors,
+            "has_result": True}
+    (folder / "meta.json").write_text(json.dumps(meta, indent=1), encoding="utf-8")
+    return meta

@@ -105,6 +105,24 @@ int etl_scatter_sub(int precision, void* field, int64_t pitch, const int64_t* i,
 int etl_gather(int precision, void* out, const void* field, int64_t pitch, const int64_t* i,
                const int64_t* j, int n);
 
+/* ------------------------------------------------------------------ run observers
+ * On-device reductions for progress reports, replacing the device-side xp.sum and
+ * block mean of etalon/studio/runner.py:258-311 (field_energy, downsample). */
+
+/* out (device, 3 doubles) = { sum ex^2 / cb_ex, sum ey^2 / cb_ey, sum hz^2 } over every
+ * node, each term (double(v) * double(v)) / double(cb) as field_energy forms it
+ * (etalon/studio/runner.py:258-280); float64 sums in a fixed order (two passes of
+ * 4096-element blocks and 256-wide binary trees), equal to NumPy's to float64
+ * rounding.  Partial sums in a buffer from etl_malloc.  Stream-ordered (lane 0). */
+int etl_field_energy(int precision, const void* ex, const void* ey, const void* hz,
+                     const void* cb_ex, const void* cb_ey, int nx, int ny, double* out);
+
+/* Block means: out (device, (nx / sx) x (ny / sy) floats) [a, b] = the mean of
+ * field[a sx + p, b sy + q] over p < sx, q < sy, summed in float64 in (p, q) row-major
+ * order, divided by sx sy in float64 and rounded to float once (the snapshot block
+ * mean of downsample, etalon/studio/runner.py:283-291).  Stream-ordered (lane 0). */
+int etl_downsample(int precision, const void* field, int nx, int ny, int sx, int sy, float* out);
+
 /* ------------------------------------------------------------------ fused time stepping
  * A plan advances the state many steps per call with one fused kernel per step
  * (H half-step, M injection, E half-step, J injection, monitor trace) and a DFT

@@ -103,20 +103,31 @@ def gradient(sim, names, design) -> tuple[float, np.ndarray]:
     """(J, ∂J/∂ε per design pixel) in float64 by the discrete adjoint.
 
     ``design`` is the PixelGrid structure (the last of ``sim.structures``); its
-    pixels map to the Ex/Ey nodes it paints."""
+    pixels map to the Ex/Ey nodes it paints.  The forward increments u, w that
+    ∂J/∂cb needs are kept only on the bounding box of those nodes (every step's,
+    for the reverse sweep): λ_Ex' and λ_Ey' of step n are λ before the transpose of
+    step n, so ∂J/∂cb accumulates there, before ``step_transpose``."""
     if sim.precision != "double":
         raise ValueError("the oracle gradient runs in float64")
     fwd = ref.Solver(sim)
     s = fwd.s
     n_steps = fwd.n_steps
+    nodes = pixel_map(fwd.g, design)
+    # bounding boxes of the design's Ex and Ey nodes: rows [r0, r1), columns [c0, c1)
+    box = {c: (int(ii.min()), int(ii.max()) + 1, int(jj.min()), int(jj.max()) + 1)
+           for c, (ii, jj, _, _) in nodes.items()}
+    (er0, er1, ec0, ec1), (wr0, wr1, wc0, wc1) = box["Ex"], box["Ey"]
+    if ec0 < 1 or ec1 > s.ny or wr0 < 1 or wr1 > s.nx:
+        raise ValueError("the design region touches the outer tangential-E edge")
     us, ws = [], []
     for _ in range(n_steps):
         ref.update_h(s)
         fwd._inject(True)
         fwd._accumulate(True)
         u, w = e_increments(s)
-        us.append(u)
-        ws.append(w)
+        # u covers Ex columns 1..ny-1 (column j at j - 1), w Ey rows 1..nx-1
+        us.append(u[er0:er1, ec0 - 1:ec1 - 1].copy())
+        ws.append(w[wr0 - 1:wr1 - 1, wc0:wc1].copy())
         ref.update_e(s)
         fwd._inject(False)
         fwd._accumulate(False)
@@ -133,7 +144,8 @@ def gradient(sim, names, design) -> tuple[float, np.ndarray]:
         A = G @ W                                # (n_f, n_nodes)
         srcs.append((m, A))
     lam = {k: np.zeros_like(getattr(s, k)) for k in STATE}
-    grad = {"cb_ex": np.zeros_like(s.cb_ex), "cb_ey": np.zeros_like(s.cb_ey)}
+    g_ex = np.zeros((er1 - er0, ec1 - ec0))
+    g_ey = np.zeros((wr1 - wr0, wc1 - wc0))
     for n in range(n_steps - 1, -1, -1):
         for m, A in srcs:
             t = (n + 0.5) * dt if m.magnetic else (n + 1.0) * dt
@@ -143,14 +155,17 @@ def gradient(sim, names, design) -> tuple[float, np.ndarray]:
             np.add.at(flat, m.flat, val)
         # sources of step n go to λ of its outputs (E' and Hz'), which the transpose
         # reads: E monitors before its E part, H monitors before its H part
-        step_transpose(s, lam, grad, us[n], ws[n])
+        g_ex += lam["ex"][er0:er1, ec0:ec1] * us[n]
+        g_ey -= lam["ey"][wr0:wr1, wc0:wc1] * ws[n]
+        step_transpose(s, lam)
     eps = fwd.eps
-    g_eps = {"Ex": -(s.cb_ex / eps["Ex"]) * grad["cb_ex"],
-             "Ey": -(s.cb_ey / eps["Ey"]) * grad["cb_ey"]}
+    grads = {"Ex": (g_ex, er0, ec0, s.cb_ex), "Ey": (g_ey, wr0, wc0, s.cb_ey)}
     npx, npy = np.asarray(design.permittivity).shape
     out = np.zeros((npx, npy))
-    for comp, (ii, jj, p, q) in pixel_map(fwd.g, design).items():
-        np.add.at(out, (p, q), g_eps[comp][ii, jj])
+    for comp, (ii, jj, p, q) in nodes.items():
+        g, r0, c0, cb = grads[comp]
+        g_eps = -(cb[ii, jj] / eps[comp][ii, jj]) * g[ii - r0, jj - c0]
+        np.add.at(out, (p, q), g_eps)
     return J, out
 
 

@@ -91,3 +91,41 @@ def test_point2d_matches_meep():
     err = float(np.max(np.abs(t21 - ref["t21"])))
     print(f"max |dT21| = {err:.3e}")
     assert err < 1e-3
+
+
+def test_m0_values_reproduce_the_reference_record():
+    """The values bench/validate_m0.py records (:214, :245-252), against the ones the
+    reference measured on its CUDA backend (bench/results/b2ff355.json:8-75, fixture
+    tests/golden/m0_record.json) to 1e-6 relative: the kernels' arithmetic is the
+    reference's, and host transcendentals differ by ulps across NumPy builds (measured
+    on MI355X: 14 of the 16 values equal to every printed digit, the largest difference
+    9e-8, in the minimum v/c at dl = 0.025)."""
+    import json
+
+    record = json.loads((GOLDEN / "m0_record.json").read_text())
+    got = {}
+    for dl in (0.1, 0.05, 0.025):
+        for c in m0.dispersion_cases(dl):
+            got[("dispersion", c.dl, c.angle)] = (
+                float(np.max(np.abs(c.v_measured / c.v_theory - 1))), float(np.min(c.v_measured)))
+    reflection = m0.cpml_reflection_case()
+    late = m0.late_time_case()
+    green = m0.green_function_case(dl=0.05)
+    checks = []
+    for d in record["dispersion"]:
+        residual, v_min = got[("dispersion", d["dl_um"], d["angle_deg"])]
+        checks.append((f"dispersion dl={d['dl_um']} angle={d['angle_deg']} residual", residual,
+                       d["max_abs_residual_vs_yee"]))
+        checks.append((f"dispersion dl={d['dl_um']} angle={d['angle_deg']} min v/c", v_min,
+                       d["min_v_over_c"]))
+    for probe, error in zip(reflection.probes, reflection.error):
+        checks.append((f"cpml {probe}", float(error), record["cpml_reflection"][probe]))
+    checks.append(("late-time energy", float(late.energy[-1] / np.max(late.energy)),
+                   record["late_time_final_energy_rel"]))
+    checks.append(("green", float(np.abs(green.hz_measured / green.hz_analytic - 1).max()),
+                   record["green_function_max_rel_error"]))
+    assert len(checks) == 2 * 6 + 2 + 2
+    for label, value, want in checks:
+        print(f"{label}: {value:.6e} (recorded {want:.6e}, rel {abs(value / want - 1):.1e})")
+    for label, value, want in checks:
+        assert value == pytest.approx(want, rel=1e-6), label

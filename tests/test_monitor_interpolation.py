@@ -79,6 +79,22 @@ def test_oracle_single_precision_casts_back():
     assert point == pytest.approx(expected, rel=1e-6, abs=0)
 
 
+def test_oracle_vacuum_pulse_probes_equal_plane_interpolation():
+    """The 184 x 184 vacuum pulse in float64 on the oracle: every component of its four
+    probes against the bilinear interpolation of a plane over the interior, to 1e-12
+    of the plane's largest value (see the GPU twin below for why not of the value)."""
+    base = vacuum_pulse("double")
+    plane = et.DFTMonitor(name="plane", size=(8.0, 8.0, 0.0), wavelengths=[1.45])
+    probes = [m.model_copy(update={"wavelengths": [1.45]}) for m in base.monitors]
+    solver = ref.Solver(base.model_copy(update={"monitors": [*probes, plane]})).run()
+    values = solver.monitor_values()
+    pairs = _oracle_pairs(solver, {m.name for m in probes})
+    assert len(pairs) == 12
+    for (name, comp), (point, expected) in pairs.items():
+        scale = np.abs(values[("plane", comp)]).max()
+        assert abs(point - expected) <= 1e-12 * scale, (name, comp)
+
+
 def test_planted_weight_swap_is_caught(monkeypatch):
     """A plausible mistake in the sampling weights — (fr, 1 - fr) for (1 - fr, fr) —
     planted in the oracle makes the check fail."""
@@ -97,6 +113,7 @@ def test_planted_weight_swap_is_caught(monkeypatch):
 
 
 def _hip_pairs(result, names, components):
+    """{(point, comp): (point value, plane interpolated there, max |plane value|)}."""
     out = {}
     for name in names:
         for comp in components:
@@ -104,7 +121,8 @@ def _hip_pairs(result, names, components):
             plane = result["plane"][comp].isel(wavelength=0)
             out[(name, comp)] = (complex(pt.values[0]),
                                  bilinear(plane.x, plane.y, plane.values, float(pt.x),
-                                          float(pt.y)))
+                                          float(pt.y)),
+                                 float(np.abs(plane.values).max()))
     return out
 
 
@@ -115,7 +133,7 @@ def test_hip_point_equals_plane_interpolation(hip, components):
 
     result = FDTDSolver(_sim("double", components)).run()
     assert result["point0"][components[0]].dims == ("wavelength",)
-    for key, (point, expected) in _hip_pairs(result, ["point0"], components).items():
+    for key, (point, expected, _) in _hip_pairs(result, ["point0"], components).items():
         assert abs(expected) > 0, key
         assert point == pytest.approx(expected, rel=1e-12, abs=0), key
 
@@ -125,7 +143,11 @@ def test_hip_point_equals_plane_interpolation(hip, components):
 def test_hip_vacuum_pulse_probes_equal_plane_interpolation(hip, precision):
     """The 184 x 184 vacuum pulse (2000 steps), its four probes and a plane monitor over
     the interior at the first wavelength: every component of every probe (the Hz ones
-    mid-cell) equals the bilinear interpolation of the plane."""
+    mid-cell) equals the bilinear interpolation of the plane, to ``rel`` of the plane's
+    largest value.  (Ex vanishes on the source's symmetry axis y = 0, where p0 and p1
+    sit: there only rounding is left, and the coordinate weights' offset of ~1e-15 from
+    the node weighs in the much larger neighbouring row, so a relative error of the
+    value itself means nothing.)"""
     from etalon_amd.solver import FDTDSolver
 
     base = vacuum_pulse(precision)
@@ -136,6 +158,6 @@ def test_hip_vacuum_pulse_probes_equal_plane_interpolation(hip, precision):
     rel = 1e-12 if precision == "double" else 1e-6
     pairs = _hip_pairs(result, [m.name for m in probes], ("Ex", "Ey", "Hz"))
     assert len(pairs) == 12
-    for key, (point, expected) in pairs.items():
-        assert abs(expected) > 0, key
-        assert point == pytest.approx(expected, rel=rel, abs=0), key
+    for key, (point, expected, scale) in pairs.items():
+        assert scale > 0, key
+        assert abs(point - expected) <= rel * scale, key
