@@ -151,10 +151,10 @@ class FDTDSolver:
         from etalon_amd.hip.plan import FusedPlan
 
         plan = self._plan
-        marks = self._write_marks()
-        if (plan is not None and marks == self._plan_marks
+        if (plan is not None
                 and all(getattr(self.state, name) is plan.pairs[name][plan.parity]
-                        for name in FIELDS)):
+                        for name in FIELDS)
+                and self._pair_marks(plan) == self._plan_marks):
             return plan
         # New state arrays, or values written into them from the host since the plan
         # checked them: a new plan checks again that psi is +0 outside the CPML strips
@@ -165,13 +165,16 @@ class FDTDSolver:
         plan = FusedPlan(self.state, self.tune)
         self._plan_ids = self.register(plan, 0)
         self._plan = plan
-        self._plan_marks = marks
+        self._plan_marks = self._pair_marks(plan)
         self._plan_tables = (0, 0)  # steps [a, b) the plan's tables cover
         return plan
 
-    def _write_marks(self) -> tuple:
-        """Host-write counts of the state's field buffers (see DeviceArray.writes)."""
-        return tuple(getattr(getattr(self.state, name), "writes", 0) for name in FIELDS)
+    @staticmethod
+    def _pair_marks(plan) -> tuple:
+        """Host-write counts of both buffers of every field pair of ``plan`` (see
+        DeviceArray.writes): the state lives in one of each pair, which one depends on
+        the steps run, and a host write into it bumps its count."""
+        return tuple(plan.pairs[name][b].writes for name in FIELDS for b in (0, 1))
 
     def register(self, plan, sim: int) -> list[int]:
         """Give simulation ``sim`` of a fused plan this solver's sources and monitors;

@@ -118,3 +118,17 @@ def test_g5_auto_picks_hip(hip):
     result = FDTDSolver(sim).run()
     assert result.metadata["backend"] == "hip"
     assert result.metadata["device"].startswith("hip:0")
+
+
+def test_plan_is_kept_across_advances_of_either_parity(hip):
+    """No host write, no new plan: advances that leave the state in either buffer of
+    the pairs (odd and even numbers of launches) keep the plan, its tables and graphs."""
+    sim = full_sim("single")
+    gpu, cpu = FDTDSolver(sim), ref.Solver(sim)
+    gpu.advance(7)
+    plan = gpu._plan
+    for count in (5, 13, 96, 1):
+        gpu.advance(count)
+        assert gpu._plan is plan, count
+    cpu.run(7 + 5 + 13 + 96 + 1)
+    _same(gpu, cpu)

@@ -46,23 +46,23 @@ def test_set_and_item_assignment_count_writes(no_device, monkeypatch):
     assert no_device.count("etl_memcpy_h2d") == 4
 
 
-def test_solver_write_marks_follow_the_state(no_device):
+def test_solver_plan_marks_cover_both_buffers_of_every_pair(no_device):
+    """The plan's marks change with a host write into either buffer of a field pair,
+    not with which buffer holds the state after a step block."""
     from etalon_amd.solver import FDTDSolver
     from etalon_amd.state import FIELDS
 
-    s = object.__new__(FDTDSolver)
+    class Plan:
+        pairs = {name: [_array(), _array()] for name in FIELDS}
 
-    class State:
-        pass
-
-    s.state = State()
-    for name in FIELDS:
-        setattr(s.state, name, _array())
-    before = s._write_marks()
-    s.state.psi_hz_x.set(np.zeros((4, 5)))
-    after = s._write_marks()
+    plan = Plan()
+    before = FDTDSolver._pair_marks(plan)
+    assert len(before) == 2 * len(FIELDS)
+    plan.pairs["psi_hz_x"][1].set(np.zeros((4, 5)))
+    after = FDTDSolver._pair_marks(plan)
     assert before != after
-    assert after[FIELDS.index("psi_hz_x")] == before[FIELDS.index("psi_hz_x")] + 1
+    k = 2 * FIELDS.index("psi_hz_x") + 1
+    assert after[k] == before[k] + 1 and after[:k] == before[:k] and after[k + 1:] == before[k + 1:]
 
 
 def _problem():
