@@ -50,6 +50,9 @@ int zero_async(void* dst, size_t bytes);
 constexpr size_t kGuard = 256;
 // Bytes asked for when `p` is the start of a live etl_malloc array, else 0.
 size_t guarded_bytes(const void* p);
+// Per-device set-up of the observer reductions (observe.hip), called by etl_init with
+// the device current.
+int observe_setup(int device);
 
 }  // namespace etl
 

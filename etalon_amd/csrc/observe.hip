@@ -12,7 +12,8 @@
 //   pass 2: per array, thread t adds partials t, t + 256, ... in that order, then the
 //           same tree over the 256 threads.
 // The NumPy reference sums pairwise in another order: the results agree to float64
-// rounding.  The partial sums come from the library's allocator.
+// rounding.  The partial sums are the scratch buffer: etl_field_energy takes them from
+// the library's allocator, etl_field_energy_noalloc from a fixed device array.
 #include <algorithm>
 
 #include "common.h"
@@ -23,6 +24,16 @@ namespace {
 constexpr int kEnergyThreads = 256;
 constexpr int kEnergyPerThread = 16;
 constexpr int64_t kEnergyPerBlock = int64_t(kEnergyThreads) * kEnergyPerThread;
+// The no-alloc variant's scratch: 3 arrays x this many block partials (1.5 MiB per
+// device, held until the process ends), so each array may hold up to
+// kEnergyNoallocBlocks * 4096 = 268435456 nodes.
+constexpr int kEnergyNoallocBlocks = 65536;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[3 * kEnergyNoallocBlocks];
+// Per device: the device address of `scratch` (the host-side name is only a stand-in).
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
 template <typename R>
 struct EnergyTerm {
     const R* f;
@@ -123,6 +134,21 @@ __global__ void downsample_kernel(const R* __restrict__ f, int ny, int sx, int s
 
 }  // namespace
 
+// Called by etl_init with `device` current: the device address of this file's scratch
+// on that device (each device holds its own copy of the array).
+int observe_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("etl_init: device %d is beyond the %d the observer scratch supports", device,
+                  kMaxDevices);
+        return ETL_EARG;
+    }
+    if (g_scratch_ptr[device] != nullptr) return ETL_OK;
+    void* p = nullptr;
+    ETL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+    return ETL_OK;
+}
+
 }  // namespace etl
 
 using namespace etl;
@@ -146,6 +172,28 @@ int etl_field_energy(int precision, const void* ex, const void* ey, const void* 
     // stream-ordered: a later allocation of the block is used after these kernels
     const int rc_free = etl_free(partials);
     return rc ? rc : rc_free;
+}
+
+int etl_field_energy_noalloc(int precision, const void* ex, const void* ey, const void* hz,
+                             const void* cb_ex, const void* cb_ey, int nx, int ny, double* out) {
+    ETL_CHECK_INIT();
+    if (int rc = check_energy_args(precision, ex, ey, hz, cb_ex, cb_ey, nx, ny, out,
+                                   "etl_field_energy_noalloc"))
+        return rc;
+    const int64_t largest = std::max(int64_t(nx) * (ny + 1), int64_t(nx + 1) * ny);
+    ETL_REQUIRE(blocks_of(largest) <= kEnergyNoallocBlocks,
+                "etl_field_energy_noalloc: %lld nodes in one array, at most %lld (its fixed "
+                "scratch holds %d block sums per array); use etl_field_energy",
+                (long long)largest, (long long)kEnergyNoallocBlocks * kEnergyPerBlock,
+                kEnergyNoallocBlocks);
+    double* partials = g_scratch_ptr[ctx().device];
+    ETL_REQUIRE(partials != nullptr, "etl_field_energy_noalloc: no scratch address for device %d",
+                ctx().device);
+    // Every partial that pass 2 reads is written by pass 1 of the same call, so the
+    // previous call's contents need no clearing.
+    return precision == ETL_F32
+               ? launch_energy<float>(ex, ey, hz, cb_ex, cb_ey, nx, ny, partials, out)
+               : launch_energy<double>(ex, ey, hz, cb_ex, cb_ey, nx, ny, partials, out);
 }
 
 int etl_downsample(int precision, const void* field, int nx, int ny, int sx, int sy, float* out) {

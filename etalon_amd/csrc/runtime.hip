@@ -171,6 +171,7 @@ int etl_init(int device) {
     snprintf(c.arch, sizeof c.arch, "%s", prop.gcnArchName);
     c.cu_count = prop.multiProcessorCount;
     c.mem = prop.totalGlobalMem;
+    if (int rc = observe_setup(device)) return rc;  // with `device` current
     ETL_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.lane[0] = c.stream;
     for (int k = 1; k < kLanes; ++k)

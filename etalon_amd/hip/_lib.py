@@ -111,6 +111,7 @@ _SIGNATURES = {
     "etl_scatter_sub": [C.c_int, vp, C.c_int64, vp, vp, vp, C.c_int],
     "etl_gather": [C.c_int, vp, vp, C.c_int64, vp, vp, C.c_int],
     "etl_field_energy": [C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int, vp],
+    "etl_field_energy_noalloc": [C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int, vp],
     "etl_downsample": [C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp],
     "etl_plan_create": [C.POINTER(EtlTeDesc), C.POINTER(EtlTune), C.POINTER(vp)],
     "etl_plan_destroy": [vp],

@@ -117,6 +117,18 @@ int etl_gather(int precision, void* out, const void* field, int64_t pitch, const
 int etl_field_energy(int precision, const void* ex, const void* ey, const void* hz,
                      const void* cb_ex, const void* cb_ey, int nx, int ny, double* out);
 
+/* etl_field_energy without any allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); no reference
+ * counterpart.  The partial sums live in a fixed device array of the library (1.5 MiB
+ * per device, held until the process ends), so each of ex, ey, hz may have at most
+ * 268435456 nodes (ETL_EARG beyond).  Same kernels and reduction order as
+ * etl_field_energy: the same bits.  Every call on a device shares that array and
+ * leaves its partials there; calls are issued on lane 0 and run in its order, and
+ * calls that overlap on one GPU (the same work replayed from a captured graph on
+ * another stream while one runs) race. */
+int etl_field_energy_noalloc(int precision, const void* ex, const void* ey, const void* hz,
+                             const void* cb_ex, const void* cb_ey, int nx, int ny, double* out);
+
 /* Block means: out (device, (nx / sx) x (ny / sy) floats) [a, b] = the mean of
  * field[a sx + p, b sy + q] over p < sx, q < sy, summed in float64 in (p, q) row-major
  * order, divided by sx sy in float64 and rounded to float once (the snapshot block
