@@ -53,6 +53,9 @@ size_t guarded_bytes(const void* p);
 // Per-device set-up of the observer reductions (observe.hip), called by etl_init with
 // the device current.
 int observe_setup(int device);
+// Size the scratch of stream s's queue for the library's largest kernel private segment
+// (joint.h, warm_scratch_fwd / warm_scratch_pair); called by etl_init for each lane.
+int warm_scratch(hipStream_t s);
 
 }  // namespace etl
 

@@ -235,6 +235,35 @@ int launch_joint_t(const FwdSet<R>* f, const AdjSet<R>* a, hipStream_t s) {
     return ETL_OK;
 }
 
+// A dispatch of one joint kernel variant over `groups` workgroups and no tiles: every
+// workgroup finds no tile of its range and returns.  A queue's scratch is sized for a
+// kernel's private segment (register spills) at the first dispatch that needs more than
+// the queue holds, and that dispatch waits for the allocation; warm_scratch runs these
+// at set-up so that the wait falls outside any timed or captured work.
+template <typename R, int TF, int TA, int PART, int JB = 8>
+int launch_joint_empty(int groups, hipStream_t s) {
+    FwdSet<R> fz{};
+    AdjSet<R> az{};
+    hipLaunchKernelGGL((joint_kernel<R, TF, TA, PART, JB>), dim3(unsigned(groups)), dim3(64), 0,
+                       s, fz, az);
+    ETL_LAUNCHED();
+    return ETL_OK;
+}
+
+// Empty dispatches (launch_joint_empty), on stream s, of the variants with the largest
+// private segments: the double forward kernel of depth 6 (joint_fwd.hip, 580 bytes per
+// lane) and the float forward + adjoint pair of depth 6 (joint_pair_f32.hip, 572), over
+// enough workgroups to fill every CU.
+// An empty one-workgroup dispatch of the forward variant of depth tf (every tile kind,
+// interior tiles of `band` rows): a plan runs these at creation so that no kernel is
+// dispatched for the first time inside its steps.
+template <typename R>
+int launch_joint_fwd_empty(int tf, int band, hipStream_t s);
+
+constexpr int kWarmGroups = 8192;  // 32 one-wave workgroups per CU of 256
+int warm_scratch_fwd(hipStream_t s);
+int warm_scratch_pair(hipStream_t s);
+
 // launch_joint_t at a run-time part.
 template <typename R, int TF, int TA, int JB = 8>
 int launch_joint_part(const FwdSet<R>* f, const AdjSet<R>* a, int part, hipStream_t s) {

@@ -33,4 +33,43 @@ int launch_joint_fwd<double>(const FwdSet<double>* f, int tf, int part, hipStrea
     return launch_joint_fwd_any<double>(f, tf, part, s);
 }
 
+template <typename R>
+int launch_joint_fwd_empty_any(int tf, int band, hipStream_t s) {
+    if (band == 8) {
+        switch (tf) {
+            case 1: return launch_joint_empty<R, 1, 0, kAllKinds, 8>(1, s);
+            case 2: return launch_joint_empty<R, 2, 0, kAllKinds, 8>(1, s);
+            case 4: return launch_joint_empty<R, 4, 0, kAllKinds, 8>(1, s);
+            case 6: return launch_joint_empty<R, 6, 0, kAllKinds, 8>(1, s);
+        }
+    } else if constexpr (sizeof(R) == 4) {
+        if (band == 16) switch (tf) {
+                case 1: return launch_joint_empty<R, 1, 0, kAllKinds, 16>(1, s);
+                case 2: return launch_joint_empty<R, 2, 0, kAllKinds, 16>(1, s);
+                case 4: return launch_joint_empty<R, 4, 0, kAllKinds, 16>(1, s);
+                case 6: return launch_joint_empty<R, 6, 0, kAllKinds, 16>(1, s);
+            }
+    }
+    set_error("joint step: forward depth %d, band %d not compiled", tf, band);
+    return ETL_EARG;
+}
+
+template <>
+int launch_joint_fwd_empty<float>(int tf, int band, hipStream_t s) {
+    return launch_joint_fwd_empty_any<float>(tf, band, s);
+}
+template <>
+int launch_joint_fwd_empty<double>(int tf, int band, hipStream_t s) {
+    return launch_joint_fwd_empty_any<double>(tf, band, s);
+}
+
+int warm_scratch_fwd(hipStream_t s) {
+    return launch_joint_empty<double, 6, 0, kAllKinds, 8>(kWarmGroups, s);
+}
+
+int warm_scratch(hipStream_t s) {
+    if (int rc = warm_scratch_fwd(s)) return rc;
+    return warm_scratch_pair(s);
+}
+
 }  // namespace etl

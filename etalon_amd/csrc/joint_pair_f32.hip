@@ -26,4 +26,8 @@ int launch_joint_pair<float>(const FwdSet<float>* f, const AdjSet<float>* a, int
     return ETL_EARG;
 }
 
+int warm_scratch_pair(hipStream_t s) {
+    return launch_joint_empty<float, 6, 6, kAllKinds, 8>(kWarmGroups, s);
+}
+
 }  // namespace etl

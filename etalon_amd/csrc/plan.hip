@@ -1027,6 +1027,15 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     if (rc == ETL_OK) rc = etl_malloc((void**)&p->counter, 256);
     if (rc == ETL_OK) rc = etl_malloc(&p->args_dev, args_bytes);
     if (rc == ETL_OK) rc = etl_malloc(&p->trace_sims_dev, trace_bytes);
+    // the depths its step blocks and tails launch, dispatched once empty (joint.h)
+    if (rc == ETL_OK && p->tune.launch == 1)
+        for (int d : {p->tune.temporal, 4, 2, 1})
+            if (rc == ETL_OK && d <= p->tune.temporal &&
+                tile_shape_supported(precision, 1, d, p->tune.band_rows))
+                rc = precision == ETL_F32
+                         ? launch_joint_fwd_empty<float>(d, p->tune.band_rows, lane_stream(p->lane))
+                         : launch_joint_fwd_empty<double>(d, p->tune.band_rows,
+                                                          lane_stream(p->lane));
     if (rc != ETL_OK) {
         destroy_plan(p);
         return rc;

@@ -179,7 +179,11 @@ int etl_init(int device) {
     for (int k = 0; k < kLanes; ++k)
         ETL_HIP(hipEventCreateWithFlags(&c.lane_ev[k], hipEventDisableTiming));
     c.device = device;
-    return ETL_OK;
+    // every lane's queue gets the scratch of the largest kernel now, not at the first
+    // dispatch of a spilling kernel in a run
+    for (int k = 0; k < kLanes; ++k)
+        if (int rc = warm_scratch(c.lane[k])) return rc;
+    return sync_lanes();
 }
 
 int etl_device_info(char* buf, size_t n) {
