@@ -109,17 +109,77 @@ __device__ __forceinline__ void store_masked(global_t<R>* __restrict__ p, const 
         if (m[k]) p[c + k] = src[k];
 }
 
+// Row accesses as a wave-uniform pointer `rowcs` to element cs of a row (the window's
+// first column) plus a per-lane byte offset from it that is the same for every row.
+// (Keeping the pointer whole in SGPRs, so that loads take the saddr form, was measured
+// slower: the scalar 64-bit adds per row cost more than the vector adds they replace.)
+template <typename T>
+__device__ __forceinline__ const global_t<T>* at_byte(const global_t<T>* p, uint32_t b) {
+    return reinterpret_cast<const global_t<T>*>(reinterpret_cast<const global_t<char>*>(p) + b);
+}
+template <typename T>
+__device__ __forceinline__ global_t<T>* at_byte(global_t<T>* p, uint32_t b) {
+    return reinterpret_cast<global_t<T>*>(reinterpret_cast<global_t<char>*>(p) + b);
+}
+
+// ldv with the row pointer and offset split: `boff` = (min(c, limit - 1) - cs) *
+// sizeof(R) >= 0, so the load is the one ldv issues.
+template <typename R, int V>
+__device__ __forceinline__ void ldr(R (&dst)[V], const global_t<R>* rowcs, uint32_t boff) {
+    const global_t<R>* p = at_byte(rowcs, boff);
+    if constexpr (V == 1) {
+        dst[0] = p[0];
+    } else {
+        const Vec<R, V> x = *reinterpret_cast<const global_t<Vec<R, V>>*>(p);
+#pragma unroll
+        for (int k = 0; k < V; ++k) dst[k] = x[k];
+    }
+}
+
+// store_row with the row pointer and offset split: `soff` = (c - cs) * sizeof(R), the
+// lane's first element; the predicates on c are store_row's.
+template <typename R, int V>
+__device__ __forceinline__ void str(global_t<R>* rowcs, uint32_t soff, const R (&src)[V], int c,
+                                    int lo, int hi) {
+    global_t<R>* p = at_byte(rowcs, soff);
+    if (c >= lo && c + V <= hi) {
+        if constexpr (V == 1) {
+            p[0] = src[0];
+        } else {
+            Vec<R, V> x;
+#pragma unroll
+            for (int k = 0; k < V; ++k) x[k] = src[k];
+            *reinterpret_cast<global_t<Vec<R, V>>*>(p) = x;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+            if (c + k >= lo && c + k < hi) p[k] = src[k];
+    }
+}
+
+template <typename R, int V>
+__device__ __forceinline__ void str_masked(global_t<R>* rowcs, uint32_t soff, const R (&src)[V],
+                                           const bool (&m)[V]) {
+    global_t<R>* p = at_byte(rowcs, soff);
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+        if (m[k]) p[k] = src[k];
+}
+
 // v of lane + 1 (wave_shl:1) and of lane - 1 (wave_shr:1): DPP moves inside the VALU,
-// no LDS round trip.  Lanes 63 and 0 receive a value outside the window.
+// no LDS round trip.  Lanes 63 and 0 receive a value outside the window (0: bound_ctrl),
+// which nothing owned depends on; with bound_ctrl the move folds into the VALU op that
+// consumes it.
 template <int CTRL, typename R>
 __device__ __forceinline__ R lane_shift(R v) {
     if constexpr (sizeof(R) == 4) {
         return __int_as_float(
-            __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+            __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
     } else {
         const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_update_dpp(0, int(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xf, 0xf, false);
+        const int lo = __builtin_amdgcn_update_dpp(0, int(b & 0xffffffffLL), CTRL, 0xf, 0xf, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xf, 0xf, true);
         return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
     }
 }
@@ -246,6 +306,20 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     const G* __restrict__ cbx = to_global(a.cb_ex);
     const G* __restrict__ cby = to_global(a.cb_ey);
 
+    // Per-lane byte offsets from element cs of a row, the same for every row: of a
+    // clamped window element (loads; arrays of pitch px and of pitch ny) and of the
+    // lane's own element (stores).
+    const uint32_t offx = uint32_t(min(c, px - 1) - cs) * uint32_t(sizeof(R));
+    const uint32_t offy = uint32_t(min(c, ny - 1) - cs) * uint32_t(sizeof(R));
+    const uint32_t soff = uint32_t(lane * V) * uint32_t(sizeof(R));
+    // element cs of row r of an array of pitch px / ny (wave-uniform)
+    auto rx = [&](auto* base, int r) __attribute__((always_inline)) {
+        return base + (int64_t(r) * px + cs);
+    };
+    auto ry = [&](auto* base, int r) __attribute__((always_inline)) {
+        return base + (int64_t(r) * ny + cs);
+    };
+
     // ---- one burst of loads: the whole window ----
     constexpr int PY = SY ? RW : 1;  // psi rows held (none in interior tiles)
     constexpr int PX = SX ? RW : 1;
@@ -256,21 +330,21 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
         const int g = g0 + l;
         const int gx = min(max(g, 0), nx - 1);  // rows of Ex, Hz, cb_ex, psi_hz, psi_ex
         const int ge = min(max(g, 0), nx);      // rows of Ey, cb_ey, psi_ey
-        ldv<R, V>(ex[l], exA, gx, px, c, px);
-        ldv<R, V>(hz[l], hzA, gx, ny, c, ny);
-        ldv<R, V>(ey[l], eyA, ge, ny, c, ny);
-        ldv<R, V>(cx[l], cbx, gx, px, c, px);
-        ldv<R, V>(cy[l], cby, ge, ny, c, ny);
+        ldr<R, V>(ex[l], rx(exA, gx), offx);
+        ldr<R, V>(hz[l], ry(hzA, gx), offy);
+        ldr<R, V>(ey[l], ry(eyA, ge), offy);
+        ldr<R, V>(cx[l], rx(cbx, gx), offx);
+        ldr<R, V>(cy[l], ry(cby, ge), offy);
         if constexpr (SY) {
-            ldv<R, V>(phy[SY ? l : 0], phyA, gx, ny, c, ny);
-            ldv<R, V>(pey[SY ? l : 0], pexA, gx, px, c, px);
+            ldr<R, V>(phy[SY ? l : 0], ry(phyA, gx), offy);
+            ldr<R, V>(pey[SY ? l : 0], rx(pexA, gx), offx);
         }
         if constexpr (SX) {
-            ldv<R, V>(phx[SX ? l : 0], phxA, gx, ny, c, ny);
-            ldv<R, V>(pex[SX ? l : 0], peyA, ge, ny, c, ny);
+            ldr<R, V>(phx[SX ? l : 0], ry(phxA, gx), offy);
+            ldr<R, V>(pex[SX ? l : 0], ry(peyA, ge), offy);
         }
     }
-    ldv<R, V>(ey[RW], eyA, min(max(g0 + RW, 0), nx), ny, c, ny);
+    ldr<R, V>(ey[RW], ry(eyA, min(max(g0 + RW, 0), nx)), offy);
 
     const R db = a.db, idx = a.inv_dx, idy = a.inv_dy;
 
@@ -447,9 +521,9 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     auto store_owned = [&](int l) __attribute__((always_inline)) {
         const int g = g0 + l;
         if (g >= nx) return;
-        store_row<R, V>(hzB + int64_t(g) * ny, hz[l], c, lo, min(hi, ny));
-        store_row<R, V>(exB + int64_t(g) * px, ex[l], c, lo, min(hi, px));
-        store_row<R, V>(eyB + int64_t(g) * ny, ey[l], c, lo, min(hi, ny));
+        str<R, V>(ry(hzB, g), soff, hz[l], c, lo, min(hi, ny));
+        str<R, V>(rx(exB, g), soff, ex[l], c, lo, min(hi, px));
+        str<R, V>(ry(eyB, g), soff, ey[l], c, lo, min(hi, ny));
         if constexpr (SY) {
             bool moy[V], moe[V];
 #pragma unroll
@@ -458,17 +532,17 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                 moy[k] = own && ysh[k];
                 moe[k] = own && yse[k];
             }
-            store_masked<R, V>(phyB + int64_t(g) * ny, phy[SY ? l : 0], c, moy);
-            store_masked<R, V>(pexB + int64_t(g) * px, pey[SY ? l : 0], c, moe);
+            str_masked<R, V>(ry(phyB, g), soff, phy[SY ? l : 0], moy);
+            str_masked<R, V>(rx(pexB, g), soff, pey[SY ? l : 0], moe);
         }
         if constexpr (SX) {
             bool mo[V];
 #pragma unroll
             for (int k = 0; k < V; ++k) mo[k] = (c + k >= lo && c + k < hi) && mH[k];
             if (g < xh_lo || g >= xh_hi)
-                store_masked<R, V>(phxB + int64_t(g) * ny, phx[SX ? l : 0], c, mo);
+                str_masked<R, V>(ry(phxB, g), soff, phx[SX ? l : 0], mo);
             if (g >= 1 && (g < xe_lo || g >= xe_hi))
-                store_masked<R, V>(peyB + int64_t(g) * ny, pex[SX ? l : 0], c, mo);
+                str_masked<R, V>(ry(peyB, g), soff, pex[SX ? l : 0], mo);
         }
     };
     if constexpr (EXTRA) {
@@ -513,8 +587,8 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     // the last band copies Ey row nx (outer tangential E)
     if (i0 + BR >= nx) {
         R eyl[V];
-        ldv<R, V>(eyl, eyA, nx, ny, c, ny);
-        store_row<R, V>(eyB + int64_t(nx) * ny, eyl, c, lo, min(hi, ny));
+        ldr<R, V>(eyl, ry(eyA, nx), offy);
+        str<R, V>(ry(eyB, nx), soff, eyl, c, lo, min(hi, ny));
     }
 }
 
