@@ -40,7 +40,7 @@ from etalon_amd.grid import build_yee_grid, pixel_node_map
 from etalon_amd.hip import _lib
 from etalon_amd.hip.backend import zeros
 from etalon_amd.solver import FDTDSolver, _phasor_table
-from etalon_amd.state import FIELDS, PROFILES, cb_arrays
+from etalon_amd.state import FIELDS, PROFILES, TEState, cb_arrays
 
 _COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 
@@ -95,6 +95,24 @@ class Tape:
 
 
 @dataclass
+class _Window:
+    """The region a recompute needs: Hz nodes [i0, i0 + nx) x [j0, j0 + ny) of the grid
+    (Ex, Ey of the same rows and columns, one more column / row), the design box's
+    nodes and every node within K + 2 of them.  Over a segment of K steps from a
+    checkpoint nothing outside it reaches the box (one step moves an influence one
+    node), so the box's history recomputed on the window alone — its edges held as
+    the window grid's boundary — is the full grid's, bit for bit.  ``state`` is the
+    window's TEState (fields restored from the full checkpoints, cb and CPML profiles
+    the grid's there)."""
+
+    i0: int
+    j0: int
+    nx: int
+    ny: int
+    state: object
+
+
+@dataclass
 class _Session:
     """What a design loop keeps between gradients of one geometry.  Every forward on
     it overwrites the solver's state, monitors and checkpoints; ``generation`` counts
@@ -107,6 +125,7 @@ class _Session:
     bufs: Optional["_AdjointBuffers"] = None
     rev: Optional["_Reverse"] = None
     rplan: Optional[object] = None
+    window: Optional[_Window] = None
     co: C.c_void_p = field(default_factory=C.c_void_p)
 
     def close(self) -> None:
@@ -141,7 +160,7 @@ class _AdjointBuffers:
 class AdjointProblem:
     def __init__(self, sim, objective: Sequence[str], checkpoint_every: Optional[int] = None,
                  mode: str = "checkpoint", tune=None, kernel: str = "tile",
-                 schedule: str = "joint", reuse: bool = True):
+                 schedule: str = "joint", reuse: bool = True, window: bool = True):
         if not sim.structures or sim.structures[-1].type != "PixelGrid":
             raise ValueError("the design region must be the simulation's last structure, a "
                              "PixelGrid")
@@ -157,6 +176,9 @@ class AdjointProblem:
         if schedule not in ("joint", "lanes", "sequential"):
             raise ValueError("schedule is 'joint', 'lanes' or 'sequential'")
         self.kernel = kernel
+        # checkpoint mode: recompute each segment on the window of nodes the design box
+        # depends on over it (False: on the whole grid; the same bits)
+        self.window = bool(window)
         # checkpoint mode: how the recompute of segment k - 1 and the reverse steps of
         # segment k share the GPU (the plain kernels run them one after the other)
         self.schedule = schedule if kernel == "tile" else "sequential"
@@ -403,16 +425,19 @@ class AdjointProblem:
             ses.bufs = self._adjoint_buffers(solver, hist)
             ses.rev = _Reverse("tile", ses.bufs.desc, self.rtune)
             ses.rev.sources(comp, flat, table)
-            ses.rplan = FusedPlan(solver.state, self.rtune)
-            self._configure_recompute(ses.rplan, solver, 0, hist)
-            ses.rplan.set_checkpoints([ses.checkpoints[name] for name in FIELDS],
-                                      len(ses.starts))
+            state, tune, ses.window = self._recompute_state(solver, ses.rev)
+            ses.rplan = FusedPlan(state, tune)
+            self._configure_recompute(ses.rplan, solver, 0, hist, ses.window)
+            self._set_recompute_checkpoints(ses.rplan, ses.checkpoints, len(ses.starts), 0,
+                                            ses.window)
             _lib.call("etl_coplan_create", ses.rplan.handle, ses.rev.handle, C.byref(ses.co))
         else:
             _lib.call("etl_adjplan_reset", ses.rev.handle)
             ses.bufs.g_ex.fill_zero()
             ses.bufs.g_ey.fill_zero()
             ses.rev.sources(comp, flat, table)
+            if ses.window is not None:
+                self._refresh_window(solver, ses.window)
         _lib.call("etl_coplan_sweep", ses.co, N, K)
         solver.backend.synchronize()
         return self._pixel_gradient(solver, ses.bufs)
@@ -454,20 +479,107 @@ class AdjointProblem:
             np.add.at(out, p, g)
         return out.reshape(self.shape)
 
-    def _configure_recompute(self, plan, solver, sim: int, hist) -> None:
+    def _window_of(self, solver) -> _Window:
+        """The recompute window of ``solver``'s grid (see _Window), with its state."""
+        from etalon_amd.hip.backend import to_device
+
+        i0, j0, ni, nj = self.box
+        nx, ny = self.grid.nx, self.grid.ny
+        m = self.K + 2
+        # the box's Hz nodes are rows i0 - 1 .. i0 + ni - 1, columns j0 - 1 .. j0 + nj - 1
+        r0, r1 = max(0, i0 - 1 - m), min(nx, i0 + ni + m)
+        c0, c1 = max(0, j0 - 1 - m), min(ny, j0 + nj + m)
+        real = solver.real
+        st = solver.state
+        put = lambda a: to_device(np.ascontiguousarray(a, dtype=real))  # noqa: E731
+        cb = self._window_cb(solver, r0, r1, c0, c1)
+        prof = {}
+        for name in PROFILES:
+            v = getattr(st, name).get()
+            if "x" in name.split("_")[0]:  # bx_*, cx_*, ikx_*: along the rows
+                prof[name] = v[r0:r1 + 1] if name.endswith("_e") else v[r0:r1]
+            else:
+                prof[name] = v[c0:c1 + 1] if name.endswith("_e") else v[c0:c1]
+        shapes = {"ex": (r1 - r0, c1 - c0 + 1), "ey": (r1 - r0 + 1, c1 - c0),
+                  "hz": (r1 - r0, c1 - c0), "psi_ex_y": (r1 - r0, c1 - c0 + 1),
+                  "psi_ey_x": (r1 - r0 + 1, c1 - c0), "psi_hz_x": (r1 - r0, c1 - c0),
+                  "psi_hz_y": (r1 - r0, c1 - c0)}
+        state = TEState(**{name: zeros(shape, real) for name, shape in shapes.items()},
+                        cb_ex=put(cb["cb_ex"]), cb_ey=put(cb["cb_ey"]),
+                        **{name: put(v) for name, v in prof.items()},
+                        db=st.db, inv_dx=st.inv_dx, inv_dy=st.inv_dy, nx=r1 - r0, ny=c1 - c0)
+        return _Window(r0, c0, r1 - r0, c1 - c0, state)
+
+    @staticmethod
+    def _window_cb(solver, r0, r1, c0, c1) -> dict:
+        """cb_ex, cb_ey of the window rows [r0, r1) x columns [c0, c1] (Ex) and rows
+        [r0, r1] x columns [c0, c1) (Ey), from the solver's permittivity: the values
+        (and bits) of its full arrays there."""
+        cb = cb_arrays(solver.grid, {"Ex": solver.eps_r["Ex"][r0:r1, c0:c1 + 1],
+                                     "Ey": solver.eps_r["Ey"][r0:r1 + 1, c0:c1]})
+        return {k: np.ascontiguousarray(v, dtype=solver.real) for k, v in cb.items()}
+
+    def _refresh_window(self, solver, win: _Window) -> None:
+        """After a new design permittivity: the window's cb from the solver's."""
+        cb = self._window_cb(solver, win.i0, win.i0 + win.nx, win.j0, win.j0 + win.ny)
+        win.state.cb_ex.set(cb["cb_ex"])
+        win.state.cb_ey.set(cb["cb_ey"])
+
+    def _recompute_state(self, solver, rev, rtune=False) -> tuple:
+        """(state, tune, window or None) of the recompute plan beside reverse plan
+        ``rev``: the window's state at the reverse plan's depth, or the full state.
+        ``rtune`` (default: this problem's) is the recompute plans' tune when set."""
+        tune = self.rtune if rtune is False else rtune
+        if not self.window:
+            return solver.state, tune, None
+        win = self._window_of(solver)
+        if tune is None and rev.kernel == "tile":
+            from etalon_amd.hip.plan import Tune
+
+            tune = Tune(temporal=rev.temporal())  # step blocks that pair in a co-plan
+        return win.state, tune, win
+
+    def _configure_recompute(self, plan, solver, sim: int, hist, win=None) -> None:
         """Sources, source table and the Hz probe on the box of simulation ``sim`` of a
-        recompute plan (no monitors: the recompute only feeds the history)."""
-        comp, ii, jj = [], [], []
+        recompute plan (no monitors: the recompute only feeds the history); on a
+        window, the source nodes inside it and the checkpoints' window."""
+        comp, ii, jj, keep = [], [], [], []
+        col = 0
         for inj in solver.injections:
-            comp += [_COMP[inj.component]] * inj.i.size
-            ii += list(inj.i)
-            jj += list(inj.j)
+            c = _COMP[inj.component]
+            for i, j in zip(inj.i, inj.j):
+                i, j = int(i), int(j)
+                if win is not None:
+                    rows = win.nx + (1 if c == 1 else 0)
+                    cols = win.ny + (1 if c == 0 else 0)
+                    i, j = i - win.i0, j - win.j0
+                    if not (0 <= i < rows and 0 <= j < cols):
+                        col += 1
+                        continue  # beyond the window: cannot reach the box in a segment
+                comp.append(c)
+                ii.append(i)
+                jj.append(j)
+                keep.append(col)
+                col += 1
         plan.set_sources(np.array(comp, np.int32), np.array(ii, np.int64),
                          np.array(jj, np.int64), sim=sim)
         delta, _ = solver._tables(0, self.n_steps)
-        if delta is not None:
-            plan.source_table(0, delta, sim=sim)
-        plan.add_probe("Hz", self._box_hz_nodes(), hist, sim=sim)
+        if delta is not None and keep:
+            plan.source_table(0, np.ascontiguousarray(delta[:, keep]), sim=sim)
+        nodes = self._box_hz_nodes()
+        if win is not None:
+            i, j = np.divmod(nodes, self.grid.ny)
+            nodes = ((i - win.i0) * win.ny + (j - win.j0)).astype(np.int64)
+        plan.add_probe("Hz", nodes, hist, sim=sim)
+
+    def _set_recompute_checkpoints(self, plan, checkpoints: dict, n_ck: int, sim: int,
+                                   win=None) -> None:
+        arrays = [checkpoints[name] for name in FIELDS]
+        if win is None:
+            plan.set_checkpoints(arrays, n_ck, sim=sim)
+        else:
+            plan.set_checkpoint_window(arrays, n_ck, self.grid.nx, self.grid.ny, win.i0,
+                                       win.j0, sim=sim)
 
     def _reverse_segments(self, tape: Tape, rev, hist) -> None:
         """Recompute each segment from its checkpoint (with the Hz probe on the box),
@@ -476,9 +588,11 @@ class AdjointProblem:
 
         solver = tape.solver
         N = self.n_steps
-        plan = FusedPlan(solver.state, self.rtune)
+        state, tune, win = self._recompute_state(solver, rev)
+        plan = FusedPlan(state, tune)
         try:
-            self._configure_recompute(plan, solver, 0, hist)
+            self._configure_recompute(plan, solver, 0, hist, win)
+            self._set_recompute_checkpoints(plan, tape.checkpoints, len(tape.starts), 0, win)
             # Schedules: "joint" — the recompute of segment k - 1 and the reverse steps
             # of segment k in one launch per step block (a co-plan); "lanes" — the
             # same two on two streams; "sequential" — one after the other.  The first
@@ -494,20 +608,11 @@ class AdjointProblem:
             def count(k: int) -> int:
                 return min(self.K, N - tape.starts[k])
 
-            def restore(k: int) -> None:
-                for name in FIELDS:
-                    a = plan.pairs[name][plan.parity]
-                    ck = tape.checkpoints[name]
-                    _lib.call("etl_memcpy_d2d_lane", C.c_void_p(a.ptr),
-                              C.c_void_p(ck.ptr + k * a.nbytes), a.nbytes, lane)
-
             def recompute(k: int) -> None:
-                restore(k)
+                plan.checkpoint(k, restore=True)  # on the plan's lane
                 plan.advance(tape.starts[k], count(k), half(k))
 
             if schedule == "joint":
-                plan.set_checkpoints([tape.checkpoints[name] for name in FIELDS],
-                                     len(tape.starts))
                 co = C.c_void_p()
                 _lib.call("etl_coplan_create", plan.handle, rev.handle, C.byref(co))
                 try:
@@ -568,6 +673,14 @@ class _Reverse:
             _lib.call("etl_adjplan_sources", self.handle, int(sim), *args)
         else:
             _lib.call("etl_adjoint_sources", self.handle, *args)
+
+    def temporal(self) -> int:
+        """The temporal depth of the reverse steps' blocks (1 for the plain kernels)."""
+        if self.kernel != "tile":
+            return 1
+        t = _lib.EtlTune()
+        _lib.call("etl_adjplan_tune", self.handle, C.byref(t))
+        return int(t.temporal)
 
     def run(self, n_first: int, count: int, hist_row0: int = 0) -> None:
         if self.kernel == "tile":

@@ -62,6 +62,7 @@ class _BatchSession:
     bufs: list = field(default_factory=list)
     rev: Optional[_Reverse] = None
     rplan: Optional[object] = None
+    windows: list = field(default_factory=list)  # per candidate: its recompute window
     co: C.c_void_p = field(default_factory=C.c_void_p)
 
     def close(self) -> None:
@@ -91,9 +92,9 @@ class BatchAdjointProblem:
             if p.kernel != "tile":
                 raise ValueError("batched candidates use the tile kernels")
             if (p.n_steps != p0.n_steps or p.K != p0.K or p.sim.precision != p0.sim.precision
-                    or p.mode != p0.mode):
+                    or p.mode != p0.mode or p.window != p0.window):
                 raise ValueError("batched candidates share the step count, the checkpoint "
-                                 "interval, the precision and the mode")
+                                 "interval, the precision, the mode and the window choice")
         self.problems = list(problems)
         # "checkpoint": segments recomputed beside the reverse steps (co-plan);
         # "store": the forward pass keeps every step's Hz on each design box and the
@@ -232,6 +233,7 @@ class BatchAdjointProblem:
         solvers = tape.batch.solvers
         if ses.rev is None:
             ses.rev, ses.rplan, ses.co, ses.bufs = self._reverse_plans(tape)
+            ses.windows = self._windows
         else:
             _lib.call("etl_adjplan_reset", ses.rev.handle)
             for k, (p, s, b) in enumerate(zip(self.problems, solvers, ses.bufs)):
@@ -239,6 +241,8 @@ class BatchAdjointProblem:
                 b.g_ey.fill_zero()
                 comp, flat, table = p._sources(s)
                 ses.rev.sources(comp, flat, table, sim=k)
+                if ses.windows and ses.windows[k] is not None:
+                    p._refresh_window(s, ses.windows[k])
         self._sweep(ses.rev, ses.co)
         tape.batch.backend.synchronize()
         return [p._pixel_gradient(s, b) for p, s, b in zip(self.problems, solvers, ses.bufs)]
@@ -252,6 +256,7 @@ class BatchAdjointProblem:
 
         solvers = tape.batch.solvers
         K = self.K
+        self._windows = []
         if self.mode == "store":
             hists = tape.histories
         else:
@@ -267,17 +272,21 @@ class BatchAdjointProblem:
                 rev.sources(comp, flat, table, sim=k)
             if self.mode == "store":
                 return rev, None, co, bufs
-            plan = FusedPlan([s.state for s in solvers], self.rtune)
-            for k, (p, s, h) in enumerate(zip(self.problems, solvers, hists)):
-                p._configure_recompute(plan, s, k, h)
+            recompute = [p._recompute_state(s, rev, self.rtune)
+                         for p, s in zip(self.problems, solvers)]
+            windows = [w for _, _, w in recompute]
+            plan = FusedPlan([st for st, _, _ in recompute], recompute[0][1])
+            for k, (p, s, h, w) in enumerate(zip(self.problems, solvers, hists, windows)):
+                p._configure_recompute(plan, s, k, h, w)
                 ck = tape.checkpoints[k]
-                plan.set_checkpoints([ck[name] for name in FIELDS], ck[FIELDS[0]].shape[0], sim=k)
+                p._set_recompute_checkpoints(plan, ck, ck[FIELDS[0]].shape[0], k, w)
             _lib.call("etl_coplan_create", plan.handle, rev.handle, C.byref(co))
         except BaseException:
             if plan is not None:
                 plan.close()
             rev.close()
             raise
+        self._windows = windows
         return rev, plan, co, bufs
 
     def run(self, eps_list: Sequence[np.ndarray]) -> tuple[list, list]:

@@ -518,6 +518,12 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
     return ETL_OK;
 }
 
+int etl_adjplan_tune(etl_adjplan_t p, EtlTune* out) {
+    ETL_REQUIRE(p && out, "etl_adjplan_tune: null argument");
+    *out = p->tune;
+    return ETL_OK;
+}
+
 int etl_adjplan_destroy(etl_adjplan_t p) {
     if (p == nullptr) return ETL_OK;
     ETL_CHECK_INIT();

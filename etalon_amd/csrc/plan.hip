@@ -31,15 +31,21 @@ __global__ void set_counter_kernel(int* counter, int value, int probe_row0) {
 
 // Checkpoint copies: entry e (one field of one simulation) copies `words` 32-bit
 // words between buffer `parity` of the field and slot `seg` of its checkpoint array.
+// A window entry (rows > 0) restores a simulation that is a window of the grid whose
+// checkpoints these are: `rows` rows of `row_words` words (the window field) from
+// slot `seg` (of `words` words), starting `src_off` words into it, `src_pitch` words
+// apart.
 struct CkEntry {
     uint32_t* ck;
     uint32_t* f[2];
     int64_t words;
+    int64_t rows, row_words, src_pitch, src_off;
 };
 
 // 16-byte moves where both ends are 16-byte aligned (the fields are: etl_malloc
 // blocks are 256-byte aligned, and a checkpoint slot starts at seg * words words),
-// word moves otherwise and for the tail.
+// word moves otherwise and for the tail.  Window entries: block x takes rows x,
+// x + gridDim.x, ..., its threads the words of a row.
 __global__ void ckpt_copy_kernel(const CkEntry* __restrict__ tab, int seg, int parity,
                                  int restore) {
     const CkEntry& e = tab[blockIdx.y];
@@ -47,6 +53,13 @@ __global__ void ckpt_copy_kernel(const CkEntry* __restrict__ tab, int seg, int p
     // be promoted to LDS (32 KB per workgroup)
     uint32_t* f = parity ? e.f[1] : e.f[0];
     uint32_t* ck = e.ck + seg * e.words;
+    if (e.rows > 0) {  // window restore (restore is 1: checked on the host)
+        const uint32_t* src = ck + e.src_off;
+        for (int64_t r = blockIdx.x; r < e.rows; r += gridDim.x)
+            for (int64_t q = threadIdx.x; q < e.row_words; q += blockDim.x)
+                f[r * e.row_words + q] = src[r * e.src_pitch + q];
+        return;
+    }
     const uint32_t* src = restore ? ck : f;
     uint32_t* dst = restore ? f : ck;
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
@@ -145,9 +158,13 @@ struct EtlPlan {
         void* ph = nullptr;                               // (ph_rows_cap, nq) complex
         int ph_rows_cap = 0;
 
-        // checkpoints: ck[k] = (ck_seg, size of field k) REALs
+        // checkpoints: ck[k] = (ck_seg, size of field k) REALs; with ck_window, the
+        // slots hold fields of a (ck_nx, ck_ny) grid of which this simulation is the
+        // window from row ck_row0, column ck_col0 (restores only)
         void* ck[7] = {nullptr};
         int ck_seg = 0;
+        bool ck_window = false;
+        int ck_nx = 0, ck_ny = 0, ck_row0 = 0, ck_col0 = 0;
 
         int rows_of(int comp) const { return comp == 1 ? nx + 1 : nx; }
         int pitch_of(int comp) const { return comp == 0 ? ny + 1 : ny; }
@@ -735,15 +752,34 @@ int64_t field_elems(const EtlPlan::Sim& s, int k) {
     return int64_t(s.rows_of(comp_of[k])) * s.pitch_of(comp_of[k]);
 }
 
+// Component (0 Ex, 1 Ey, 2 Hz) of field k (ex ey hz psi_ex_y psi_ey_x psi_hz_x psi_hz_y).
+int comp_of_field(int k) {
+    static const int comp[7] = {0, 1, 2, 0, 1, 2, 2};
+    return comp[k];
+}
+
 // (Re)upload the checkpoint copy entries of every simulation.
 int ck_upload(EtlPlan* p) {
     std::vector<CkEntry> tab;
+    const int64_t w = p->elem() / 4;  // words per REAL
     for (const auto& sim : p->sims)
-        for (int k = 0; k < 7; ++k)
-            tab.push_back(CkEntry{static_cast<uint32_t*>(sim.ck[k]),
-                                  {static_cast<uint32_t*>(sim.desc.field[k][0]),
-                                   static_cast<uint32_t*>(sim.desc.field[k][1])},
-                                  field_elems(sim, k) * p->elem() / 4});
+        for (int k = 0; k < 7; ++k) {
+            CkEntry e{static_cast<uint32_t*>(sim.ck[k]),
+                      {static_cast<uint32_t*>(sim.desc.field[k][0]),
+                       static_cast<uint32_t*>(sim.desc.field[k][1])},
+                      field_elems(sim, k) * w, 0, 0, 0, 0};
+            if (sim.ck_window) {
+                const int c = comp_of_field(k);
+                const int64_t full_rows = c == 1 ? sim.ck_nx + 1 : sim.ck_nx;
+                const int64_t full_pitch = c == 0 ? sim.ck_ny + 1 : sim.ck_ny;
+                e.words = full_rows * full_pitch * w;
+                e.rows = sim.rows_of(c);
+                e.row_words = int64_t(sim.pitch_of(c)) * w;
+                e.src_pitch = full_pitch * w;
+                e.src_off = (int64_t(sim.ck_row0) * full_pitch + sim.ck_col0) * w;
+            }
+            tab.push_back(e);
+        }
     etl_free(p->ck_tab);
     p->ck_tab = nullptr;
     if (int rc = upload((void**)&p->ck_tab, tab.data(), tab.size() * sizeof(CkEntry))) return rc;
@@ -755,17 +791,25 @@ int ck_upload(EtlPlan* p) {
 // Save the state into (restore = 0) or load it from (restore = 1) checkpoint slot `seg`
 // of every simulation, on the plan's lane, in the buffers of the current parity.
 int ck_copy(EtlPlan* p, int seg, int restore) {
-    for (size_t k = 0; k < p->sims.size(); ++k)
+    for (size_t k = 0; k < p->sims.size(); ++k) {
         ETL_REQUIRE(p->sims[k].ck_seg > seg && seg >= 0,
                     "checkpoint %d: simulation %d has %d checkpoint slots", seg, int(k),
                     p->sims[k].ck_seg);
+        ETL_REQUIRE(restore || !p->sims[k].ck_window,
+                    "checkpoint %d: simulation %d restores a window of another grid's "
+                    "checkpoints and cannot save into them", seg, int(k));
+    }
     if (p->ck_stale)
         if (int rc = ck_upload(p)) return rc;
-    int64_t words = 0;
+    // one 16-byte move per thread and block row for the largest field (one block per
+    // row of the largest window), at most 2048 blocks
+    int64_t blocks = 1;
     for (const auto& sim : p->sims)
-        for (int k = 0; k < 7; ++k) words = std::max(words, field_elems(sim, k) * p->elem() / 4);
-    // one 16-byte move per thread and block row for the largest field, at most 2048 blocks
-    const int gx = int(std::max<int64_t>(1, std::min<int64_t>(2048, (words / 4 + 255) / 256)));
+        for (int k = 0; k < 7; ++k)
+            blocks = std::max(blocks, sim.ck_window
+                                          ? int64_t(sim.rows_of(comp_of_field(k)))
+                                          : (field_elems(sim, k) * p->elem() / 4 / 4 + 255) / 256);
+    const int gx = int(std::min<int64_t>(2048, blocks));
     hipLaunchKernelGGL(ckpt_copy_kernel, dim3(unsigned(gx), unsigned(p->ck_entries)), dim3(256),
                        0, lane_stream(p->lane), p->ck_tab, seg, p->parity, restore);
     ETL_LAUNCHED();
@@ -1295,8 +1339,45 @@ int etl_plan_set_checkpoints_sim(etl_plan_t p, int sim, void* const* ptr, int n_
                     "REALs", k, n_seg, (long long)field_elems(*s, k));
     for (int k = 0; k < 7; ++k) s->ck[k] = ptr[k];
     s->ck_seg = n_seg;
+    s->ck_window = false;
     p->ck_stale = true;
     return ETL_OK;
+}
+
+int etl_plan_set_checkpoint_window_sim(etl_plan_t p, int sim, void* const* ptr, int n_seg,
+                                       int full_nx, int full_ny, int row0, int col0) {
+    ETL_CHECK_INIT();
+    EtlPlan::Sim* s = sim_of(p, sim);
+    ETL_REQUIRE(s != nullptr && ptr != nullptr && n_seg >= 1,
+                "etl_plan_set_checkpoint_window: bad arguments (simulation %d, %d slots)", sim,
+                n_seg);
+    ETL_REQUIRE(full_nx >= 1 && full_ny >= 1 && row0 >= 0 && col0 >= 0 &&
+                    row0 + s->nx <= full_nx && col0 + s->ny <= full_ny,
+                "etl_plan_set_checkpoint_window: a %d x %d window from (%d, %d) does not fit a "
+                "%d x %d grid", s->nx, s->ny, row0, col0, full_nx, full_ny);
+    for (int k = 0; k < 7; ++k) {
+        const int c = comp_of_field(k);
+        const int64_t full = int64_t(c == 1 ? full_nx + 1 : full_nx) * (c == 0 ? full_ny + 1 : full_ny);
+        ETL_REQUIRE(ptr[k] != nullptr && guarded_bytes(ptr[k]) >= size_t(n_seg) * size_t(full) * p->elem(),
+                    "etl_plan_set_checkpoint_window: field %d needs an etl_malloc array of %d x %lld "
+                    "REALs", k, n_seg, (long long)full);
+    }
+    for (int k = 0; k < 7; ++k) s->ck[k] = ptr[k];
+    s->ck_seg = n_seg;
+    s->ck_window = true;
+    s->ck_nx = full_nx;
+    s->ck_ny = full_ny;
+    s->ck_row0 = row0;
+    s->ck_col0 = col0;
+    p->ck_stale = true;
+    return ETL_OK;
+}
+
+int etl_plan_checkpoint(etl_plan_t p, int seg, int restore) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr && (restore == 0 || restore == 1),
+                "etl_plan_checkpoint: bad arguments");
+    return ck_copy(p, seg, restore);
 }
 
 int etl_plan_advance_checkpointed(etl_plan_t p, int64_t n, int64_t count, int K, int* parity) {

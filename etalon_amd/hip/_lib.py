@@ -132,6 +132,7 @@ _SIGNATURES = {
     "etl_adjoint_sources": [vp, C.c_int, C.POINTER(C.c_int), i64p, C.c_int, vp],
     "etl_adjoint_run": [vp, C.c_int64, C.c_int],
     "etl_adjplan_create": [C.POINTER(EtlAdjointDesc), C.c_int, C.POINTER(EtlTune), C.POINTER(vp)],
+    "etl_adjplan_tune": [vp, C.POINTER(EtlTune)],
     "etl_adjplan_destroy": [vp],
     "etl_adjplan_sources": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), i64p, C.c_int, vp],
     "etl_adjplan_run": [vp, C.c_int64, C.c_int],
@@ -143,6 +144,9 @@ _SIGNATURES = {
     "etl_coplan_run": [vp, C.c_int64, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int],
     "etl_coplan_sweep": [vp, C.c_int64, C.c_int],
     "etl_plan_set_checkpoints_sim": [vp, C.c_int, C.POINTER(vp), C.c_int],
+    "etl_plan_set_checkpoint_window_sim": [vp, C.c_int, C.POINTER(vp), C.c_int, C.c_int,
+                                           C.c_int, C.c_int, C.c_int],
+    "etl_plan_checkpoint": [vp, C.c_int, C.c_int],
     "etl_plan_advance_checkpointed": [vp, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_int)],
     "etl_adjplan_lambda": [vp, C.c_int, C.c_int, C.POINTER(vp)],
     "etl_plan_tune": [vp, C.POINTER(EtlTune)],

@@ -152,6 +152,20 @@ class FusedPlan:
         ptrs = (C.c_void_p * 7)(*[a.ptr for a in arrays])
         _lib.call("etl_plan_set_checkpoints_sim", self.handle, int(sim), ptrs, int(n_seg))
 
+    def set_checkpoint_window(self, arrays: Sequence, n_seg: int, full_nx: int, full_ny: int,
+                              row0: int, col0: int, sim: int = 0) -> None:
+        """Simulation ``sim`` is the window from Hz node (row0, col0) of a full_nx x
+        full_ny grid whose checkpoint storage ``arrays`` is (FIELDS order, n_seg slots of
+        the full fields): restoring a slot copies that sub-block of every field."""
+        ptrs = (C.c_void_p * 7)(*[a.ptr for a in arrays])
+        _lib.call("etl_plan_set_checkpoint_window_sim", self.handle, int(sim), ptrs, int(n_seg),
+                  int(full_nx), int(full_ny), int(row0), int(col0))
+
+    def checkpoint(self, seg: int, restore: bool) -> None:
+        """Save the state of every simulation into slot ``seg`` or restore it from there
+        (stream-ordered, on the plan's lane), in the buffers that hold the state now."""
+        _lib.call("etl_plan_checkpoint", self.handle, int(seg), 1 if restore else 0)
+
     def advance_checkpointed(self, n0: int, count: int, every: int) -> int:
         """Steps [n0, n0 + count), the state saved into slot c // every before each
         segment of ``every`` steps starting at step c."""

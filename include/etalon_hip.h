@@ -255,6 +255,17 @@ int etl_plan_set_lane(etl_plan_t plan, int lane);
  * field k (ex ey hz psi_ex_y psi_ey_x psi_hz_x psi_hz_y order, each slot the field's
  * full array).  For the checkpointed adjoint; no reference counterpart. */
 int etl_plan_set_checkpoints_sim(etl_plan_t plan, int sim, void* const* ptr, int n_seg);
+/* Simulation `sim` is the window of a full_nx x full_ny grid whose checkpoints ptr[k]
+ * holds (n_seg slots of the full grid's field k, as etl_plan_set_checkpoints_sim):
+ * its Hz node (0, 0) is the full grid's (row0, col0), and restoring slot s copies that
+ * sub-block of every field into it (the window of a recompute that needs only the
+ * region a design box depends on over one segment; saves are refused).  For the
+ * checkpointed adjoint; no reference counterpart. */
+int etl_plan_set_checkpoint_window_sim(etl_plan_t plan, int sim, void* const* ptr, int n_seg,
+                                       int full_nx, int full_ny, int row0, int col0);
+/* Save (restore = 0) or load (restore = 1) checkpoint slot `seg` of every simulation,
+ * in the buffers that hold its state now, on the plan's lane.  Stream-ordered. */
+int etl_plan_checkpoint(etl_plan_t plan, int seg, int restore);
 /* Advance [n, n + count) (n a multiple of K), saving the state of every simulation
  * into checkpoint slot c / K before each segment of K steps starting at step c. */
 int etl_plan_advance_checkpointed(etl_plan_t plan, int64_t n, int64_t count, int K,
@@ -319,6 +330,9 @@ int etl_adjoint_run(etl_adjoint_t adj, int64_t n_first, int count);
 typedef struct EtlAdjPlan* etl_adjplan_t;
 int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* tune,
                        etl_adjplan_t* out);
+/* The tuning an adjoint plan resolved (its temporal depth, launch mode, band), e.g. for
+ * a recompute plan to run step blocks of the same depth beside it in a co-plan. */
+int etl_adjplan_tune(etl_adjplan_t plan, EtlTune* out);
 int etl_adjplan_destroy(etl_adjplan_t plan);
 int etl_adjplan_sources(etl_adjplan_t plan, int sim, int n, const int* comp, const int64_t* flat,
                         int rows, const void* table);

@@ -199,3 +199,60 @@ def test_design_loop_reuses_the_session_bitwise(hip):
         assert np.array_equal(g, g0)
     assert loop._session is not None and loop._session.rev is not None
     loop.close()
+
+
+def _wide_problem(precision, steps=640):
+    """A 12 x 10 um grid (256 x 216 cells): the 0.6 um design box's recompute window
+    (box + 66 nodes per side at K = 64) is a fraction of it, and the source lies outside
+    the window."""
+    return design_problem(precision, seed=6, steps=steps, size=(12.0, 10.0))
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("schedule", ["joint", "lanes", "sequential"])
+def test_window_recompute_equals_the_full_grid_bitwise(hip, precision, schedule):
+    """Recomputing each segment on the design box's window gives the whole grid's
+    gradient bit for bit, in every schedule."""
+    sim, design = _wide_problem(precision)
+    eps = np.asarray(design.permittivity)
+    win = AdjointProblem(sim, ["out"], checkpoint_every=64, schedule=schedule)
+    full = AdjointProblem(sim, ["out"], checkpoint_every=64, schedule=schedule, window=False)
+    Jw, gw = win.run(eps)
+    Jf, gf = full.run(eps)
+    i0, j0, ni, nj = win.box
+    m, nx, ny = win.K + 2, win.grid.nx, win.grid.ny
+    wx = min(nx, i0 + ni + m) - max(0, i0 - 1 - m)
+    wy = min(ny, j0 + nj + m) - max(0, j0 - 1 - m)
+    print(f"{precision} {schedule}: grid {nx} x {ny}, window {wx} x {wy}, "
+          f"max |g| {np.abs(gf).max():.3e}", flush=True)
+    assert wx * wy < nx * ny / 2
+    assert np.abs(gf).max() > 0
+    assert Jw == Jf
+    assert np.array_equal(gw, gf)
+
+
+def test_window_recompute_design_loop_and_batch_bitwise(hip):
+    """A design loop on windows (their cb refreshed for each new permittivity) and a
+    batch of candidates on windows equal fresh whole-grid problems bit for bit."""
+    from etalon_amd.batch_adjoint import BatchAdjointProblem
+
+    sim, design = _wide_problem("single")
+    rng = np.random.default_rng(9)
+    eps1 = np.asarray(design.permittivity)
+    eps2 = rng.uniform(2.085, 12.11, size=eps1.shape)
+    loop = AdjointProblem(sim, ["out"], checkpoint_every=64)
+    for eps in (eps1, eps2):
+        J, g = loop.run(eps)
+        J0, g0 = AdjointProblem(sim, ["out"], checkpoint_every=64, window=False,
+                                reuse=False).run(eps)
+        assert J == J0 and np.array_equal(g, g0)
+    loop.close()
+    batch = BatchAdjointProblem([AdjointProblem(sim, ["out"], checkpoint_every=64)
+                                 for _ in range(2)])
+    for eps_list in ([eps1, eps2], [eps2, eps1]):
+        Js, gs = batch.run(eps_list)
+        for J, g, e in zip(Js, gs, eps_list):
+            J0, g0 = AdjointProblem(sim, ["out"], checkpoint_every=64, window=False,
+                                    reuse=False).run(e)
+            assert J == J0 and np.array_equal(g, g0)
+    batch.close()
