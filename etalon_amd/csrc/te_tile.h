@@ -43,6 +43,12 @@
 
 namespace etl {
 
+// Rows of the window loaded ahead of the front that first needs them (time-skewed
+// tiles), beyond the two fronts a row must lead by.
+#ifndef ETL_LOAD_AHEAD
+#define ETL_LOAD_AHEAD 3
+#endif
+
 // V consecutive REALs with the alignment of one REAL: Ex rows have an odd pitch
 // (ny + 1) and gfx950 global loads/stores need only dword alignment.
 template <typename R, int V>
@@ -320,13 +326,18 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
         return base + (int64_t(r) * ny + cs);
     };
 
-    // ---- one burst of loads: the whole window ----
+    // ---- loads: the window's rows (all up front for source/monitor tiles; in the
+    // time-skewed order a few fronts ahead of their first use, so that rows stored at
+    // the top and rows not yet loaded at the bottom hold no registers) ----
     constexpr int PY = SY ? RW : 1;  // psi rows held (none in interior tiles)
     constexpr int PX = SX ? RW : 1;
     R ex[RW][V], hz[RW][V], ey[RW + 1][V], cx[RW][V], cy[RW][V];
     R phy[PY][V], phx[PX][V], pey[PY][V], pex[PX][V];
-#pragma unroll
-    for (int l = 0; l < RW; ++l) {
+    auto load_row = [&](int l) __attribute__((always_inline)) {
+        if (l == RW) {  // Ey has one row more
+            ldr<R, V>(ey[RW], ry(eyA, min(max(g0 + RW, 0), nx)), offy);
+            return;
+        }
         const int g = g0 + l;
         const int gx = min(max(g, 0), nx - 1);  // rows of Ex, Hz, cb_ex, psi_hz, psi_ex
         const int ge = min(max(g, 0), nx);      // rows of Ey, cb_ey, psi_ey
@@ -343,8 +354,12 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
             ldr<R, V>(phx[SX ? l : 0], ry(phxA, gx), offy);
             ldr<R, V>(pex[SX ? l : 0], ry(peyA, ge), offy);
         }
-    }
-    ldr<R, V>(ey[RW], ry(eyA, min(max(g0 + RW, 0), nx)), offy);
+    };
+    // rows loaded before front f of the skewed order: [0, f + 2 + kLoadAhead]
+    constexpr int kLoadAhead = ETL_LOAD_AHEAD;
+    constexpr int first_rows = EXTRA ? RW + 1 : (RW + 1 < 2 + kLoadAhead ? RW + 1 : 2 + kLoadAhead);
+#pragma unroll
+    for (int l = 0; l < first_rows; ++l) load_row(l);
 
     const R db = a.db, idx = a.inv_dx, idy = a.inv_dy;
 
@@ -573,6 +588,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
         // is stored as soon as its last step is done.
 #pragma unroll
         for (int f = 0; f < RW; ++f) {
+            if (f + 2 + kLoadAhead <= RW) load_row(f + 2 + kLoadAhead);
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const int l = f - t;

@@ -11,7 +11,9 @@ import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent.parent / "libetalon_hip.so"
+# The in-tree build (python -m etalon_amd.build); $ETALON_AMD_LIB names another build.
+LIB_PATH = Path(os.environ.get("ETALON_AMD_LIB") or
+                Path(__file__).resolve().parent.parent / "libetalon_hip.so")
 
 F32, F64 = 0, 1
 _lock = threading.Lock()
