@@ -88,14 +88,15 @@ REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
 # The counts belong to one build of the kernels: PMC_BUILD is the fingerprint of the
 # sources they were measured with (kernel_build_id), and a build with another
 # fingerprint reports no measured traffic until the counters are taken again.
-PMC_BUILD = "645ef0bd73e23b1e"
-PMC_KIB = {("single", 1, 6, 16, 1): (47527.0, 49859.5),
-           ("single", 1, 4, 16, 1): (45949.8, 49675.3),
-           ("double", 1, 6, 8, 1): (103206.7, 110419.9),
-           ("double", 1, 4, 8, 1): (90073.0, 98812.3)}
-FETCH_SCALE, WRITE_SCALE = 1.9998, 1.0
-PMC_KIB_8024 = {("single", 6, 16): (744470.3, 795740.1),
-                ("double", 6, 8): (2222508.3, 1602349.5)}
+PMC_BUILD = "90be6866a05c0454"
+PMC_KIB = {("single", 1, 6, 16, 1): (63748.2, 50019.5),
+           ("double", 1, 6, 8, 1): (104222.3, 104942.1)}
+FETCH_SCALE, WRITE_SCALE = 1.99985, 1.0
+PMC_KIB_8024 = {("single", 6, 16): (668246.1, 765483.0),
+                ("double", 6, 8): (2180558.7, 1562426.0)}
+# Infinity Cache (MALL) of the MI355X: a launch whose working set fits it has part of
+# its L2 misses served there, not by HBM.
+MALL_BYTES = 256 * 2**20
 # The sources that define the forward joint kernel and its tile work-list.
 KERNEL_SOURCES = ("common.h", "fused.h", "te_tile.h", "adj_tile.h", "joint.h", "coplan.h",
                   "joint_fwd.hip", "plan.hip")
@@ -227,9 +228,9 @@ def roofline(precision: str, r: dict, traffic: "int | None") -> dict:
         "unit": "GB/s",
         "frac": round(gbs / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "basis": ("measured: HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE of "
-                  "this build of the kernel (scaled by a copy kernel's factors), over the "
-                  "launch's event-timed duration" if traffic is not None else
+        "basis": ("measured: bytes per launch between L2 and memory from rocprofv3 FETCH_SIZE "
+                  "and WRITE_SIZE of this build of the kernel (scaled by a copy kernel's "
+                  "factors), over the launch's event-timed duration" if traffic is not None else
                   "compulsory: no PMC counts for this build of the kernel, so the bytes are "
                   "the compulsory ones (each array read or written once per launch), a lower "
                   "bound on its traffic"),
@@ -246,6 +247,15 @@ def roofline(precision: str, r: dict, traffic: "int | None") -> dict:
     }
     if traffic is not None:
         roof["traffic_over_compulsory"] = round(traffic / compulsory, 3)
+    # both buffers of Ex, Ey and Hz, cb_ex and cb_ey (the CPML strips' psi arrays aside)
+    working_set = tm["kernel_cells"] * BYTES_PER_CELL[precision]
+    if working_set < MALL_BYTES:
+        roof["cache_note"] = (f"the launch's working set ({working_set / 1e6:.0f} MB: both "
+                              "buffers of Ex, Ey, Hz and cb_ex, cb_ey) fits the 256 MiB "
+                              "Infinity Cache, so FETCH_SIZE and WRITE_SIZE count the L2 "
+                              "misses to the memory fabric, part of which that cache serves "
+                              "instead of HBM; hbm_resident is the grid whose launches stream "
+                              "from HBM")
     return roof
 
 
