@@ -53,11 +53,13 @@ def checkpoint_interval(n_steps: int) -> int:
 @functools.lru_cache(maxsize=8)
 def _forward_phasors(n_steps: int, dt: float, magnetic: bool, freqs: bytes,
                      dtype: str) -> np.ndarray:
-    """The forward pass's phasor table of steps [0, n_steps) in its precision, widened
-    to complex128; shared by the candidates of one Δt and frequency list; read-only."""
+    """[Re P | Im P] (n_steps, 2 n_f) float64 of the forward pass's phasor table P of
+    steps [0, n_steps), in its precision; shared by the candidates of one Δt and
+    frequency list; read-only."""
     P = _phasor_table(0, n_steps, dt, magnetic, freqs, dtype).astype(np.complex128)
-    P.setflags(write=False)
-    return P
+    Q = np.ascontiguousarray(np.concatenate([P.real, P.imag], axis=1))
+    Q.setflags(write=False)
+    return Q
 
 
 def recompute_tune(tune, precision: str):
@@ -275,10 +277,8 @@ class AdjointProblem:
         else:
             tape.starts = list(range(0, N, K))
             n_ck = len(tape.starts)
-            for name in FIELDS:
-                a = getattr(solver.state, name)
-                tape.checkpoints[name] = zeros((n_ck, a.size), a.dtype)
-            plan.set_checkpoints([tape.checkpoints[name] for name in FIELDS], n_ck)
+            tape.checkpoints = self._checkpoint_storage(solver, n_ck)
+            self._set_forward_checkpoints(plan, tape.checkpoints, n_ck, 0)
             plan.advance_checkpointed(0, N, K)  # slot k = the state at step k K
             plan.bind()
             if session:
@@ -350,9 +350,13 @@ class AdjointProblem:
         return J
 
     def _sources(self, solver):
-        """Distinct adjoint-source nodes (comp, flat) and the table (N, n) of dJ/dv."""
+        """Distinct adjoint-source nodes (comp, flat) and the table (N, n) of dJ/dv.
+
+        Column k of a monitor's block is S_k = 2 Re(P conj(A_k)) over the forward
+        pass's phasors P (N, n_f) and A = (acc Wᵀ) W, computed as one real product
+        [Re P | Im P] [Re A; Im A]; columns of one node sum in monitor order."""
         N, dt = self.n_steps, self.grid.dt
-        cols: dict = {}
+        blocks = []
         for mon in solver.monitors:
             smp = mon.sampling
             if smp.monitor not in self.objective:
@@ -360,19 +364,40 @@ class AdjointProblem:
             W = _weights(smp)
             acc = mon.acc.get().astype(np.complex128)        # (n_f, n_nodes)
             A = (acc @ W.T) @ W                              # (n_f, n_nodes)
-            # (N, n_f): the phasors the forward pass used, in its precision
-            P = _forward_phasors(N, dt, smp.is_magnetic,
+            Q = _forward_phasors(N, dt, smp.is_magnetic,
                                  np.asarray(smp.frequencies, np.float64).tobytes(),
                                  np.dtype(solver.complex).str)
-            S = 2.0 * np.real(P @ np.conj(A))                # (N, n_nodes)
-            for k, flat in enumerate(smp.flat_index):
-                key = (_COMP[smp.component], int(flat))
-                cols[key] = cols.get(key, 0.0) + S[:, k]
-        keys = sorted(cols)
+            keys = [(_COMP[smp.component], int(f)) for f in smp.flat_index]
+            blocks.append((Q, A, keys))
+        if not blocks:
+            return np.zeros(0, np.int32), np.zeros(0, np.int64), np.zeros((N, 0), solver.real)
+
+        def columns(Q, A):
+            S = Q @ np.concatenate([A.real, A.imag], axis=0)  # (N, n_nodes)
+            S *= 2.0
+            return S
+
+        keys = blocks[0][2]
+        if len(blocks) == 1 and len(set(keys)) == len(keys):
+            # one monitor, distinct nodes: its columns in node order (0.0 + S as the
+            # sums below start from)
+            order = sorted(range(len(keys)), key=keys.__getitem__)
+            Q, A, _ = blocks[0]
+            S = columns(Q, A[:, order])
+            S += 0.0
+            keys = [keys[k] for k in order]
+            table = S.astype(solver.real)
+        else:
+            cols: dict = {}
+            for Q, A, bkeys in blocks:
+                S = columns(Q, A)
+                for k, key in enumerate(bkeys):
+                    cols[key] = cols.get(key, 0.0) + S[:, k]
+            keys = sorted(cols)
+            table = np.ascontiguousarray(np.stack([cols[k] for k in keys], axis=1),
+                                         dtype=solver.real)
         comp = np.array([k[0] for k in keys], np.int32)
         flat = np.array([k[1] for k in keys], np.int64)
-        table = np.ascontiguousarray(np.stack([cols[k] for k in keys], axis=1), dtype=solver.real) \
-            if keys else np.zeros((N, 0), solver.real)
         return comp, flat, table
 
     # ------------------------------------------------------------------ backward
@@ -479,16 +504,46 @@ class AdjointProblem:
             np.add.at(out, p, g)
         return out.reshape(self.shape)
 
+    def _window_bounds(self) -> tuple:
+        """(i0, j0, nx, ny) of the recompute window (see _Window): the design box's Hz
+        nodes, rows i0 - 1 .. i0 + ni - 1 and columns j0 - 1 .. j0 + nj - 1, and every
+        node within K + 2 of them, cut to the grid."""
+        i0, j0, ni, nj = self.box
+        nx, ny = self.grid.nx, self.grid.ny
+        m = self.K + 2
+        r0, r1 = max(0, i0 - 1 - m), min(nx, i0 + ni + m)
+        c0, c1 = max(0, j0 - 1 - m), min(ny, j0 + nj + m)
+        return r0, c0, r1 - r0, c1 - c0
+
+    def _checkpoint_storage(self, solver, n_ck: int) -> dict:
+        """Zeroed checkpoint arrays of the forward pass, FIELDS order: ``n_ck`` slots of
+        each whole field, or with ``window`` of each field's block on the recompute
+        window (all a recompute restores)."""
+        real = solver.real
+        if not self.window:
+            return {name: zeros((n_ck, getattr(solver.state, name).size), real)
+                    for name in FIELDS}
+        _, _, wx, wy = self._window_bounds()
+        size = {"ex": wx * (wy + 1), "ey": (wx + 1) * wy, "hz": wx * wy, "psi_ex_y": wx * (wy + 1),
+                "psi_ey_x": (wx + 1) * wy, "psi_hz_x": wx * wy, "psi_hz_y": wx * wy}
+        return {name: zeros((n_ck, size[name]), real) for name in FIELDS}
+
+    def _set_forward_checkpoints(self, plan, checkpoints: dict, n_ck: int, sim: int) -> None:
+        """Simulation ``sim`` of the forward plan saves into ``checkpoints``
+        (_checkpoint_storage's arrays)."""
+        arrays = [checkpoints[name] for name in FIELDS]
+        if not self.window:
+            plan.set_checkpoints(arrays, n_ck, sim=sim)
+        else:
+            r0, c0, wx, wy = self._window_bounds()
+            plan.set_checkpoint_window(arrays, n_ck, wx, wy, r0, c0, sim=sim)
+
     def _window_of(self, solver) -> _Window:
         """The recompute window of ``solver``'s grid (see _Window), with its state."""
         from etalon_amd.hip.backend import to_device
 
-        i0, j0, ni, nj = self.box
-        nx, ny = self.grid.nx, self.grid.ny
-        m = self.K + 2
-        # the box's Hz nodes are rows i0 - 1 .. i0 + ni - 1, columns j0 - 1 .. j0 + nj - 1
-        r0, r1 = max(0, i0 - 1 - m), min(nx, i0 + ni + m)
-        c0, c1 = max(0, j0 - 1 - m), min(ny, j0 + nj + m)
+        r0, c0, wx, wy = self._window_bounds()
+        r1, c1 = r0 + wx, c0 + wy
         real = solver.real
         st = solver.state
         put = lambda a: to_device(np.ascontiguousarray(a, dtype=real))  # noqa: E731
@@ -574,12 +629,13 @@ class AdjointProblem:
 
     def _set_recompute_checkpoints(self, plan, checkpoints: dict, n_ck: int, sim: int,
                                    win=None) -> None:
-        arrays = [checkpoints[name] for name in FIELDS]
-        if win is None:
-            plan.set_checkpoints(arrays, n_ck, sim=sim)
-        else:
-            plan.set_checkpoint_window(arrays, n_ck, self.grid.nx, self.grid.ny, win.i0,
-                                       win.j0, sim=sim)
+        """Simulation ``sim`` of a recompute plan restores from the forward pass's
+        checkpoints: whole fields, or on a window (whose fields are the blocks the
+        forward pass saved) the window's."""
+        if (win is not None) != self.window:
+            raise ValueError("window checkpoints go with a window recompute, whole-field "
+                             "checkpoints with a whole-grid one")
+        plan.set_checkpoints([checkpoints[name] for name in FIELDS], n_ck, sim=sim)
 
     def _reverse_segments(self, tape: Tape, rev, hist) -> None:
         """Recompute each segment from its checkpoint (with the Hz probe on the box),

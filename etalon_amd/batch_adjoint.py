@@ -147,9 +147,8 @@ class BatchAdjointProblem:
                 plan.add_probe("Hz", nodes, hist, sim=k)
                 tape.histories.append(hist)
             else:
-                ck = {name: zeros((n_ck, getattr(s.state, name).size), s.real)
-                      for name in FIELDS}
-                plan.set_checkpoints([ck[name] for name in FIELDS], n_ck, sim=k)
+                ck = p._checkpoint_storage(s, n_ck)
+                p._set_forward_checkpoints(plan, ck, n_ck, k)
                 tape.checkpoints.append(ck)
         self._advance_forward(plan)
         for s in batch.solvers:

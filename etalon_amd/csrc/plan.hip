@@ -31,15 +31,14 @@ __global__ void set_counter_kernel(int* counter, int value, int probe_row0) {
 
 // Checkpoint copies: entry e (one field of one simulation) copies `words` 32-bit
 // words between buffer `parity` of the field and slot `seg` of its checkpoint array.
-// A window entry (rows > 0) restores a simulation that is a window of the grid whose
-// checkpoints these are: `rows` rows of `row_words` words (the window field) from
-// slot `seg` (of `words` words), starting `src_off` words into it, `src_pitch` words
-// apart.
+// A window entry (rows > 0) has slots that hold a window of the field: `rows` rows of
+// `row_words` words, stored one after another in the slot (of `words` words), taken
+// from the field `off` words in, `pitch` words apart.
 struct CkEntry {
     uint32_t* ck;
     uint32_t* f[2];
     int64_t words;
-    int64_t rows, row_words, src_pitch, src_off;
+    int64_t rows, row_words, pitch, off;
 };
 
 // 16-byte moves where both ends are 16-byte aligned (the fields are: etl_malloc
@@ -53,11 +52,13 @@ __global__ void ckpt_copy_kernel(const CkEntry* __restrict__ tab, int seg, int p
     // be promoted to LDS (32 KB per workgroup)
     uint32_t* f = parity ? e.f[1] : e.f[0];
     uint32_t* ck = e.ck + seg * e.words;
-    if (e.rows > 0) {  // window restore (restore is 1: checked on the host)
-        const uint32_t* src = ck + e.src_off;
+    if (e.rows > 0) {
+        uint32_t* fw = f + e.off;
         for (int64_t r = blockIdx.x; r < e.rows; r += gridDim.x)
-            for (int64_t q = threadIdx.x; q < e.row_words; q += blockDim.x)
-                f[r * e.row_words + q] = src[r * e.src_pitch + q];
+            for (int64_t q = threadIdx.x; q < e.row_words; q += blockDim.x) {
+                if (restore) fw[r * e.pitch + q] = ck[r * e.row_words + q];
+                else ck[r * e.row_words + q] = fw[r * e.pitch + q];
+            }
         return;
     }
     const uint32_t* src = restore ? ck : f;
@@ -158,9 +159,9 @@ struct EtlPlan {
         void* ph = nullptr;                               // (ph_rows_cap, nq) complex
         int ph_rows_cap = 0;
 
-        // checkpoints: ck[k] = (ck_seg, size of field k) REALs; with ck_window, the
-        // slots hold fields of a (ck_nx, ck_ny) grid of which this simulation is the
-        // window from row ck_row0, column ck_col0 (restores only)
+        // checkpoints: ck[k] = (ck_seg, size of field k) REALs; with ck_window, each
+        // slot holds only the window of the fields with Hz nodes [ck_row0, + ck_nx) x
+        // [ck_col0, + ck_ny) (Ex one column more, Ey one row more)
         void* ck[7] = {nullptr};
         int ck_seg = 0;
         bool ck_window = false;
@@ -770,13 +771,13 @@ int ck_upload(EtlPlan* p) {
                       field_elems(sim, k) * w, 0, 0, 0, 0};
             if (sim.ck_window) {
                 const int c = comp_of_field(k);
-                const int64_t full_rows = c == 1 ? sim.ck_nx + 1 : sim.ck_nx;
-                const int64_t full_pitch = c == 0 ? sim.ck_ny + 1 : sim.ck_ny;
-                e.words = full_rows * full_pitch * w;
-                e.rows = sim.rows_of(c);
-                e.row_words = int64_t(sim.pitch_of(c)) * w;
-                e.src_pitch = full_pitch * w;
-                e.src_off = (int64_t(sim.ck_row0) * full_pitch + sim.ck_col0) * w;
+                const int64_t rows = c == 1 ? sim.ck_nx + 1 : sim.ck_nx;
+                const int64_t cols = c == 0 ? sim.ck_ny + 1 : sim.ck_ny;
+                e.words = rows * cols * w;
+                e.rows = rows;
+                e.row_words = cols * w;
+                e.pitch = int64_t(sim.pitch_of(c)) * w;
+                e.off = (int64_t(sim.ck_row0) * sim.pitch_of(c) + sim.ck_col0) * w;
             }
             tab.push_back(e);
         }
@@ -795,9 +796,6 @@ int ck_copy(EtlPlan* p, int seg, int restore) {
         ETL_REQUIRE(p->sims[k].ck_seg > seg && seg >= 0,
                     "checkpoint %d: simulation %d has %d checkpoint slots", seg, int(k),
                     p->sims[k].ck_seg);
-        ETL_REQUIRE(restore || !p->sims[k].ck_window,
-                    "checkpoint %d: simulation %d restores a window of another grid's "
-                    "checkpoints and cannot save into them", seg, int(k));
     }
     if (p->ck_stale)
         if (int rc = ck_upload(p)) return rc;
@@ -807,7 +805,7 @@ int ck_copy(EtlPlan* p, int seg, int restore) {
     for (const auto& sim : p->sims)
         for (int k = 0; k < 7; ++k)
             blocks = std::max(blocks, sim.ck_window
-                                          ? int64_t(sim.rows_of(comp_of_field(k)))
+                                          ? int64_t(comp_of_field(k) == 1 ? sim.ck_nx + 1 : sim.ck_nx)
                                           : (field_elems(sim, k) * p->elem() / 4 / 4 + 255) / 256);
     const int gx = int(std::min<int64_t>(2048, blocks));
     hipLaunchKernelGGL(ckpt_copy_kernel, dim3(unsigned(gx), unsigned(p->ck_entries)), dim3(256),
@@ -1345,28 +1343,28 @@ int etl_plan_set_checkpoints_sim(etl_plan_t p, int sim, void* const* ptr, int n_
 }
 
 int etl_plan_set_checkpoint_window_sim(etl_plan_t p, int sim, void* const* ptr, int n_seg,
-                                       int full_nx, int full_ny, int row0, int col0) {
+                                       int win_nx, int win_ny, int row0, int col0) {
     ETL_CHECK_INIT();
     EtlPlan::Sim* s = sim_of(p, sim);
     ETL_REQUIRE(s != nullptr && ptr != nullptr && n_seg >= 1,
                 "etl_plan_set_checkpoint_window: bad arguments (simulation %d, %d slots)", sim,
                 n_seg);
-    ETL_REQUIRE(full_nx >= 1 && full_ny >= 1 && row0 >= 0 && col0 >= 0 &&
-                    row0 + s->nx <= full_nx && col0 + s->ny <= full_ny,
-                "etl_plan_set_checkpoint_window: a %d x %d window from (%d, %d) does not fit a "
-                "%d x %d grid", s->nx, s->ny, row0, col0, full_nx, full_ny);
+    ETL_REQUIRE(win_nx >= 1 && win_ny >= 1 && row0 >= 0 && col0 >= 0 &&
+                    row0 + win_nx <= s->nx && col0 + win_ny <= s->ny,
+                "etl_plan_set_checkpoint_window: a %d x %d window from (%d, %d) does not fit "
+                "the %d x %d grid", win_nx, win_ny, row0, col0, s->nx, s->ny);
     for (int k = 0; k < 7; ++k) {
         const int c = comp_of_field(k);
-        const int64_t full = int64_t(c == 1 ? full_nx + 1 : full_nx) * (c == 0 ? full_ny + 1 : full_ny);
-        ETL_REQUIRE(ptr[k] != nullptr && guarded_bytes(ptr[k]) >= size_t(n_seg) * size_t(full) * p->elem(),
-                    "etl_plan_set_checkpoint_window: field %d needs an etl_malloc array of %d x %lld "
-                    "REALs", k, n_seg, (long long)full);
+        const int64_t win = int64_t(c == 1 ? win_nx + 1 : win_nx) * (c == 0 ? win_ny + 1 : win_ny);
+        ETL_REQUIRE(ptr[k] != nullptr && guarded_bytes(ptr[k]) >= size_t(n_seg) * size_t(win) * p->elem(),
+                    "etl_plan_set_checkpoint_window: field %d needs an etl_malloc array of %d x "
+                    "%lld REALs", k, n_seg, (long long)win);
     }
     for (int k = 0; k < 7; ++k) s->ck[k] = ptr[k];
     s->ck_seg = n_seg;
     s->ck_window = true;
-    s->ck_nx = full_nx;
-    s->ck_ny = full_ny;
+    s->ck_nx = win_nx;
+    s->ck_ny = win_ny;
     s->ck_row0 = row0;
     s->ck_col0 = col0;
     p->ck_stale = true;

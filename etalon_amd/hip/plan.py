@@ -152,14 +152,16 @@ class FusedPlan:
         ptrs = (C.c_void_p * 7)(*[a.ptr for a in arrays])
         _lib.call("etl_plan_set_checkpoints_sim", self.handle, int(sim), ptrs, int(n_seg))
 
-    def set_checkpoint_window(self, arrays: Sequence, n_seg: int, full_nx: int, full_ny: int,
+    def set_checkpoint_window(self, arrays: Sequence, n_seg: int, win_nx: int, win_ny: int,
                               row0: int, col0: int, sim: int = 0) -> None:
-        """Simulation ``sim`` is the window from Hz node (row0, col0) of a full_nx x
-        full_ny grid whose checkpoint storage ``arrays`` is (FIELDS order, n_seg slots of
-        the full fields): restoring a slot copies that sub-block of every field."""
+        """Checkpoint storage that holds only a window of simulation ``sim``'s fields:
+        arrays[k] (FIELDS order) holds ``n_seg`` slots, each the block of field k with
+        Hz nodes [row0, row0 + win_nx) x [col0, col0 + win_ny) (Ex one column more, Ey
+        one row more) stored row after row.  A plan over that window's grid takes the
+        same arrays with ``set_checkpoints``."""
         ptrs = (C.c_void_p * 7)(*[a.ptr for a in arrays])
         _lib.call("etl_plan_set_checkpoint_window_sim", self.handle, int(sim), ptrs, int(n_seg),
-                  int(full_nx), int(full_ny), int(row0), int(col0))
+                  int(win_nx), int(win_ny), int(row0), int(col0))
 
     def checkpoint(self, seg: int, restore: bool) -> None:
         """Save the state of every simulation into slot ``seg`` or restore it from there

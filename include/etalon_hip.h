@@ -255,14 +255,15 @@ int etl_plan_set_lane(etl_plan_t plan, int lane);
  * field k (ex ey hz psi_ex_y psi_ey_x psi_hz_x psi_hz_y order, each slot the field's
  * full array).  For the checkpointed adjoint; no reference counterpart. */
 int etl_plan_set_checkpoints_sim(etl_plan_t plan, int sim, void* const* ptr, int n_seg);
-/* Simulation `sim` is the window of a full_nx x full_ny grid whose checkpoints ptr[k]
- * holds (n_seg slots of the full grid's field k, as etl_plan_set_checkpoints_sim):
- * its Hz node (0, 0) is the full grid's (row0, col0), and restoring slot s copies that
- * sub-block of every field into it (the window of a recompute that needs only the
- * region a design box depends on over one segment; saves are refused).  For the
- * checkpointed adjoint; no reference counterpart. */
+/* Checkpoint storage of simulation `sim` that holds only a window of its fields: ptr[k]
+ * (etl_malloc) holds n_seg slots, each the block of field k with Hz nodes [row0, row0 +
+ * win_nx) x [col0, col0 + win_ny) (Ex one column more, Ey one row more) stored row after
+ * row; saves and restores copy that block.  A plan over the window itself (a grid of
+ * win_nx x win_ny) takes the same arrays with etl_plan_set_checkpoints_sim.  For the
+ * checkpointed adjoint, whose recompute needs only the region a design box depends on
+ * over one segment; no reference counterpart. */
 int etl_plan_set_checkpoint_window_sim(etl_plan_t plan, int sim, void* const* ptr, int n_seg,
-                                       int full_nx, int full_ny, int row0, int col0);
+                                       int win_nx, int win_ny, int row0, int col0);
 /* Save (restore = 0) or load (restore = 1) checkpoint slot `seg` of every simulation,
  * in the buffers that hold its state now, on the plan's lane.  Stream-ordered. */
 int etl_plan_checkpoint(etl_plan_t plan, int seg, int restore);

@@ -226,6 +226,11 @@ def test_window_recompute_equals_the_full_grid_bitwise(hip, precision, schedule)
     print(f"{precision} {schedule}: grid {nx} x {ny}, window {wx} x {wy}, "
           f"max |g| {np.abs(gf).max():.3e}", flush=True)
     assert wx * wy < nx * ny / 2
+    if schedule == "joint":  # the design-loop session keeps the tape's checkpoints
+        n_ck = len(win._session.starts)
+        assert win._session.checkpoints["hz"].shape == (n_ck, wx * wy)
+        assert win._session.checkpoints["ex"].shape == (n_ck, wx * (wy + 1))
+        assert full._session.checkpoints["hz"].shape == (n_ck, nx * ny)
     assert np.abs(gf).max() > 0
     assert Jw == Jf
     assert np.array_equal(gw, gf)
