@@ -229,15 +229,23 @@ class AdjointProblem:
         if self._box_checked:  # the CPML profiles depend on the grid only
             return
         i0, j0, ni, nj = self.box
-        s = solver.state
+        prof = self._profiles(solver)
         for name, lo, hi in (("bx_e", i0 - 1, i0 + ni + 1), ("bx_h", i0 - 1, i0 + ni + 1),
                              ("by_e", j0 - 1, j0 + nj + 1), ("by_h", j0 - 1, j0 + nj + 1)):
-            b = getattr(s, name).get()[lo:hi]
-            c = getattr(s, name.replace("b", "c", 1)).get()[lo:hi]
-            ik = getattr(s, name.replace("b", "ik", 1)).get()[lo:hi]
+            b = prof[name][lo:hi]
+            c = prof[name.replace("b", "c", 1)][lo:hi]
+            ik = prof[name.replace("b", "ik", 1)][lo:hi]
             if np.any(b != 1) or np.any(c != 0) or np.any(ik != 1):
                 raise ValueError("the design region must lie outside the CPML")
         self._box_checked = True
+
+    @staticmethod
+    def _profiles(solver) -> dict:
+        """The CPML profiles of ``solver``'s state on the host (PROFILES names)."""
+        st = solver.state
+        if st.host_profiles is not None:
+            return st.host_profiles
+        return {name: getattr(st, name).get() for name in PROFILES}
 
     def _box_hz_nodes(self) -> np.ndarray:
         i0, j0, ni, nj = self.box
@@ -548,20 +556,27 @@ class AdjointProblem:
         st = solver.state
         put = lambda a: to_device(np.ascontiguousarray(a, dtype=real))  # noqa: E731
         cb = self._window_cb(solver, r0, r1, c0, c1)
+        host = self._profiles(solver)
         prof = {}
         for name in PROFILES:
-            v = getattr(st, name).get()
+            v = host[name]
             if "x" in name.split("_")[0]:  # bx_*, cx_*, ikx_*: along the rows
                 prof[name] = v[r0:r1 + 1] if name.endswith("_e") else v[r0:r1]
             else:
                 prof[name] = v[c0:c1 + 1] if name.endswith("_e") else v[c0:c1]
+        # the twelve window profiles in one device array (one upload), each a view
+        packed = put(np.concatenate([prof[name] for name in PROFILES]))
+        at = 0
+        for name in PROFILES:
+            n = prof[name].size
+            prof[name] = packed.rows(at, at + n)
+            at += n
         shapes = {"ex": (r1 - r0, c1 - c0 + 1), "ey": (r1 - r0 + 1, c1 - c0),
                   "hz": (r1 - r0, c1 - c0), "psi_ex_y": (r1 - r0, c1 - c0 + 1),
                   "psi_ey_x": (r1 - r0 + 1, c1 - c0), "psi_hz_x": (r1 - r0, c1 - c0),
                   "psi_hz_y": (r1 - r0, c1 - c0)}
         state = TEState(**{name: zeros(shape, real) for name, shape in shapes.items()},
-                        cb_ex=put(cb["cb_ex"]), cb_ey=put(cb["cb_ey"]),
-                        **{name: put(v) for name, v in prof.items()},
+                        cb_ex=put(cb["cb_ex"]), cb_ey=put(cb["cb_ey"]), **prof,
                         db=st.db, inv_dx=st.inv_dx, inv_dy=st.inv_dy, nx=r1 - r0, ny=c1 - c0)
         return _Window(r0, c0, r1 - r0, c1 - c0, state)
 

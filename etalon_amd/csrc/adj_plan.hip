@@ -496,7 +496,9 @@ int etl_adjplan_create(const EtlAdjointDesc* descs, int n_sims, const EtlTune* t
         if (rc == ETL_OK)
             rc = precision == ETL_F32 ? adj_strips<float>(d, s.strips) : adj_strips<double>(d, s.strips);
     }
-    if (rc == ETL_OK) {
+    // side streams and their events: only launch mode 3 (one kernel per tile kind)
+    // forks onto them
+    if (rc == ETL_OK && p->tune.launch == 3) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||

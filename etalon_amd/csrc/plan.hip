@@ -1052,7 +1052,9 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
             (t.vec == 1 && (t.band_rows == 8 || (t.band_rows == 16 && precision == ETL_F32)))
                 ? 1
                 : 3;
-    if (rc == ETL_OK) {
+    // side streams and their events: only launch mode 3 (one kernel per tile kind)
+    // forks onto them
+    if (rc == ETL_OK && p->tune.launch == 3) {
         if (hipStreamCreateWithFlags(&p->s_strip, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&p->s_extra, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||

@@ -9,7 +9,7 @@ backends start from bit-identical inputs.
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Any, Callable
+from typing import Any, Callable, Optional
 
 import numpy as np
 
@@ -50,6 +50,9 @@ class TEState:
     inv_dy: Any
     nx: int
     ny: int
+    # host copies of the CPML profiles as cast (PROFILES names), when the builder kept
+    # them: readers of the profiles need no device copy
+    host_profiles: Optional[dict] = None
 
 
 def cb_arrays(grid: YeeGrid2D, eps_r: dict) -> dict:
@@ -85,8 +88,9 @@ def build_te_state(grid: YeeGrid2D, eps_r: dict, cpml_x: CPMLAxis, cpml_y: CPMLA
               "psi_hz_x": "Hz", "psi_hz_y": "Hz"}
     arrays = {name: put(np.zeros(grid.shape(comp))) for name, comp in shapes.items()}
     arrays.update({name: put(coef[name]) for name in ("cb_ex", "cb_ey", *PROFILES)})
+    host = {name: np.ascontiguousarray(coef[name], dtype=dtype) for name in PROFILES}
     return TEState(
         **arrays,
         db=dtype(coef["db"]), inv_dx=dtype(coef["inv_dx"]), inv_dy=dtype(coef["inv_dy"]),
-        nx=grid.nx, ny=grid.ny,
+        nx=grid.nx, ny=grid.ny, host_profiles=host,
     )
