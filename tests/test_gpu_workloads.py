@@ -14,7 +14,9 @@
   2048 steps, checkpoints every 64): the batch's gradients equal each candidate's
   own AdjointProblem bit for bit; candidate 0's float64 gradient equals the NumPy
   adjoint (tests/golden/adjoint_cfg5.npz) and float64 central differences, and its
-  float32 gradient the float64 one to 1e-3.
+  float32 gradient the float64 one to 1e-3.  At the benchmark's full size (32
+  candidates, 8000 steps) the first and last candidates equal their own problems bit
+  for bit and candidate 0 agrees with float64 central differences.
 """
 
 from pathlib import Path
@@ -186,3 +188,51 @@ def test_batched_adjoint_workload_float64_candidate(hip, cfg5):
         p, q = divmod(int(flat), eps.shape[1])
         fd = (Jfd[2 * k] - Jfd[2 * k + 1]) / (2 * 1e-4 * eps[p, q])
         assert abs(g[p, q] - fd) / scale < 1e-3, (p, q, g[p, q], fd)
+
+
+def test_batched_adjoint_workload_at_full_size(hip):
+    """The benchmark's batch as it runs: 32 candidates, 8000 steps (125 checkpoint
+    segments, window checkpoints).  The first and last candidates' gradients equal their
+    own AdjointProblem's bit for bit (same tune), and candidate 0's float32 gradient
+    agrees with float64 central differences on 8 random pixels to 1e-3 of its largest
+    entry (the 16 perturbed float64 runs advanced as one batch)."""
+    from etalon_amd.adjoint import AdjointProblem
+    from etalon_amd.batch import BatchSolver
+    from etalon_amd.batch_adjoint import BatchAdjointProblem
+    from etalon_amd.hip.plan import Tune
+    from etalon_amd.workloads import phc_adjoint_candidates
+
+    sims, eps, names = phc_adjoint_candidates(n=32, seed=0, steps=8000)
+    tune = Tune(temporal=6, band_rows=16)
+    batch = BatchAdjointProblem([AdjointProblem(s, names, tune=tune) for s in sims], tune=tune)
+    assert (batch.n_steps, batch.K) == (8000, 64)
+    Js, grads = batch.run(eps)
+    batch.close()
+    for k in (0, 31):
+        J, g = AdjointProblem(sims[k], names, tune=tune, reuse=False).run(eps[k])
+        assert Js[k] == J, k
+        assert np.array_equal(grads[k], g), k
+    sims64, eps64, _ = phc_adjoint_candidates(n=1, seed=0, steps=8000, precision="double")
+    assert np.array_equal(eps64[0], eps[0])
+    prob64 = AdjointProblem(sims64[0], names)
+    scale = np.abs(grads[0]).max()
+    rng = np.random.default_rng(3)
+    picks = rng.choice(eps[0].size, size=8, replace=False)
+    runs = []
+    for flat in picks:
+        p, q = divmod(int(flat), eps[0].shape[1])
+        for sign in (1.0, -1.0):
+            e = eps[0].copy()
+            e[p, q] += sign * 1e-4 * eps[0][p, q]
+            runs.append(prob64.with_permittivity(e))
+    fd_batch = BatchSolver(runs)
+    fd_batch.run()
+    Jfd = [prob64._objective(s) for s in fd_batch.solvers]
+    worst = 0.0
+    for k, flat in enumerate(picks):
+        p, q = divmod(int(flat), eps[0].shape[1])
+        fd = (Jfd[2 * k] - Jfd[2 * k + 1]) / (2 * 1e-4 * eps[0][p, q])
+        rel = abs(grads[0][p, q] - fd) / scale
+        worst = max(worst, rel)
+        assert rel < 1e-3, (p, q, grads[0][p, q], fd)
+    print(f"32 candidates x 8000 steps: max |g32 - fd64| / max |g| = {worst:.2e}", flush=True)
