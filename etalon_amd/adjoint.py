@@ -64,13 +64,16 @@ def _forward_phasors(n_steps: int, dt: float, magnetic: bool, freqs: bytes,
 
 def recompute_tune(tune, precision: str):
     """The tune of an adjoint problem's recompute and reverse plans: the forward tune,
-    with an unset temporal depth 4 in double (large double forward plans default to
-    depth 6; the double reverse kernels are compiled up to depth 4)."""
+    in double with an unset temporal depth 4 and unset tile rows 8 (large double forward
+    plans default to depth 6 and 16-row tiles; the double reverse kernels, and the
+    double kernels pairing a recompute with reverse steps, are compiled up to depth 4
+    with 8-row forward tiles)."""
     from etalon_amd.hip.plan import Tune
 
-    if precision != "double" or (tune is not None and tune.temporal):
+    if precision != "double":
         return tune
-    return Tune(temporal=4) if tune is None else replace(tune, temporal=4)
+    t = tune if tune is not None else Tune()
+    return replace(t, temporal=t.temporal or 4, band_rows=t.band_rows or 8)
 
 
 def _weights(sampling) -> np.ndarray:

@@ -113,12 +113,12 @@ inline int strip_kind(int r0, int r1, int cs, int T, int W, int x_lo, int x_hi, 
 struct TileShape {
     int precision, vec, depth, band;
 };
-// Depth 6 (vec 1; 16-row tiles in float only) is compiled for the one-launch joint
-// kernels only.
+// Depth 6 (vec 1) is compiled for the one-launch joint kernels only.
 constexpr TileShape kTileShapes[] = {
     {0, 1, 1, 8},  {0, 1, 2, 8},  {0, 1, 4, 8},  {0, 1, 6, 8},  {0, 1, 1, 16}, {0, 1, 2, 16},
     {0, 1, 4, 16}, {0, 1, 6, 16}, {0, 2, 1, 8},  {0, 2, 2, 8},  {0, 2, 4, 8},  {1, 1, 1, 8},
     {1, 1, 2, 8},  {1, 1, 4, 8},  {1, 1, 6, 8},  {1, 1, 1, 16}, {1, 1, 2, 16},
+    {1, 1, 4, 16}, {1, 1, 6, 16},
 };
 inline bool tile_shape_supported(int precision, int vec, int depth, int band) {
     for (const TileShape& s : kTileShapes)

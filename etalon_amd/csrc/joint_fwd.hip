@@ -12,13 +12,13 @@ int launch_joint_fwd_any(const FwdSet<R>* f, int tf, int part, hipStream_t s) {
             case 4: return launch_joint_part<R, 4, 0>(f, nullptr, part, s);
         }
         if (tf == 6) return launch_joint_part<R, 6, 0>(f, nullptr, part, s);
-    } else if constexpr (sizeof(R) == 4) {  // 16-row interior tiles: float only
-        if (f->band == 16) switch (tf) {
-                case 1: return launch_joint_part<R, 1, 0, 16>(f, nullptr, part, s);
-                case 2: return launch_joint_part<R, 2, 0, 16>(f, nullptr, part, s);
-                case 4: return launch_joint_part<R, 4, 0, 16>(f, nullptr, part, s);
-                case 6: return launch_joint_part<R, 6, 0, 16>(f, nullptr, part, s);
-            }
+    } else if (f->band == 16) {
+        switch (tf) {
+            case 1: return launch_joint_part<R, 1, 0, 16>(f, nullptr, part, s);
+            case 2: return launch_joint_part<R, 2, 0, 16>(f, nullptr, part, s);
+            case 4: return launch_joint_part<R, 4, 0, 16>(f, nullptr, part, s);
+            case 6: return launch_joint_part<R, 6, 0, 16>(f, nullptr, part, s);
+        }
     }
     set_error("joint step: forward depth %d, band %d not compiled", tf, f->band);
     return ETL_EARG;
@@ -42,13 +42,13 @@ int launch_joint_fwd_empty_any(int tf, int band, hipStream_t s) {
             case 4: return launch_joint_empty<R, 4, 0, kAllKinds, 8>(1, s);
             case 6: return launch_joint_empty<R, 6, 0, kAllKinds, 8>(1, s);
         }
-    } else if constexpr (sizeof(R) == 4) {
-        if (band == 16) switch (tf) {
-                case 1: return launch_joint_empty<R, 1, 0, kAllKinds, 16>(1, s);
-                case 2: return launch_joint_empty<R, 2, 0, kAllKinds, 16>(1, s);
-                case 4: return launch_joint_empty<R, 4, 0, kAllKinds, 16>(1, s);
-                case 6: return launch_joint_empty<R, 6, 0, kAllKinds, 16>(1, s);
-            }
+    } else if (band == 16) {
+        switch (tf) {
+            case 1: return launch_joint_empty<R, 1, 0, kAllKinds, 16>(1, s);
+            case 2: return launch_joint_empty<R, 2, 0, kAllKinds, 16>(1, s);
+            case 4: return launch_joint_empty<R, 4, 0, kAllKinds, 16>(1, s);
+            case 6: return launch_joint_empty<R, 6, 0, kAllKinds, 16>(1, s);
+        }
     }
     set_error("joint step: forward depth %d, band %d not compiled", tf, band);
     return ETL_EARG;

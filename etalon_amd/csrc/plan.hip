@@ -1015,7 +1015,7 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     if (tune) t = *tune;
     if (t.vec == 0) t.vec = 1;
     const bool large = large_plan(descs, n_sims, t);
-    const int large_band = precision == ETL_F32 ? 16 : 8;
+    const int large_band = 16;
     if (t.temporal == 0)
         t.temporal = large && (t.band_rows == 0 || t.band_rows == large_band) ? 6 : 4;
     if (t.band_rows == 0) t.band_rows = large ? large_band : 8;
@@ -1035,10 +1035,8 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
                 "etl_plan_create: launch must be 0, 1 or 3 (got %d)", t.launch);
     ETL_REQUIRE(t.temporal != 6 || t.launch != 3,
                 "etl_plan_create: depth 6 runs with one launch per step block (launch 1)");
-    ETL_REQUIRE(t.launch != 1 || (t.vec == 1 && (t.band_rows == 8 ||
-                                                 (t.band_rows == 16 && precision == ETL_F32))),
-                "etl_plan_create: one launch per step block needs vec 1 and band 8 (or 16 in "
-                "float)");
+    ETL_REQUIRE(t.launch != 1 || (t.vec == 1 && (t.band_rows == 8 || t.band_rows == 16)),
+                "etl_plan_create: one launch per step block needs vec 1 and band 8 or 16");
 
     EtlPlan* p = new EtlPlan();
     p->precision = precision;
@@ -1049,9 +1047,7 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     // one launch per step block wherever the joint kernel is compiled for the shape
     if (rc == ETL_OK && p->tune.launch == 0)
         p->tune.launch =
-            (t.vec == 1 && (t.band_rows == 8 || (t.band_rows == 16 && precision == ETL_F32)))
-                ? 1
-                : 3;
+            (t.vec == 1 && (t.band_rows == 8 || t.band_rows == 16)) ? 1 : 3;
     // side streams and their events: only launch mode 3 (one kernel per tile kind)
     // forks onto them
     if (rc == ETL_OK && p->tune.launch == 3) {

@@ -57,6 +57,9 @@ int launch_fused<double>(const FusedArgs<double>* args, const TileList& tl, int 
     if (vec == 1 && depth == 2 && band == 16)
         return launch_shape<double, 1, 2, 16>(args, tl, waves, counter, kstep0, parity, s_int,
                                                 s_strip, s_extra);
+    if (vec == 1 && depth == 4 && band == 16)
+        return launch_shape<double, 1, 4, 16>(args, tl, waves, counter, kstep0, parity, s_int,
+                                                s_strip, s_extra);
     set_error("fused step: no compiled tile shape vec=%d depth=%d band=%d", vec, depth, band);
     return ETL_EARG;
 }

@@ -26,10 +26,10 @@ MAX_SOURCE_NODES = 16  # kMaxSources in csrc/fused.h, per simulation
 class Tune:
     """Fused-kernel shape; 0 = the library default: vec 1, one wave per workgroup,
     ``launch`` 1 (one kernel per step block over every tile kind) where the shape
-    allows it (vec 1 with band 8, or band 16 in float), else 3 (one kernel per tile
-    kind on three streams); for large plans (at least 8 tiles of 16 x 56 per CU)
-    temporal depth 6 with 96-step trace chunks and, in float, 16-row tiles; else
-    8-row tiles and depth 4 with 64-step chunks.  Depth 6 runs with launch 1 only."""
+    allows it (vec 1 with band 8 or 16), else 3 (one kernel per tile kind on three
+    streams); for large plans (at least 8 tiles of 16 x 56 per CU) temporal depth 6
+    with 96-step trace chunks and 16-row tiles; else 8-row tiles and depth 4 with
+    64-step chunks.  Depth 6 runs with launch 1 only."""
 
     band_rows: int = 0
     vec: int = 0

@@ -239,12 +239,14 @@ def test_reset_and_run_again_bitwise(hip, precision, dtype):
 # (vec, temporal depth, band rows): every compiled tile shape is its own code path
 TUNES_F32 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16), (1, 4, 16),
              (2, 1, 8), (2, 2, 8), (2, 4, 8)]
-TUNES_F64 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16)]
+TUNES_F64 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16), (1, 4, 16)]
 # launch 3: one kernel per tile kind on three streams; 1: one kernel per step block
 SHAPES = [("single", np.float32, *s, 3) for s in TUNES_F32] + \
     [("double", np.float64, *s, 3) for s in TUNES_F64] + \
     [(p, d, 1, t, 8, 1) for p, d in (("single", np.float32), ("double", np.float64))
-     for t in (1, 2, 4, 6)] + [("single", np.float32, 1, t, 16, 1) for t in (1, 2, 4, 6)]
+     for t in (1, 2, 4, 6)] + [(p, d, 1, t, 16, 1) for p, d in (("single", np.float32),
+                                                                 ("double", np.float64))
+                               for t in (1, 2, 4, 6)]
 
 
 @pytest.mark.parametrize(("precision", "dtype", "vec", "depth", "band", "launch"), SHAPES)
