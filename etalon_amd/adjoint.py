@@ -603,13 +603,15 @@ class AdjointProblem:
         ``rev``: the window's state at the reverse plan's depth, or the full state.
         ``rtune`` (default: this problem's) is the recompute plans' tune when set."""
         tune = self.rtune if rtune is False else rtune
-        if not self.window:
-            return solver.state, tune, None
-        win = self._window_of(solver)
         if tune is None and rev.kernel == "tile":
             from etalon_amd.hip.plan import Tune
 
-            tune = Tune(temporal=rev.temporal())  # step blocks that pair in a co-plan
+            # step blocks that pair in a co-plan (a forward plan's own default may be
+            # deeper than any reverse kernel)
+            tune = Tune(temporal=rev.temporal())
+        if not self.window:
+            return solver.state, tune, None
+        win = self._window_of(solver)
         return win.state, tune, win
 
     def _configure_recompute(self, plan, solver, sim: int, hist, win=None) -> None:

@@ -25,7 +25,8 @@ from etalon_amd.hip.plan import FusedPlan, Tune
 CANDIDATES = {
     "single": [(1, 4, 8, 1, 3), (1, 4, 8, 2, 3), (1, 4, 8, 4, 3), (1, 4, 16, 1, 3),
                (1, 4, 16, 2, 3), (2, 4, 8, 1, 3), (1, 2, 8, 1, 3), (1, 4, 8, 1, 1),
-               (1, 2, 8, 1, 1), (1, 4, 16, 1, 1), (1, 6, 8, 1, 1), (1, 6, 16, 1, 1)],
+               (1, 2, 8, 1, 1), (1, 4, 16, 1, 1), (1, 6, 8, 1, 1), (1, 6, 16, 1, 1),
+               (1, 8, 16, 1, 1)],
     "double": [(1, 4, 8, 1, 3), (1, 4, 8, 2, 3), (1, 4, 8, 4, 3), (1, 2, 16, 1, 3),
                (1, 4, 16, 1, 3), (1, 2, 8, 1, 3), (1, 4, 8, 1, 1), (1, 2, 8, 1, 1),
                (1, 4, 16, 1, 1), (1, 6, 8, 1, 1), (1, 6, 16, 1, 1)],

@@ -113,12 +113,13 @@ inline int strip_kind(int r0, int r1, int cs, int T, int W, int x_lo, int x_hi, 
 struct TileShape {
     int precision, vec, depth, band;
 };
-// Depth 6 (vec 1) is compiled for the one-launch joint kernels only.
+// Depths 6 and 8 (vec 1) are compiled for the one-launch joint kernels only (8: float
+// forward plans with 16-row tiles).
 constexpr TileShape kTileShapes[] = {
     {0, 1, 1, 8},  {0, 1, 2, 8},  {0, 1, 4, 8},  {0, 1, 6, 8},  {0, 1, 1, 16}, {0, 1, 2, 16},
     {0, 1, 4, 16}, {0, 1, 6, 16}, {0, 2, 1, 8},  {0, 2, 2, 8},  {0, 2, 4, 8},  {1, 1, 1, 8},
     {1, 1, 2, 8},  {1, 1, 4, 8},  {1, 1, 6, 8},  {1, 1, 1, 16}, {1, 1, 2, 16},
-    {1, 1, 4, 16}, {1, 1, 6, 16},
+    {1, 1, 4, 16}, {1, 1, 6, 16}, {0, 1, 8, 16},
 };
 inline bool tile_shape_supported(int precision, int vec, int depth, int band) {
     for (const TileShape& s : kTileShapes)
@@ -142,13 +143,13 @@ inline bool large_grids(const Desc* descs, int n_sims, int cu_count) {
 }
 
 // Depth of the next launch with r >= 1 steps left at plan depth T: T, or for a tail
-// of fewer than T steps the largest compiled smaller depth that fits (4, 2, then 1)
+// of fewer than T steps the largest compiled smaller depth that fits (6, 4, 2, then 1)
 // — a launch moves the bytes of a whole window whatever its depth, so a tail of
 // depth-1 launches would cost nearly a full block per step.  Forward and adjoint
 // plans of one depth decompose a step count alike, so a co-plan pairs their tails.
 inline int next_depth(int r, int T, int precision, int vec, int band) {
     if (r >= T) return T;
-    for (int d = 4; d >= 2; d /= 2)
+    for (int d : {6, 4, 2})
         if (d < T && d <= r && tile_shape_supported(precision, vec, d, band)) return d;
     return 1;
 }

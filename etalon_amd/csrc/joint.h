@@ -88,10 +88,11 @@ inline AdjSet<R> adj_set(const AdjTileArgs<R>* args, const TileList& tl, const i
 }
 
 // Depths launch_joint accepts for (forward, adjoint): (T, 0), (0, T), (T, T), T in 1, 2, 4,
-// and 6 in float (the dispatchers refuse what is not compiled).
+// and 6 in float, and forward-only 8 (float, 16-row tiles; the dispatchers refuse what is
+// not compiled).
 inline bool joint_depths_supported(int tf, int ta) {
     auto ok = [](int t) { return t == 1 || t == 2 || t == 4 || t == 6; };
-    return (ok(tf) && ta == 0) || (tf == 0 && ok(ta)) || (ok(tf) && tf == ta);
+    return ((ok(tf) || tf == 8) && ta == 0) || (tf == 0 && ok(ta)) || (ok(tf) && tf == ta);
 }
 
 // Instantiations, compiled in separate files: forward only, adjoint only, both.

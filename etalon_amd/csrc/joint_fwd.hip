@@ -1,4 +1,5 @@
-// One-launch forward step blocks (joint.h), float and double, depths 1, 2, 4, 6.
+// One-launch forward step blocks (joint.h), float and double, depths 1, 2, 4, 6, and 8
+// in float with 16-row tiles.
 #include "joint.h"
 
 namespace etl {
@@ -19,6 +20,8 @@ int launch_joint_fwd_any(const FwdSet<R>* f, int tf, int part, hipStream_t s) {
             case 4: return launch_joint_part<R, 4, 0, 16>(f, nullptr, part, s);
             case 6: return launch_joint_part<R, 6, 0, 16>(f, nullptr, part, s);
         }
+        if constexpr (sizeof(R) == 4)
+            if (tf == 8) return launch_joint_part<R, 8, 0, 16>(f, nullptr, part, s);
     }
     set_error("joint step: forward depth %d, band %d not compiled", tf, f->band);
     return ETL_EARG;
@@ -49,6 +52,8 @@ int launch_joint_fwd_empty_any(int tf, int band, hipStream_t s) {
             case 4: return launch_joint_empty<R, 4, 0, kAllKinds, 16>(1, s);
             case 6: return launch_joint_empty<R, 6, 0, kAllKinds, 16>(1, s);
         }
+        if constexpr (sizeof(R) == 4)
+            if (tf == 8) return launch_joint_empty<R, 8, 0, kAllKinds, 16>(1, s);
     }
     set_error("joint step: forward depth %d, band %d not compiled", tf, band);
     return ETL_EARG;

@@ -1003,6 +1003,17 @@ bool large_plan(const EtlTeDesc* descs, int n_sims, const EtlTune& t) {
 
 }  // namespace
 
+// Whether the float fields a step block streams (both buffers of Ex, Ey, Hz and cb_ex,
+// cb_ey: 32 bytes per cell) exceed the 256 MiB Infinity Cache, so that every launch
+// reads them from HBM.  There temporal depth 8 moves the same bytes per launch for 8
+// steps instead of 6 (8024^2: 73 against 86 us per step); where they fit, the cache
+// serves the reads and depth 8 measured level with 6 (2024^2: 5.86 us per step both).
+static bool beyond_cache(const EtlTeDesc* descs, int n_sims) {
+    int64_t cells = 0;
+    for (int k = 0; k < n_sims; ++k) cells += int64_t(descs[k].nx) * descs[k].ny;
+    return cells * 32 > (int64_t(256) << 20);
+}
+
 int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tune,
                           etl_plan_t* out) {
     ETL_CHECK_INIT();
@@ -1017,9 +1028,11 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     const bool large = large_plan(descs, n_sims, t);
     const int large_band = 16;
     if (t.temporal == 0)
-        t.temporal = large && (t.band_rows == 0 || t.band_rows == large_band) ? 6 : 4;
+        t.temporal = large && (t.band_rows == 0 || t.band_rows == large_band)
+                         ? (precision == ETL_F32 && beyond_cache(descs, n_sims) ? 8 : 6)
+                         : 4;
     if (t.band_rows == 0) t.band_rows = large ? large_band : 8;
-    if (t.trace_chunk == 0) t.trace_chunk = t.temporal == 6 ? 96 : 64;
+    if (t.trace_chunk == 0) t.trace_chunk = t.temporal >= 6 ? 96 : 64;
     if (t.waves == 0) t.waves = 1;
     if (tune == nullptr) t.use_graph = 1;
     ETL_REQUIRE(tile_shape_supported(precision, t.vec, t.temporal, t.band_rows) &&
@@ -1033,8 +1046,9 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
                 "etl_plan_create: waves per workgroup must be 1, 2 or 4 (got %d)", t.waves);
     ETL_REQUIRE(t.launch == 0 || t.launch == 1 || t.launch == 3,
                 "etl_plan_create: launch must be 0, 1 or 3 (got %d)", t.launch);
-    ETL_REQUIRE(t.temporal != 6 || t.launch != 3,
-                "etl_plan_create: depth 6 runs with one launch per step block (launch 1)");
+    ETL_REQUIRE(t.temporal < 6 || t.launch != 3,
+                "etl_plan_create: depth %d runs with one launch per step block (launch 1)",
+                t.temporal);
     ETL_REQUIRE(t.launch != 1 || (t.vec == 1 && (t.band_rows == 8 || t.band_rows == 16)),
                 "etl_plan_create: one launch per step block needs vec 1 and band 8 or 16");
 
@@ -1069,7 +1083,7 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     if (rc == ETL_OK) rc = etl_malloc(&p->trace_sims_dev, trace_bytes);
     // the depths its step blocks and tails launch, dispatched once empty (joint.h)
     if (rc == ETL_OK && p->tune.launch == 1)
-        for (int d : {p->tune.temporal, 4, 2, 1})
+        for (int d : {8, 6, 4, 2, 1})  // the plan's depth and its tails' (next_depth)
             if (rc == ETL_OK && d <= p->tune.temporal &&
                 tile_shape_supported(precision, 1, d, p->tune.band_rows))
                 rc = precision == ETL_F32

@@ -179,9 +179,12 @@ typedef struct EtlTune {
     int vec;           /* REAL elements per lane along y */
     int use_graph;     /* replay steps from a hipGraph (1) or launch directly (0) */
     int trace_chunk;   /* steps per DFT flush: a multiple of 2 * temporal (0: 64, or 96
-                        * at depth 6) */
+                        * at depths 6 and 8) */
     int temporal;      /* steps fused per launch: 1 (none), 2, 4, or 6 with launch 1
-                        * (band 8, or 16 in float); 0: 6 for large plans, else 4 */
+                        * (band 8, or 16 in float), or 8 in float with launch 1 and
+                        * band 16; 0: for large plans 8 in float when the fields exceed
+                        * the 256 MiB Infinity Cache (32 bytes per cell), else 6; 4 for
+                        * the others */
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
                         * band 8, or 16 in float; one wave per workgroup); 3: one kernel

@@ -171,9 +171,10 @@ def test_vacuum_pulse_2000_steps_bitwise(hip, precision, dtype):
 @pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
 def test_largest_grid_bitwise(hip, precision, dtype):
     """8024 x 8024 cells (64.4 M; the fields exceed the GPU's caches): the throughput
-    workload at 8000 x 8000 interior cells from random Ex, Ey, Hz (zero psi), 10 steps of
-    the default plan in two advances (depths 6, 1, then 2, 1), every field and the
-    monitor against the oracle, bit for bit."""
+    workload at 8000 x 8000 interior cells from random Ex, Ey, Hz (zero psi), 18 steps of
+    the default plan (depth 8 in float, 6 in double) in three advances (6 + 1, 2 + 1,
+    then one block of 8, or 6 + 2), every field and the monitor against the oracle, bit
+    for bit."""
     from etalon_amd.solver import FDTDSolver
     from etalon_amd.workloads import throughput_sim
 
@@ -189,9 +190,12 @@ def test_largest_grid_bitwise(hip, precision, dtype):
         getattr(gpu.state, name).set(host)
     gpu.advance(7)
     gpu.advance(3)
-    cpu.run(10)
-    assert gpu.n == cpu.n == 10
-    print(f"{precision} 8024^2: tune {gpu._plan.resolved_tune().__dict__}", flush=True)
+    gpu.advance(8)
+    cpu.run(18)
+    assert gpu.n == cpu.n == 18
+    tune = gpu._plan.resolved_tune()
+    print(f"{precision} 8024^2: tune {tune.__dict__}", flush=True)
+    assert tune.temporal == (8 if precision == "single" else 6)
     for name in FIELDS:
         value = getattr(cpu.s, name)
         assert np.abs(value).max() > 0, name
@@ -246,7 +250,8 @@ SHAPES = [("single", np.float32, *s, 3) for s in TUNES_F32] + \
     [(p, d, 1, t, 8, 1) for p, d in (("single", np.float32), ("double", np.float64))
      for t in (1, 2, 4, 6)] + [(p, d, 1, t, 16, 1) for p, d in (("single", np.float32),
                                                                  ("double", np.float64))
-                               for t in (1, 2, 4, 6)]
+                               for t in (1, 2, 4, 6)] + \
+    [("single", np.float32, 1, 8, 16, 1)]
 
 
 @pytest.mark.parametrize(("precision", "dtype", "vec", "depth", "band", "launch"), SHAPES)
