@@ -1003,15 +1003,23 @@ bool large_plan(const EtlTeDesc* descs, int n_sims, const EtlTune& t) {
 
 }  // namespace
 
-// Whether the float fields a step block streams (both buffers of Ex, Ey, Hz and cb_ex,
-// cb_ey: 32 bytes per cell) exceed the 256 MiB Infinity Cache, so that every launch
-// reads them from HBM.  There temporal depth 8 moves the same bytes per launch for 8
-// steps instead of 6 (8024^2: 73 against 86 us per step); where they fit, the cache
-// serves the reads and depth 8 measured level with 6 (2024^2: 5.86 us per step both).
-static bool beyond_cache(const EtlTeDesc* descs, int n_sims) {
-    int64_t cells = 0;
-    for (int k = 0; k < n_sims; ++k) cells += int64_t(descs[k].nx) * descs[k].ny;
-    return cells * 32 > (int64_t(256) << 20);
+// Whether a float plan runs temporal depth 8 by default: the fields a step block
+// streams (both buffers of Ex, Ey, Hz and cb_ex, cb_ey: 32 bytes per cell) exceed the
+// 256 MiB Infinity Cache, so that every launch reads them from HBM, and every grid has
+// at least 4 M cells, so that interior tiles dominate.  Depth 8 moves the same window
+// bytes per launch for 8 steps instead of 6 (8024^2: 73 against 86 us per step); where
+// the fields fit the cache it measured level with depth 6 (2024^2: 5.86 us per step
+// both), and on batches of smaller grids, a third of whose tiles are CPML strip and
+// source sub-tiles with the wider halo, slower (32 x 1024^2: 0.493 against 0.47 s for
+// 8000 steps).
+static bool deep_plan(const EtlTeDesc* descs, int n_sims) {
+    int64_t cells = 0, smallest = INT64_MAX;
+    for (int k = 0; k < n_sims; ++k) {
+        const int64_t c = int64_t(descs[k].nx) * descs[k].ny;
+        cells += c;
+        smallest = std::min(smallest, c);
+    }
+    return cells * 32 > (int64_t(256) << 20) && smallest >= (int64_t(4) << 20);
 }
 
 int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tune,
@@ -1029,7 +1037,7 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     const int large_band = 16;
     if (t.temporal == 0)
         t.temporal = large && (t.band_rows == 0 || t.band_rows == large_band)
-                         ? (precision == ETL_F32 && beyond_cache(descs, n_sims) ? 8 : 6)
+                         ? (precision == ETL_F32 && deep_plan(descs, n_sims) ? 8 : 6)
                          : 4;
     if (t.band_rows == 0) t.band_rows = large ? large_band : 8;
     if (t.trace_chunk == 0) t.trace_chunk = t.temporal >= 6 ? 96 : 64;

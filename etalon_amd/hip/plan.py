@@ -29,7 +29,8 @@ class Tune:
     allows it (vec 1 with band 8 or 16), else 3 (one kernel per tile kind on three
     streams); for large plans (at least 8 tiles of 16 x 56 per CU) temporal depth 6
     with 96-step trace chunks and 16-row tiles — depth 8 in float when the fields
-    (32 bytes per cell) exceed the 256 MiB Infinity Cache; else 8-row tiles and depth 4
+    (32 bytes per cell) exceed the 256 MiB Infinity Cache and every grid has at least
+    4 M cells; else 8-row tiles and depth 4
     with 64-step chunks.  Depths 6 and 8 run with launch 1 only."""
 
     band_rows: int = 0

@@ -183,8 +183,8 @@ typedef struct EtlTune {
     int temporal;      /* steps fused per launch: 1 (none), 2, 4, or 6 with launch 1
                         * (band 8, or 16 in float), or 8 in float with launch 1 and
                         * band 16; 0: for large plans 8 in float when the fields exceed
-                        * the 256 MiB Infinity Cache (32 bytes per cell), else 6; 4 for
-                        * the others */
+                        * the 256 MiB Infinity Cache (32 bytes per cell) and every grid
+                        * has at least 4 M cells, else 6; 4 for the others */
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
                         * band 8, or 16 in float; one wave per workgroup); 3: one kernel
