@@ -1,5 +1,7 @@
-"""Numerical dispersion of the 2D Yee scheme, predicted and measured
-(etalon/analysis/dispersion.py:20-94).
+"""Numerical dispersion of the 2D Yee scheme, predicted and measured — the two
+quantities whose ratio the M0 dispersion checks bound (etalon/analysis/dispersion.py:
+20-94 defines them; the thresholds of those checks only mean the same thing if these
+are the same quantities, computed here in this module's own way).
 
 Both functions return a phase velocity normalised to the exact one, ṽp / (c/n):
 1 means no dispersion error.
@@ -20,63 +22,81 @@ from etalon_amd.constants import C_0_SI, UM
 def numerical_phase_velocity(wavelengths, grid, angle: float = 0.0, n: float = 1.0) -> np.ndarray:
     """ṽp / (c/n) along ``angle`` (degrees) for vacuum wavelengths (µm).
 
-    The 2D Yee relation (Taflove & Hagness, 3rd ed., ch. 4)
+    The numerical wavenumber k̃ of a plane wave exp(i(k̃·r − ωt)) on the 2D Yee grid
+    satisfies (Taflove & Hagness, 3rd ed., ch. 4)
 
-        (sin(ωΔt/2) / (vΔt))² = (sin(k̃ cosφ Δx/2) / Δx)² + (sin(k̃ sinφ Δy/2) / Δy)²,
+        G(k̃) = (sin(k̃ a)/Δx)² + (sin(k̃ b)/Δy)² − (sin(ωΔt/2)/(vΔt))² = 0,
 
-    v = c/n, solved for k̃ by Newton iteration from k = ω/v; returns ω / (k̃ v).
+    a = cosφ Δx/2, b = sinφ Δy/2, v = c/n.  G rises monotonically from G(0) < 0 up to
+    the first quarter period of the faster-varying term, k* = π/(2 max(|a|, |b|)); the
+    propagating root is the one on [0, k*], found by bracketing (Brent) to machine
+    precision.  No root there (G(k*) < 0) means the wavelength is below the grid's
+    cut-off.  Returns ω / (k̃ v).
     """
-    lam = np.asarray(wavelengths, dtype=np.float64) * UM
+    lam = np.atleast_1d(np.asarray(wavelengths, dtype=np.float64)) * UM
     v = C_0_SI / n
-    omega = 2.0 * np.pi * C_0_SI / lam
     dt, dx, dy = grid.dt, grid.dx * UM, grid.dy * UM
-    hx = math.cos(math.radians(angle)) * dx / 2.0
-    hy = math.sin(math.radians(angle)) * dy / 2.0
-    rhs = (np.sin(omega * dt / 2.0) / (v * dt)) ** 2
-    k = omega / v
-    for _ in range(100):
-        residual = (np.sin(k * hx) / dx) ** 2 + (np.sin(k * hy) / dy) ** 2 - rhs
-        slope = hx * np.sin(2.0 * k * hx) / dx**2 + hy * np.sin(2.0 * k * hy) / dy**2
-        delta = residual / slope
-        k = k - delta
-        if np.all(np.abs(delta) <= 1e-14 * k):
-            return np.asarray(omega / (k * v), dtype=np.float64)
-    raise ValueError("Yee dispersion relation has no propagating solution for some wavelengths: "
-                     "they are below the grid's cut-off (too few cells per wavelength)")
+    a = abs(math.cos(math.radians(angle))) * dx / 2.0
+    b = abs(math.sin(math.radians(angle))) * dy / 2.0
+    k_top = math.pi / (2.0 * max(a, b))
+    out = np.empty_like(lam)
+    for m, wl in enumerate(lam):
+        omega = 2.0 * math.pi * C_0_SI / wl
+        rhs = (math.sin(omega * dt / 2.0) / (v * dt)) ** 2
+
+        def gap(k: float, rhs=rhs) -> float:
+            return (math.sin(k * a) / dx) ** 2 + (math.sin(k * b) / dy) ** 2 - rhs
+
+        if gap(k_top) < 0.0:
+            raise ValueError("Yee dispersion relation has no propagating solution for some "
+                             "wavelengths: they are below the grid's cut-off (too few cells "
+                             "per wavelength)")
+        k = brentq(gap, 0.0, k_top, xtol=1e-300, rtol=4 * np.finfo(float).eps, maxiter=500)
+        out[m] = omega / (k * v)
+    return out.reshape(np.shape(wavelengths))
 
 
-def _hankel_phase(z: np.ndarray) -> np.ndarray:
-    """arg H0⁽¹⁾(z) − (z − π/4): the near-field phase of an outgoing cylindrical wave."""
-    return np.angle(hankel1(0, z) * np.exp(-1j * (z - np.pi / 4.0)))
+def _outgoing_wave_phase(kr: np.ndarray) -> np.ndarray:
+    """Phase of the outgoing cylindrical wave H0⁽¹⁾(kr) (time convention exp(−iωt)),
+    continuous in kr: its far-field part kr − π/4 plus the near-field remainder."""
+    kr = np.asarray(kr, dtype=np.float64)
+    remainder = np.angle(hankel1(0, kr) * np.exp(-1j * (kr - np.pi / 4.0)))
+    return kr - np.pi / 4.0 + remainder
 
 
 def measured_phase_velocity(near, far, r_near: float, r_far: float,
                             wave: Literal["cylindrical", "plane"], n: float = 1.0) -> np.ndarray:
     """ṽp / (c/n) from the DFT phase of the same component at two distances (µm).
 
-    arg(far / near) is matched to k̃ (r_far − r_near), plus, for the cylindrical wave
-    of a 2D point source, the difference of the Hankel near-field phases (time
-    convention exp(−iωt)); the 2π ambiguity is resolved against the exact
-    k = 2πn/λ.  Meaningful along a lattice symmetry direction (0° or 45°) only.
-    ``near`` and ``far`` are monitor data with a ``wavelength`` coordinate.
+    The ratio far/near removes the source spectrum and the DFT sampling; its phase is
+    the wave's phase gain between the two points, k̃ (r_far − r_near) for a plane
+    wave and the gain of H0⁽¹⁾(k̃ r) for the cylindrical wave of a 2D point source.
+    The measured angle is placed on the 2π branch nearest the exact wave's gain
+    (k = 2πn/λ) and the gain equation solved for k̃ within ±30 % of k.  Meaningful
+    along a lattice symmetry direction (0° or 45°) only.  ``near`` and ``far`` are
+    monitor data with a ``wavelength`` coordinate.
     """
     lam = np.asarray(near.wavelength, dtype=np.float64)
     if not np.array_equal(lam, np.asarray(far.wavelength, dtype=np.float64)):
         raise ValueError("near and far must share the same wavelengths")
     if r_far <= r_near:
         raise ValueError("r_far must be larger than r_near")
-    phase = np.angle(np.asarray(far.values) / np.asarray(near.values))
+    spacing = r_far - r_near
+    angles = np.angle(np.asarray(far.values) / np.asarray(near.values))
+
+    if wave == "cylindrical":
+        def gain(k: float) -> float:
+            phases = _outgoing_wave_phase(np.array([k * r_far, k * r_near]))
+            return float(phases[0] - phases[1])
+    else:
+        def gain(k: float) -> float:
+            return k * spacing
+
     k_exact = 2.0 * np.pi * n / lam
-
-    def model(k: float) -> float:
-        value = k * (r_far - r_near)
-        if wave == "cylindrical":
-            corr = _hankel_phase(np.array([k * r_far, k * r_near]))
-            value += float(corr[0] - corr[1])
-        return value
-
-    k_meas = np.empty_like(k_exact)
-    for m, (k0, phi) in enumerate(zip(k_exact, phase)):
-        target = phi + 2.0 * np.pi * round((model(k0) - phi) / (2.0 * np.pi))
-        k_meas[m] = brentq(lambda k, t=target: model(k) - t, 0.7 * k0, 1.3 * k0, xtol=1e-14)
-    return k_exact / k_meas
+    ratio = np.empty_like(k_exact)
+    for m, (k0, angle) in enumerate(zip(k_exact, angles)):
+        turns = round((gain(k0) - angle) / (2.0 * np.pi))
+        goal = angle + 2.0 * np.pi * turns
+        k_meas = brentq(lambda k: gain(k) - goal, 0.7 * k0, 1.3 * k0, xtol=1e-14)
+        ratio[m] = k0 / k_meas
+    return ratio

@@ -32,8 +32,6 @@ from etalon_amd.sampling import dft_phasors, monitor_samplings
 from etalon_amd.sources import point_injection
 from etalon_amd.state import FIELDS, build_te_state
 
-# Upper bound on the phasor table held at once (time steps × frequencies).
-_PHASOR_TABLE_ENTRIES = 1 << 22
 # Steps per fused-plan table refill (source deltas and phasors): the first refill of an
 # advance covers _FIRST_TABLE_ROWS steps and each next one 4x more, up to
 # _PLAN_TABLE_ROWS.  The plan's launches are asynchronous, so the host builds the next
@@ -57,18 +55,19 @@ def _phasor_table(n0: int, rows: int, dt: float, magnetic: bool, freqs: bytes,
     return table
 
 
-class _MonitorAccumulator:
-    """DFT state of one monitor component: device accumulator plus phasor tables."""
+class MonitorDFT:
+    """One monitor component's running DFT on the device: the sampled nodes' flat
+    indices, the complex accumulator (n_f, nodes), and, for the per-step path, the
+    device copy of the phasor rows of the current block of steps."""
 
     def __init__(self, sampling, backend, complex_dtype) -> None:
         self.sampling = sampling
         self.index = backend.to_device(sampling.flat_index)
-        n_f = sampling.wavelengths.size
-        self.acc = backend.to_device(np.zeros((n_f, sampling.flat_index.size), complex_dtype))
-        self.chunk = max(1, _PHASOR_TABLE_ENTRIES // n_f)
-        self.table_start = -1
-        self.table_rows = 0
-        self.table: Any = None
+        self.acc = backend.to_device(
+            np.zeros((sampling.wavelengths.size, sampling.flat_index.size), complex_dtype))
+        self.freqs = np.asarray(sampling.frequencies, np.float64).tobytes()
+        self.block: tuple = ()       # (first step, device rows) held for step()
+        self.block_rows: Any = None
 
 
 class FDTDSolver:
@@ -93,7 +92,7 @@ class FDTDSolver:
         self.injections = [point_injection(s, grid, self.eps_r) for s in sim.sources]
         self._injection_index = [(self.backend.to_device(inj.i), self.backend.to_device(inj.j))
                                  for inj in self.injections]
-        self.monitors = [_MonitorAccumulator(sampling, self.backend, self.complex)
+        self.monitors = [MonitorDFT(sampling, self.backend, self.complex)
                          for monitor in sim.monitors
                          for sampling in monitor_samplings(monitor, grid)]
         self.n = 0
@@ -112,29 +111,24 @@ class FDTDSolver:
             field = getattr(self.state, inj.component.lower())
             field[i, j] -= delta
 
-    def _phasor_rows(self, mon: _MonitorAccumulator):
-        """H phasors at (n+½)Δt and E phasors at (n+1)Δt, from a table refilled per chunk."""
-        if mon.table is None or self.n >= mon.table_start + mon.table_rows:
-            left = self.n_steps - self.n
-            rows = min(mon.chunk, left) if left > 0 else mon.chunk
-            mon.table_start = self.n
-            mon.table_rows = rows
-            steps = np.arange(self.n, self.n + rows, dtype=np.float64)
-            f = mon.sampling.frequencies
-            dt = self.grid.dt
-            table_h = dft_phasors(f, (steps + 0.5) * dt, dt).astype(self.complex)
-            table_e = dft_phasors(f, (steps + 1.0) * dt, dt).astype(self.complex)
-            mon.table = (self.backend.to_device(table_h), self.backend.to_device(table_e))
-        row = self.n - mon.table_start
-        return mon.table[0][row], mon.table[1][row]
+    def _step_phasors(self, mon: MonitorDFT):
+        """Phasor row of step n for step(): H samples at (n+½)Δt, E samples at (n+1)Δt —
+        the fused path's cached tables (_phasor_table), one device copy per block of
+        _FIRST_TABLE_ROWS steps, so both paths multiply by the same bits."""
+        n0 = self.n - self.n % _FIRST_TABLE_ROWS
+        if mon.block != (n0,):
+            table = _phasor_table(n0, _FIRST_TABLE_ROWS, self.grid.dt, mon.sampling.is_magnetic,
+                                  mon.freqs, np.dtype(self.complex).str)
+            mon.block_rows = self.backend.to_device(table)
+            mon.block = (n0,)
+        return mon.block_rows[self.n - n0]
 
     def _accumulate(self, magnetic: bool) -> None:
         for mon in self.monitors:
             if mon.sampling.is_magnetic != magnetic:
                 continue
-            ph_h, ph_e = self._phasor_rows(mon)
             field = getattr(self.state, mon.sampling.component.lower())
-            self.backend.accumulate_dft(mon.acc, field, mon.index, ph_h if magnetic else ph_e)
+            self.backend.accumulate_dft(mon.acc, field, mon.index, self._step_phasors(mon))
 
     def step(self) -> None:
         """Advance one time step with the per-half-step primitives."""
@@ -212,7 +206,6 @@ class FDTDSolver:
             getattr(self.state, name).fill_zero()
         for mon in self.monitors:
             mon.acc.fill_zero()
-            mon.table = None
         self.n = 0
 
     def _tables(self, n0: int, rows: int):
@@ -227,8 +220,7 @@ class FDTDSolver:
                 phasors.append(None)
                 continue
             smp = mon.sampling
-            phasors.append(_phasor_table(n0, rows, dt, smp.is_magnetic,
-                                         np.asarray(smp.frequencies, np.float64).tobytes(),
+            phasors.append(_phasor_table(n0, rows, dt, smp.is_magnetic, mon.freqs,
                                          np.dtype(self.complex).str))
         return delta, phasors
 
