@@ -174,7 +174,7 @@ def run(precision: str, steps: int, warmup: int, interior: int = 2000) -> dict:
     cells = nx * ny
     # Kernel-level timing with HIP events: the same plan, launched directly.
     plan = solver._plan
-    n_time = 512  # a multiple of the trace chunk: every launch has the full depth
+    n_time = 480  # 5 trace chunks of 96, a multiple of depths 6 and 8: every launch is full
     delta, phasors = solver._tables(solver.n, n_time)
     if delta is not None:
         plan.source_table(solver.n, delta)
