@@ -88,12 +88,12 @@ REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
 # The counts belong to one build of the kernels: PMC_BUILD is the fingerprint of the
 # sources they were measured with (kernel_build_id), and a build with another
 # fingerprint reports no measured traffic until the counters are taken again.
-PMC_BUILD = "06a64d49176e48e7"
-PMC_KIB = {("single", 1, 6, 16, 1): (63734.6, 50014.0),
-           ("double", 1, 6, 16, 1): (143322.0, 106915.9)}
+PMC_BUILD = "804d4740a13daa89"
+PMC_KIB = {("single", 1, 6, 16, 1): (63712.9, 50017.5),
+           ("double", 1, 6, 16, 1): (143567.9, 106937.1)}
 FETCH_SCALE, WRITE_SCALE = 1.99985, 1.0
-PMC_KIB_8024 = {("single", 8, 16): (696241.9, 799415.3),
-                ("double", 6, 16): (1449187.5, 1576303.6)}
+PMC_KIB_8024 = {("single", 8, 16): (696308.8, 799426.7),
+                ("double", 6, 16): (1449191.4, 1576342.8)}
 # Infinity Cache (MALL) of the MI355X: a launch whose working set fits it has part of
 # its L2 misses served there, not by HBM.
 MALL_BYTES = 256 * 2**20
