@@ -213,12 +213,16 @@ def roofline(precision: str, r: dict, traffic: "int | None") -> dict:
     The bytes are the measured ones (rocprofv3 FETCH_SIZE and WRITE_SIZE of this build
     of the kernels) when the PMC table holds them, else the compulsory ones — every
     array the launch needs read once and every array it updates written once, 32 B per
-    cell in float32 and 64 in float64, a lower bound on its traffic.  The effective
+    cell in float32 and 64 in float64 (24 and 48 in the uniform-material interior tiles,
+    which read no cb array), a lower bound on its traffic.  The effective
     figure (the algorithmic bytes of the T unfused steps a launch advances) is kept
     beside it: temporal blocking puts it above the peak."""
     tm = r["timing"]
     kernel_s = tm["kernel_ms"] * 1e-3
-    compulsory = BYTES_PER_CELL[precision] * tm["kernel_cells"]
+    # cells of uniform-material tiles read no cb array: two REALs less per cell
+    cb_bytes = 2 * BYTES_PER_CELL[precision] // 8
+    compulsory = (BYTES_PER_CELL[precision] * tm["kernel_cells"]
+                  - cb_bytes * tm.get("uniform_cells", 0))
     moved = traffic if traffic is not None else compulsory
     gbs = moved / kernel_s / 1e9
     roof = {
@@ -236,6 +240,7 @@ def roofline(precision: str, r: dict, traffic: "int | None") -> dict:
                   "bound on its traffic"),
         "kernel_build": kernel_build_id(),
         "compulsory_bytes": compulsory,
+        "uniform_cells": tm.get("uniform_cells", 0),
         "compulsory_gbs": round(compulsory / kernel_s / 1e9, 1),
         "compulsory_frac": round(compulsory / kernel_s / 1e9 / HBM_PEAK_GBS, 4),
         "effective_gbs": round(r["kernel_bytes"] / kernel_s / 1e9, 1),

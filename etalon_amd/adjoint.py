@@ -62,6 +62,14 @@ def _forward_phasors(n_steps: int, dt: float, magnetic: bool, freqs: bytes,
     return Q
 
 
+def design_tune(tune):
+    """The tune of the plans of a design loop, whose cb arrays every gradient rewrites:
+    ``tune`` with uniform-material tiles off (each rewrite would lay the tiles out again)."""
+    from etalon_amd.hip.plan import Tune
+
+    return replace(tune if tune is not None else Tune(), uniform_cb=-1)
+
+
 def recompute_tune(tune, precision: str):
     """The tune of an adjoint problem's recompute and reverse plans: the forward tune,
     in double with an unset temporal depth 4 and unset tile rows 8 (large double forward
@@ -199,8 +207,8 @@ class AdjointProblem:
         # plan over this grid picks the same temporal depth, so a co-plan pairs its step
         # blocks; double reverse kernels run at depth 4 at most, and the recompute
         # follows them.
-        self.tune = tune
-        self.rtune = recompute_tune(tune, sim.precision)
+        self.tune = design_tune(tune)
+        self.rtune = recompute_tune(self.tune, sim.precision)
         grid = build_yee_grid(sim)
         self.grid = grid
         self.n_steps = grid.n_steps(sim.run_time)
@@ -603,12 +611,10 @@ class AdjointProblem:
         ``rev``: the window's state at the reverse plan's depth, or the full state.
         ``rtune`` (default: this problem's) is the recompute plans' tune when set."""
         tune = self.rtune if rtune is False else rtune
-        if tune is None and rev.kernel == "tile":
-            from etalon_amd.hip.plan import Tune
-
+        if rev.kernel == "tile" and (tune is None or tune.temporal == 0):
             # step blocks that pair in a co-plan (a forward plan's own default may be
             # deeper than any reverse kernel)
-            tune = Tune(temporal=rev.temporal())
+            tune = replace(design_tune(tune), temporal=rev.temporal())
         if not self.window:
             return solver.state, tune, None
         win = self._window_of(solver)

@@ -29,7 +29,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from etalon_amd.adjoint import AdjointProblem, _Reverse, recompute_tune
+from etalon_amd.adjoint import AdjointProblem, _Reverse, design_tune, recompute_tune
 from etalon_amd.batch import BatchSolver
 from etalon_amd.hip import _lib
 from etalon_amd.hip.backend import zeros
@@ -100,8 +100,8 @@ class BatchAdjointProblem:
         # "store": the forward pass keeps every step's Hz on each design box and the
         # reverse steps read it (no recompute; N x box nodes values per candidate)
         self.mode = p0.mode
-        self.tune = tune  # the forward pass
-        self.rtune = recompute_tune(tune, p0.sim.precision)  # the recompute and reverse
+        self.tune = design_tune(tune)  # the forward pass
+        self.rtune = recompute_tune(self.tune, p0.sim.precision)  # the recompute and reverse
         self.n_steps, self.K = p0.n_steps, p0.K
         # a design loop's gradients keep the batch, plans, graphs and buffers of the
         # first one: new permittivities only replace each candidate's cb_ex and cb_ey.

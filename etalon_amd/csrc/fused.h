@@ -79,12 +79,15 @@ struct TileList {
     int n_extra_strip;
     int n_strip_y;  // strip tiles meeting only y strips (columns)
     int n_strip_x;  // then those meeting only x strips (rows)
+    int n_uniform;  // the last n_uniform interior tiles have one cb_ex and one cb_ey value
+                    // over their window (the plan's uvals, two REALs each, in tile order)
 };
 
 // The tile list of descriptors stored in the order interior, strip (y, x, corner),
 // extra, extra-strip.
 inline TileList make_tile_list(const int4* t, int n_interior, int n_strip, int n_strip_y,
-                               int n_strip_x, int n_extra, int n_extra_strip) {
+                               int n_strip_x, int n_extra, int n_extra_strip,
+                               int n_uniform = 0) {
     return TileList{t,
                     n_interior,
                     t + n_interior,
@@ -94,7 +97,8 @@ inline TileList make_tile_list(const int4* t, int n_interior, int n_strip, int n
                     t + n_interior + n_strip + n_extra,
                     n_extra_strip,
                     n_strip_y,
-                    n_strip_x};
+                    n_strip_x,
+                    n_uniform};
 }
 
 // Which strips a sub-tile's window (rows [r0 - T, r1 + T), columns [cs, cs + W))

@@ -13,9 +13,10 @@
 
 namespace etl {
 
-// Tiles of a step block by kind: 0 interior (8 rows), then kSubBand-row sub-tiles:
-// 1 y-strip, 2 x-strip, 3 corner strip, 4 extra, 5 extra-strip.
-constexpr int kJointKinds = 6;
+// Tiles of a step block by kind: 0 interior (8 or 16 rows), then kSubBand-row
+// sub-tiles: 1 y-strip, 2 x-strip, 3 corner strip, 4 extra, 5 extra-strip; 6 interior
+// tiles of uniform material (forward plans: the last n_uniform interior tiles).
+constexpr int kJointKinds = 7;
 
 // A forward step block; `counter`, `kstep0`, `parity` as tile_kernel takes.
 template <typename R>
@@ -27,6 +28,7 @@ struct FwdSet {
     int kstep0;
     int parity;
     int band;  // rows of an interior tile (8, or 16 for float forward-only launches)
+    const void* uvals;  // kind 6: cb_ex, cb_ey of each tile (REAL pairs)
 };
 
 // An adjoint step block, the same kinds; `step`, `rtop`, `parity` as adj_tile_kernel
@@ -49,7 +51,7 @@ __host__ __device__ inline int joint_groups(int n) { return 8 * ((n + 7) / 8); }
 inline void split_kinds(const TileList& tl, const int4* (&t)[kJointKinds], int (&n)[kJointKinds]) {
     const int ny_ = tl.n_strip_y, nx_ = tl.n_strip_x, nxy = tl.n_strip - ny_ - nx_;
     t[0] = tl.interior;
-    n[0] = tl.n_interior;
+    n[0] = tl.n_interior - tl.n_uniform;
     t[1] = tl.strip;
     n[1] = ny_;
     t[2] = tl.strip + ny_;
@@ -60,11 +62,13 @@ inline void split_kinds(const TileList& tl, const int4* (&t)[kJointKinds], int (
     n[4] = tl.n_extra;
     t[5] = tl.extra_strip;
     n[5] = tl.n_extra_strip;
+    t[6] = tl.interior + n[0];
+    n[6] = tl.n_uniform;
 }
 
 template <typename R>
 inline FwdSet<R> fwd_set(const FusedArgs<R>* args, const TileList& tl, const int* counter,
-                         int kstep0, int parity, int band = 8) {
+                         int kstep0, int parity, int band = 8, const void* uvals = nullptr) {
     FwdSet<R> f{};
     f.args = args;
     split_kinds(tl, f.tiles, f.n);
@@ -72,6 +76,7 @@ inline FwdSet<R> fwd_set(const FusedArgs<R>* args, const TileList& tl, const int
     f.kstep0 = kstep0;
     f.parity = parity;
     f.band = band;
+    f.uvals = uvals;
     return f;
 }
 
@@ -123,7 +128,8 @@ int launch_joint(const FwdSet<R>* f, int tf, const AdjSet<R>* a, int ta, hipStre
 enum JointPart { kAllKinds = 0, kInteriorOnly = 1, kEdgeOnly = 2 };
 
 __host__ __device__ constexpr bool part_has(int part, int kind) {
-    return part == kAllKinds || (part == kInteriorOnly ? kind == 0 : kind != 0);
+    const bool interior = kind == 0 || kind == 6;
+    return part == kAllKinds || (part == kInteriorOnly ? interior : !interior);
 }
 
 // One kind K of a forward (adjoint) step block: when workgroup b falls in its range
@@ -135,8 +141,9 @@ __device__ __forceinline__ bool fwd_kind(const FwdSet<R>& f, int& b, R* lds) {
         b -= g;
         return false;
     }
-    tile_body<R, 1, T, BR, S, EXTRA>(f.args, f.tiles[K], f.n[K], b, f.n[K], 1, 0, f.counter,
-                                         f.kstep0, f.parity, lds);
+    tile_body<R, 1, T, BR, S, EXTRA, K == 6>(f.args, f.tiles[K], f.n[K], b, f.n[K], 1, 0,
+                                                 f.counter, f.kstep0, f.parity, lds,
+                                                 static_cast<const R*>(f.uvals));
     return true;
 }
 
@@ -166,6 +173,7 @@ __device__ __forceinline__ void joint_body(const FwdSet<R>& f, const AdjSet<R>& 
     if constexpr (!ETL_JOINT_EDGE_FIRST && part_has(PART, 0)) {
         if constexpr (TF > 0) {
             if (fwd_kind<R, TF, 0, JB, 0, false>(f, b, lds)) return;
+            if (fwd_kind<R, TF, 6, JB, 0, false>(f, b, lds)) return;
         }
         if constexpr (TA > 0) {
             if (adj_kind<R, TA, 0, 8, 0, false>(a, b)) return;
@@ -192,6 +200,7 @@ __device__ __forceinline__ void joint_body(const FwdSet<R>& f, const AdjSet<R>& 
     if constexpr (ETL_JOINT_EDGE_FIRST && part_has(PART, 0)) {
         if constexpr (TF > 0) {
             if (fwd_kind<R, TF, 0, JB, 0, false>(f, b, lds)) return;
+            if (fwd_kind<R, TF, 6, JB, 0, false>(f, b, lds)) return;
         }
         if constexpr (TA > 0) adj_kind<R, TA, 0, 8, 0, false>(a, b);
     }

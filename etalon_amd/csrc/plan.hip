@@ -178,6 +178,8 @@ struct EtlPlan {
     void* trace_sims_dev = nullptr;  // TraceSim<R>[n_sims]
     int4* tiles_dev = nullptr;       // interior, then strip, then extra tile descriptors
     int n_interior = 0, n_strip = 0, n_extra = 0, n_extra_strip = 0;
+    int n_uniform = 0;               // the last interior tiles, of uniform material
+    void* uvals = nullptr;           // their cb_ex, cb_ey: REAL pairs in tile order
     int n_strip_y = 0, n_strip_x = 0;  // the strip tiles: y only, then x only, then both
     int64_t interior_cells = 0;      // Hz cells the interior tiles own
     int4* pairs = nullptr;           // DFT pairs of every simulation
@@ -227,16 +229,90 @@ void free_layout(EtlPlan* p) {
     etl_free(p->pair_sim);
     etl_free(p->accs);
     etl_free(p->tiles_dev);
+    etl_free(p->uvals);
     p->tiles_dev = nullptr;
     p->pairs = nullptr;
     p->pair_sim = nullptr;
     p->accs = nullptr;
+    p->uvals = nullptr;
+    p->n_uniform = 0;
 }
 
 int upload(void** dst, const void* src, size_t bytes) {
     int rc = etl_malloc(dst, bytes);
     if (rc) return rc;
     return etl_memcpy_h2d(*dst, src, bytes);
+}
+
+// Uniform-material interior tiles: those whose window (rows [i0 - T, i0 + band + T),
+// columns [cs, cs + W), the cb rows an interior tile loads) holds one cb_ex and one
+// cb_ey value, bit for bit.  `tiles` (interior tiles, any simulations) is reordered
+// stably so that they come last; `vals` receives their two values each.  Reads every
+// simulation's cb arrays back once (the layout is rebuilt when they change).
+template <typename R>
+int split_uniform(EtlPlan* p, std::vector<int4>& tiles, std::vector<R>& vals) {
+    const int T = p->tune.temporal, band = p->tune.band_rows, W = 64 * p->tune.vec;
+    const int wo = p->wo();
+    std::vector<char> uniform(tiles.size(), 0);
+    std::vector<R> pair(2 * tiles.size(), R(0));
+    for (int si = 0; si < int(p->sims.size()); ++si) {
+        const auto& sim = p->sims[si];
+        bool any = false;
+        for (const int4& t : tiles) any = any || t.w == si;
+        if (!any) continue;
+        const int nx = sim.nx, ny = sim.ny, px = ny + 1;
+        std::vector<R> cx(size_t(nx) * px), cy(size_t(nx + 1) * ny);
+        int rc;
+        if ((rc = etl_memcpy_d2h(cx.data(), sim.desc.cb_ex, cx.size() * sizeof(R))) ||
+            (rc = etl_memcpy_d2h(cy.data(), sim.desc.cb_ey, cy.size() * sizeof(R))))
+            return rc;
+        // run[r * pitch + c]: how many elements from (r, c) on equal it, capped at W
+        auto runs = [W](const std::vector<R>& a, int rows, int pitch) {
+            std::vector<uint8_t> run(a.size());
+            for (int r = 0; r < rows; ++r) {
+                const R* row = a.data() + size_t(r) * pitch;
+                uint8_t* out = run.data() + size_t(r) * pitch;
+                out[pitch - 1] = 1;
+                for (int c = pitch - 2; c >= 0; --c)
+                    out[c] = std::memcmp(&row[c], &row[c + 1], sizeof(R)) == 0
+                                 ? uint8_t(std::min(W, out[c + 1] + 1))
+                                 : uint8_t(1);
+            }
+            return run;
+        };
+        const std::vector<uint8_t> rx = runs(cx, nx, px), ry = runs(cy, nx + 1, ny);
+        for (size_t k = 0; k < tiles.size(); ++k) {
+            const int4& t = tiles[k];
+            if (t.w != si) continue;
+            const int cs = t.y * wo - T;
+            const int r0 = t.x - T, r1 = t.x + band + T;
+            if (cs < 0 || cs + W > ny || r0 < 0 || r1 > nx) continue;  // never for interior
+            const R vx = cx[size_t(r0) * px + cs], vy = cy[size_t(r0) * ny + cs];
+            bool same = true;
+            for (int r = r0; r < r1 && same; ++r)
+                same = rx[size_t(r) * px + cs] >= W && ry[size_t(r) * ny + cs] >= W &&
+                       std::memcmp(&cx[size_t(r) * px + cs], &vx, sizeof(R)) == 0 &&
+                       std::memcmp(&cy[size_t(r) * ny + cs], &vy, sizeof(R)) == 0;
+            uniform[k] = same;
+            pair[2 * k] = vx;
+            pair[2 * k + 1] = vy;
+        }
+    }
+    std::vector<int4> mixed, flat;
+    vals.clear();
+    for (size_t k = 0; k < tiles.size(); ++k) {
+        if (uniform[k]) {
+            flat.push_back(tiles[k]);
+            vals.push_back(pair[2 * k]);
+            vals.push_back(pair[2 * k + 1]);
+        } else {
+            mixed.push_back(tiles[k]);
+        }
+    }
+    p->n_uniform = int(flat.size());
+    tiles.swap(mixed);
+    tiles.insert(tiles.end(), flat.begin(), flat.end());
+    return ETL_OK;
 }
 
 // Tile descriptors of the three kernel variants, the monitor CSR over extra tiles,
@@ -388,6 +464,22 @@ int build_layout(EtlPlan* p) {
             (rc = etl_malloc(&sim.trace, size_t(p->tune.trace_chunk) * sim.n_slots * p->elem())))
             return rc;
     }
+    // uniform-material interior tiles last, with their two cb values
+    if (p->tune.uniform_cb > 0 && !interior.empty()) {
+        int rc_u;
+        if (p->precision == ETL_F32) {
+            std::vector<float> vals;
+            rc_u = split_uniform<float>(p, interior, vals);
+            if (rc_u == ETL_OK && !vals.empty())
+                rc_u = upload(&p->uvals, vals.data(), vals.size() * sizeof(float));
+        } else {
+            std::vector<double> vals;
+            rc_u = split_uniform<double>(p, interior, vals);
+            if (rc_u == ETL_OK && !vals.empty())
+                rc_u = upload(&p->uvals, vals.data(), vals.size() * sizeof(double));
+        }
+        if (rc_u) return rc_u;
+    }
     std::vector<int4> tiles(interior);
     tiles.insert(tiles.end(), strip_y.begin(), strip_y.end());
     tiles.insert(tiles.end(), strip_x.begin(), strip_x.end());
@@ -520,7 +612,8 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
     hipStream_t s = lane_stream(p->lane);
     const int4* t = p->tiles_dev;
     const TileList tl = make_tile_list(t, p->n_interior, p->n_strip, p->n_strip_y,
-                                       p->n_strip_x, p->n_extra, p->n_extra_strip);
+                                       p->n_strip_x, p->n_extra, p->n_extra_strip,
+                                       p->n_uniform);
     const int T = p->tune.temporal;
     int k = 0, n = 0;
     while (k < count) {
@@ -531,8 +624,8 @@ int enqueue_block(EtlPlan* p, int count, int parity, hipEvent_t* events, int* la
                 ETL_HIP(hipEventRecord(events[4 * n], s));
                 ETL_HIP(hipEventRecord(events[4 * n + 1], s));
             }
-            const FwdSet<R> fs =
-                fwd_set<R>(args, tl, p->counter, k, parity ^ (n & 1), p->tune.band_rows);
+            const FwdSet<R> fs = fwd_set<R>(args, tl, p->counter, k, parity ^ (n & 1),
+                                            p->tune.band_rows, p->uvals);
             if (int rc = launch_joint<R>(&fs, depth, nullptr, 0, s)) return rc;
             if (events) {
                 ETL_HIP(hipEventRecord(events[4 * n + 2], s));
@@ -1043,6 +1136,8 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     if (t.trace_chunk == 0) t.trace_chunk = t.temporal >= 6 ? 96 : 64;
     if (t.waves == 0) t.waves = 1;
     if (tune == nullptr) t.use_graph = 1;
+    ETL_REQUIRE(t.uniform_cb >= -1 && t.uniform_cb <= 1, "etl_plan_create: uniform_cb %d",
+                t.uniform_cb);
     ETL_REQUIRE(tile_shape_supported(precision, t.vec, t.temporal, t.band_rows) &&
                     tile_shape_supported(precision, t.vec, 1, t.band_rows),
                 "etl_plan_create: no compiled tile shape vec=%d temporal=%d band_rows=%d",
@@ -1070,6 +1165,9 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
     if (rc == ETL_OK && p->tune.launch == 0)
         p->tune.launch =
             (t.vec == 1 && (t.band_rows == 8 || t.band_rows == 16)) ? 1 : 3;
+    // uniform-material tiles: the joint kernel's kind 6, so launch mode 1 only
+    if (p->tune.uniform_cb == 0) p->tune.uniform_cb = 1;
+    if (p->tune.launch != 1) p->tune.uniform_cb = -1;
     // side streams and their events: only launch mode 3 (one kernel per tile kind)
     // forks onto them
     if (rc == ETL_OK && p->tune.launch == 3) {
@@ -1115,6 +1213,13 @@ int etl_plan_destroy(etl_plan_t p) {
     ETL_CHECK_INIT();
     if (int rc_ = sync_lanes()) return rc_;
     destroy_plan(p);
+    return ETL_OK;
+}
+
+int etl_plan_touch(etl_plan_t p) {
+    ETL_CHECK_INIT();
+    ETL_REQUIRE(p != nullptr, "etl_plan_touch: null plan");
+    p->dirty = true;
     return ETL_OK;
 }
 
@@ -1458,6 +1563,7 @@ int etl_plan_time(etl_plan_t p, int64_t n, int64_t count, EtlTiming* out, int* p
         out->launches = launches;
         out->depth = p->tune.temporal;
         out->launch = p->tune.launch;
+        out->uniform_cells = int64_t(p->n_uniform) * p->tune.band_rows * p->wo();
         if (parity) *parity = p->parity;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);

@@ -263,13 +263,17 @@ __device__ __forceinline__ void trace_tile(const global_t<const int4v>* __restri
 // round-robin, so consecutive workgroup indices are spread XCD-major: each XCD
 // receives a contiguous range of tiles, whose neighbours share its L2.  EXTRA tiles
 // synchronise the workgroup (one wave per workgroup) and stage rows through `lds`
-// (BR * 64 * V REALs).
-template <typename R, int V, int T, int BR, int S, bool EXTRA>
+// (BR * 64 * V REALs).  UCB tiles (interior only) have one cb_ex and one cb_ey value
+// over their whole window, uvals[2 li] and uvals[2 li + 1]: they load no cb row and
+// multiply by those values, the same bits the rows hold.
+template <typename R, int V, int T, int BR, int S, bool EXTRA, bool UCB = false>
 __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                                           const int4* __restrict__ tiles, int n_tiles, int b,
                                           int n_groups, int waves, int wave,
                                           const int* __restrict__ counter, int kstep0, int parity,
-                                          R* __restrict__ lds) {
+                                          R* __restrict__ lds,
+                                          const R* __restrict__ uvals = nullptr) {
+    static_assert(!UCB || (S == 0 && !EXTRA), "uniform-material tiles are interior tiles");
     constexpr int RW = BR + 2 * T;  // window rows (Ey: one more)
     // S: which CPML strips the window meets — bit 0 the y strips (columns), bit 1 the x
     // strips (rows, and the grid's first and last rows).  Only those psi arrays are
@@ -344,8 +348,10 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
         ldr<R, V>(ex[l], rx(exA, gx), offx);
         ldr<R, V>(hz[l], ry(hzA, gx), offy);
         ldr<R, V>(ey[l], ry(eyA, ge), offy);
-        ldr<R, V>(cx[l], rx(cbx, gx), offx);
-        ldr<R, V>(cy[l], ry(cby, ge), offy);
+        if constexpr (!UCB) {
+            ldr<R, V>(cx[l], rx(cbx, gx), offx);
+            ldr<R, V>(cy[l], ry(cby, ge), offy);
+        }
         if constexpr (SY) {
             ldr<R, V>(phy[SY ? l : 0], ry(phyA, gx), offy);
             ldr<R, V>(pey[SY ? l : 0], rx(pexA, gx), offx);
@@ -362,6 +368,11 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
     for (int l = 0; l < first_rows; ++l) load_row(l);
 
     const R db = a.db, idx = a.inv_dx, idy = a.inv_dy;
+    R ucx = R(0), ucy = R(0);  // the window's cb_ex and cb_ey (UCB tiles)
+    if constexpr (UCB) {
+        ucx = to_global(uvals)[2 * li];
+        ucy = to_global(uvals)[2 * li + 1];
+    }
 
     // y-strip membership and profiles of this lane's columns
     bool mH[V], ysh[V], yse[V];
@@ -511,7 +522,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                 } else {
                     tt = d;
                 }
-                const R exn = ex[l][k] + cx[l][k] * tt;
+                const R exn = ex[l][k] + (UCB ? ucx : cx[l][k]) * tt;
                 ex[l][k] = upd ? exn : ex[l][k];
             }
             if (ey_row) {  // Ey(g, j), g in [1, nx - 1]; rows 0 and nx stay unchanged
@@ -524,7 +535,7 @@ __device__ __forceinline__ void tile_body(const FusedArgs<R>* __restrict__ args,
                 } else {
                     tt = d;
                 }
-                ey[l][k] = ey[l][k] - cy[l][k] * tt;
+                ey[l][k] = ey[l][k] - (UCB ? ucy : cy[l][k]) * tt;
             }
         }
         if constexpr (EXTRA) {

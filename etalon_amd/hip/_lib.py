@@ -51,6 +51,7 @@ class EtlTune(C.Structure):
         ("temporal", C.c_int),
         ("waves", C.c_int),
         ("launch", C.c_int),
+        ("uniform_cb", C.c_int),
     ]
 
 
@@ -86,6 +87,7 @@ class EtlTiming(C.Structure):
         ("launches", C.c_int),
         ("depth", C.c_int),
         ("launch", C.c_int),
+        ("uniform_cells", C.c_int64),
     ]
 
 
@@ -117,6 +119,7 @@ _SIGNATURES = {
     "etl_downsample": [C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp],
     "etl_plan_create": [C.POINTER(EtlTeDesc), C.POINTER(EtlTune), C.POINTER(vp)],
     "etl_plan_destroy": [vp],
+    "etl_plan_touch": [vp],
     "etl_plan_set_sources": [vp, C.c_int, C.POINTER(C.c_int), i64p, i64p],
     "etl_plan_source_table": [vp, C.c_int64, C.c_int, vp],
     "etl_plan_add_monitor": [vp, C.c_int, C.c_int, i64p, C.c_int, vp, C.POINTER(C.c_int)],

@@ -31,7 +31,10 @@ class Tune:
     with 96-step trace chunks and 16-row tiles — depth 8 in float when the fields
     (32 bytes per cell) exceed the 256 MiB Infinity Cache and every grid has at least
     4 M cells; else 8-row tiles and depth 4
-    with 64-step chunks.  Depths 6 and 8 run with launch 1 only."""
+    with 64-step chunks.  Depths 6 and 8 run with launch 1 only.  ``uniform_cb`` (launch
+    1; 0 = on, -1 = off): interior tiles whose window holds one cb_ex and one cb_ey value
+    read those two values instead of the cb rows (plans whose cb arrays are rewritten
+    every run, as in a design loop, turn it off: each write lays the tiles out again)."""
 
     band_rows: int = 0
     vec: int = 0
@@ -40,10 +43,11 @@ class Tune:
     temporal: int = 0
     waves: int = 0
     launch: int = 0
+    uniform_cb: int = 0
 
     def as_struct(self) -> _lib.EtlTune:
         return _lib.EtlTune(self.band_rows, self.vec, self.use_graph, self.trace_chunk,
-                            self.temporal, self.waves, self.launch)
+                            self.temporal, self.waves, self.launch, self.uniform_cb)
 
 
 def _describe(state, pair: dict) -> _lib.EtlTeDesc:
@@ -84,6 +88,18 @@ class FusedPlan:
         self.handle = handle
         self.parity = 0
         self.n_src = [0] * len(self.states)
+        self._cb_marks = self._material_marks()
+
+    def _material_marks(self) -> tuple:
+        return tuple((s.cb_ex.writes, s.cb_ey.writes) for s in self.states)
+
+    def _check_materials(self) -> None:
+        """cb_ex or cb_ey written from the host since the plan laid out its tiles: lay
+        them out again (uniform-material tiles read the cb values of the layout)."""
+        marks = self._material_marks()
+        if marks != self._cb_marks:
+            _lib.call("etl_plan_touch", self.handle)
+            self._cb_marks = marks
 
     def close(self) -> None:
         if self.handle:
@@ -143,6 +159,7 @@ class FusedPlan:
 
     def advance(self, n0: int, count: int, probe_row0: int = 0) -> int:
         """Steps [n0, n0 + count); probes write rows probe_row0 .. probe_row0 + count - 1."""
+        self._check_materials()
         parity = C.c_int()
         _lib.call("etl_plan_advance_rows", self.handle, int(n0), int(count), int(probe_row0),
                   C.byref(parity))
@@ -173,6 +190,7 @@ class FusedPlan:
     def advance_checkpointed(self, n0: int, count: int, every: int) -> int:
         """Steps [n0, n0 + count), the state saved into slot c // every before each
         segment of ``every`` steps starting at step c."""
+        self._check_materials()
         parity = C.c_int()
         _lib.call("etl_plan_advance_checkpointed", self.handle, int(n0), int(count), int(every),
                   C.byref(parity))
@@ -186,13 +204,15 @@ class FusedPlan:
     def prepare(self) -> None:
         """Build the layout (tile work-list, monitor slots) now rather than at the
         first table upload or advance."""
+        self._check_materials()
         _lib.call("etl_plan_prepare", self.handle)
 
     def resolved_tune(self) -> Tune:
         """The tuning in effect (library defaults filled in)."""
         t = _lib.EtlTune()
         _lib.call("etl_plan_tune", self.handle, C.byref(t))
-        return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal, t.waves, t.launch)
+        return Tune(t.band_rows, t.vec, t.use_graph, t.trace_chunk, t.temporal, t.waves, t.launch,
+                    t.uniform_cb)
 
     def time(self, n0: int, count: int) -> dict:
         """HIP-event timing of ``count`` steps launched directly: ``launch_ms`` (average
@@ -201,6 +221,7 @@ class FusedPlan:
         interior-tile kernel alone), ``total_ms``, ``kernel_cells`` (Hz cells that
         kernel advances per step), ``launches``, ``depth`` (steps per launch) and
         ``launch``."""
+        self._check_materials()
         t, parity = _lib.EtlTiming(), C.c_int()
         _lib.call("etl_plan_time", self.handle, int(n0), int(count), C.byref(t), C.byref(parity))
         self.parity = parity.value

@@ -190,6 +190,11 @@ typedef struct EtlTune {
                         * band 8, or 16 in float; one wave per workgroup); 3: one kernel
                         * per tile kind on three streams; 0: 1 where the shape allows
                         * it, else 3 */
+    int uniform_cb;    /* launch 1: interior tiles whose window holds one cb_ex and one
+                        * cb_ey value (bit for bit; found when the plan lays out its tiles)
+                        * read those two values instead of the cb rows.  0: on (1) with
+                        * launch 1, -1: off.  After writing cb_ex or cb_ey of a plan's
+                        * simulation, call etl_plan_touch. */
 } EtlTune;
 
 /* Components: 0 = Ex, 1 = Ey, 2 = Hz. */
@@ -203,6 +208,10 @@ int etl_plan_create(const EtlTeDesc* desc, const EtlTune* tune, etl_plan_t* out)
 int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tune,
                           etl_plan_t* out);
 int etl_plan_destroy(etl_plan_t plan);
+/* The plan's cb_ex or cb_ey arrays were written: lay out the tiles again (the uniform-
+ * material tiles, EtlTune.uniform_cb) before the next step.  No reference counterpart
+ * (the reference has no plan). */
+int etl_plan_touch(etl_plan_t plan);
 /* Point sources: node (comp, i, j) receives field -= delta[n][k] in step n; a node
  * listed more than once is updated once per listing, in list order. */
 int etl_plan_set_sources(etl_plan_t plan, int n_nodes, const int* comp, const int64_t* i,
@@ -285,6 +294,8 @@ typedef struct EtlTiming {
     int launches;            /* fused launches timed */
     int depth;               /* steps per launch (the plan's temporal depth) */
     int launch;              /* the plan's launch mode (1 or 3) */
+    int64_t uniform_cells;   /* of kernel_cells, those the uniform-material interior tiles
+                              * own (EtlTune.uniform_cb): they read no cb array */
 } EtlTiming;
 int etl_plan_time(etl_plan_t plan, int64_t n, int64_t count, EtlTiming* out, int* parity);
 
