@@ -88,12 +88,12 @@ REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
 # The counts belong to one build of the kernels: PMC_BUILD is the fingerprint of the
 # sources they were measured with (kernel_build_id), and a build with another
 # fingerprint reports no measured traffic until the counters are taken again.
-PMC_BUILD = "804d4740a13daa89"
-PMC_KIB = {("single", 1, 6, 16, 1): (63712.9, 50017.5),
-           ("double", 1, 6, 16, 1): (143567.9, 106937.1)}
+PMC_BUILD = "05bfaa8b23f6d586"
+PMC_KIB = {("single", 1, 6, 16, 1): (39076.3, 49871.8),
+           ("double", 1, 6, 16, 1): (93860.4, 106132.5)}
 FETCH_SCALE, WRITE_SCALE = 1.99985, 1.0
-PMC_KIB_8024 = {("single", 8, 16): (696308.8, 799426.7),
-                ("double", 6, 16): (1449191.4, 1576342.8)}
+PMC_KIB_8024 = {("single", 8, 16): (440907.8, 795899.6),
+                ("double", 6, 16): (905126.3, 1562534.8)}
 # Infinity Cache (MALL) of the MI355X: a launch whose working set fits it has part of
 # its L2 misses served there, not by HBM.
 MALL_BYTES = 256 * 2**20
@@ -883,8 +883,10 @@ def main() -> int:
         line["hbm_resident"] = hbm_resident(args.precision, args.hbm_steps)
     if not args.no_extras:
         other = "double" if args.precision == "single" else "single"
-        line[f"{'f64' if other == 'double' else 'f32'}"] = precision_summary(
-            other, args.steps, args.warmup, args.interior)
+        summary = precision_summary(other, args.steps, args.warmup, args.interior)
+        if args.hbm_steps > 0:
+            summary["hbm_resident"] = hbm_resident(other, args.hbm_steps)
+        line[f"{'f64' if other == 'double' else 'f32'}"] = summary
         line["batched"] = batched_summary(args)
         line["adjoint"] = adjoint_summary(args)
     if not args.no_cpu_baseline:
