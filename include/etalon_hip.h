@@ -166,12 +166,12 @@ typedef struct EtlTeDesc {
     const void *by_e, *cy_e, *iky_e, *by_h, *cy_h, *iky_h;
 } EtlTeDesc;
 
-/* Shape of the fused kernel (0 = library default: vec 1, band 8 or 16, temporal 4 or
- * 6, graph on, 64- or 96-step trace chunks, one wave per workgroup).  (vec, temporal,
- * band_rows) must be a compiled register-tile shape: float32 vec 1 with band 8 or
- * 16, or vec 2 with band 8, at temporal depth 1, 2 or 4, and float32 vec 1 at depth 6
- * (band 8 or 16, launch 1); float64 vec 1 with band 8 (depth 1, 2, 4, and 6 with
- * launch 1) or band 16 (depth 1, 2). */
+/* Shape of the fused kernel (0 = library default: vec 1, band 8 or 16, temporal 4, 6
+ * or 8, graph on, 64- or 96-step trace chunks, one wave per workgroup).  (vec,
+ * temporal, band_rows) must be a compiled register-tile shape: float32 vec 1 with band
+ * 8 or 16 at temporal depth 1, 2, 4 or 6, and with band 16 at depth 8; float32 vec 2
+ * with band 8 at depth 1, 2 or 4; float64 vec 1 with band 8 or 16 at depth 1, 2, 4 or
+ * 6.  Depths 6 and 8 run with launch 1 only. */
 typedef struct EtlTune {
     int band_rows;     /* output rows of one wave's tile; 0: 16 for large float plans
                         * (vec 1, not launch 3, at least 8 tiles of 16 x 56 per CU),
@@ -181,13 +181,12 @@ typedef struct EtlTune {
     int trace_chunk;   /* steps per DFT flush: a multiple of 2 * temporal (0: 64, or 96
                         * at depths 6 and 8) */
     int temporal;      /* steps fused per launch: 1 (none), 2, 4, or 6 with launch 1
-                        * (band 8, or 16 in float), or 8 in float with launch 1 and
-                        * band 16; 0: for large plans 8 in float when the fields exceed
+                        * (band 8 or 16), or 8 in float with launch 1 and band 16; 0: for large plans 8 in float when the fields exceed
                         * the 256 MiB Infinity Cache (32 bytes per cell) and every grid
                         * has at least 4 M cells, else 6; 4 for the others */
     int waves;         /* tiles (waves) per workgroup of the interior/strip kernels: 1, 2, 4 */
     int launch;        /* 1: one kernel per step block covering every tile kind (vec 1,
-                        * band 8, or 16 in float; one wave per workgroup); 3: one kernel
+                        * band 8 or 16; one wave per workgroup); 3: one kernel
                         * per tile kind on three streams; 0: 1 where the shape allows
                         * it, else 3 */
     int uniform_cb;    /* launch 1: interior tiles whose window holds one cb_ex and one
