@@ -88,12 +88,12 @@ REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
 # The counts belong to one build of the kernels: PMC_BUILD is the fingerprint of the
 # sources they were measured with (kernel_build_id), and a build with another
 # fingerprint reports no measured traffic until the counters are taken again.
-PMC_BUILD = "05bfaa8b23f6d586"
-PMC_KIB = {("single", 1, 6, 16, 1): (39076.3, 49871.8),
-           ("double", 1, 6, 16, 1): (93860.4, 106132.5)}
+PMC_BUILD = "996121ae896827e4"
+PMC_KIB = {("single", 1, 6, 16, 1): (39156.1, 49862.2),
+           ("double", 1, 6, 16, 1): (93238.7, 106642.7)}
 FETCH_SCALE, WRITE_SCALE = 1.99985, 1.0
-PMC_KIB_8024 = {("single", 8, 16): (440907.8, 795899.6),
-                ("double", 6, 16): (905126.3, 1562534.8)}
+PMC_KIB_8024 = {("single", 8, 16): (439709.6, 795824.4),
+                ("double", 6, 16): (902790.5, 1563610.3)}
 # Infinity Cache (MALL) of the MI355X: a launch whose working set fits it has part of
 # its L2 misses served there, not by HBM.
 MALL_BYTES = 256 * 2**20

@@ -194,6 +194,12 @@ struct EtlPlan {
     bool ck_stale = true;
 
     int* counter = nullptr;
+    // What the device counter holds once the work enqueued so far has run, when the
+    // host knows it: counter[0] == counter_at (-1: unknown), counter[1] == 0,
+    // counter[2] == counter_probe.  An advance that starts there needs no counter set.
+    int64_t counter_at = -1;
+    int64_t counter_next = -1;  // counter[0] after the steps being enqueued
+    int counter_probe = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipGraph_t graph_src[2] = {nullptr, nullptr};
 
@@ -798,16 +804,27 @@ int prepare_steps(EtlPlan* p, int64_t n, int64_t count, int probe_row0,
                         (long long)count, probe_row0, pr.out_rows);
     const int64_t start = n - n0;
     ETL_REQUIRE(start >= 0 && start + count < (int64_t(1) << 31), "etl_plan_advance: step range");
-    if (counter_out) {
+    const int64_t known = p->counter_at;
+    p->counter_at = -1;  // until the steps are enqueued (run_steps)
+    p->counter_next = start + count;
+    if (counter_out) {   // the co-plan's kernels set and advance the counter
         counter_out[0] = int(start);
         counter_out[1] = 0;
         counter_out[2] = int(start) - probe_row0;
         return ETL_OK;
     }
+    // The DFT (or bump) kernel that ends the previous advance leaves counter[0] at its
+    // end step and counter[1] at 0: an advance that continues from there, with the same
+    // probe offset or no probes, finds the counter set.
+    bool probes = false;
+    for (const auto& sim : p->sims) probes = probes || !sim.probes.empty();
+    if (known == start && (!probes || p->counter_probe == int(start) - probe_row0))
+        return ETL_OK;
     hipStream_t s = lane_stream(p->lane);
     hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(64), 0, s, p->counter, int(start),
                        probe_row0);
     ETL_LAUNCHED();
+    p->counter_probe = int(start) - probe_row0;
     return ETL_OK;
 }
 
@@ -837,6 +854,7 @@ int run_steps(EtlPlan* p, int64_t n, int64_t count, hipEvent_t* events, int* n_l
         total += launches;
     }
     if (n_launch) *n_launch = total;
+    p->counter_at = p->counter_next;  // each block's DFT or bump kernel added its rows
     return ETL_OK;
 }
 

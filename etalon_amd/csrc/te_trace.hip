@@ -13,34 +13,41 @@ namespace {
 
 // acc[pair] += sum over the chunk's rows, in step order, of trace * phasor.  The
 // rows are read in batches of B independent loads before the sequential fold, so
-// the memory latency is paid once per batch.  counter[0] is the step of the chunk's
-// first row; the last block to finish advances it by `rows` (every block has read it
-// by then) and clears the block tally counter[1] for the next chunk.
+// the memory latency is paid once per batch; the last batch reads rows past the end
+// as the last row again (in bounds, folded in by no one), so a chunk of up to B rows
+// waits for memory once.  counter[0] is the step of the chunk's first row; the last
+// block to finish advances it by `rows` (every block has read it by then) and clears
+// the block tally counter[1] for the next chunk.
 template <typename R>
 __global__ __launch_bounds__(256) void trace_dft_kernel(const TraceSim<R>* __restrict__ sims,
                                                         int rows, int* __restrict__ counter,
                                                         const int4* __restrict__ pairs,
                                                         const int* __restrict__ pair_sim,
                                                         int n_pairs, R* const* __restrict__ accs) {
-    constexpr int B = sizeof(R) == 4 ? 32 : 16;
+    constexpr int B = 32;  // two loads per row (trace value, phasor pair): 64 in flight
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int base = __atomic_load_n(counter, __ATOMIC_RELAXED);
     if (p < n_pairs) {
         const int4 pr = pairs[p];  // slot, q, accumulator, complex element index
         const TraceSim<R> sim = sims[pair_sim[p]];
         const int n_slots = sim.n_slots, nq = sim.nq;
-        R* acc = accs[pr.z] + 2 * int64_t(pr.w);
+        // global-address-space views: the pointers come from memory, where the compiler
+        // cannot see their address space, and flat loads would also wait on the LDS and
+        // scalar counters
+        using GW = __attribute__((address_space(1))) R;
+        using GR = __attribute__((address_space(1))) const R;
+        GW* acc = (GW*)(accs[pr.z] + 2 * int64_t(pr.w));
+        GR* __restrict__ php = (GR*)(sim.ph + 2 * (int64_t(base) * nq + pr.y));
+        GR* __restrict__ tp = (GR*)(sim.trace + pr.x);
         R re = acc[0], im = acc[1];
-        const R* __restrict__ php = sim.ph + 2 * (int64_t(base) * nq + pr.y);
-        const R* __restrict__ tp = sim.trace + pr.x;
-        int r = 0;
-        for (; r + B <= rows; r += B) {
+        int r0 = 0;
+        for (; r0 + B <= rows; r0 += B) {
             R v[B], pre[B], pim[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                v[b] = tp[int64_t(r + b) * n_slots];
-                pre[b] = php[2 * int64_t(r + b) * nq];
-                pim[b] = php[2 * int64_t(r + b) * nq + 1];
+                v[b] = tp[int64_t(r0 + b) * n_slots];
+                pre[b] = php[2 * int64_t(r0 + b) * nq];
+                pim[b] = php[2 * int64_t(r0 + b) * nq + 1];
             }
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -48,10 +55,23 @@ __global__ __launch_bounds__(256) void trace_dft_kernel(const TraceSim<R>* __res
                 im = im + v[b] * pim[b];
             }
         }
-        for (; r < rows; ++r) {
-            const R v = tp[int64_t(r) * n_slots];
-            re = re + v * php[2 * int64_t(r) * nq];
-            im = im + v * php[2 * int64_t(r) * nq + 1];
+        if (r0 < rows) {  // the last rows, fewer than B
+            R v[B], pre[B], pim[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int64_t r = min(r0 + b, rows - 1);
+                v[b] = tp[r * n_slots];
+                pre[b] = php[2 * r * nq];
+                pim[b] = php[2 * r * nq + 1];
+            }
+            const int m = rows - r0;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                if (b < m) {
+                    re = re + v[b] * pre[b];
+                    im = im + v[b] * pim[b];
+                }
+            }
         }
         acc[0] = re;
         acc[1] = im;

@@ -185,6 +185,8 @@ def _check_single_hip_runtime() -> None:
 def load(path: Path = LIB_PATH):
     """The ctypes library with signatures set (no device initialisation)."""
     global _lib
+    if _lib is not None:
+        return _lib
     with _lock:
         if _lib is not None:
             return _lib

@@ -59,6 +59,17 @@ class _Buffer:
         weakref.finalize(self, _free, lib, self.ptr)
 
 
+# Host writes into any device buffer so far (every ``_Buffer.writes`` bump adds one):
+# a plan that checked its buffers at epoch e skips the per-buffer comparison while the
+# epoch is still e.
+_write_epoch = 0
+
+
+def host_write_epoch() -> int:
+    """Host writes into device buffers so far, over all buffers of the process."""
+    return _write_epoch
+
+
 def _free(lib, ptr):
     try:
         lib.etl_free(C.c_void_p(ptr))
@@ -123,7 +134,9 @@ class DeviceArray:
         host = np.ascontiguousarray(host, dtype=self.dtype)
         if host.shape != self.shape:
             host = np.ascontiguousarray(np.broadcast_to(host, self.shape))
+        global _write_epoch
         self._buf.writes += 1
+        _write_epoch += 1
         if self.nbytes:
             _lib.call("etl_memcpy_h2d", C.c_void_p(self.ptr), host.ctypes.data, self.nbytes)
 

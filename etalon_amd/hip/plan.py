@@ -15,6 +15,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from etalon_amd.hip import _lib
+from etalon_amd.hip import backend as _backend
 from etalon_amd.hip.backend import DeviceArray, _precision_of, zeros
 from etalon_amd.state import FIELDS, PROFILES
 
@@ -77,6 +78,8 @@ class FusedPlan:
                                                           getattr(s, name).dtype)]
                            for name in FIELDS} for s in self.states]
         self.pairs = self.pair_sets[0]
+        # the first simulation's seven arrays (FIELDS order) in buffer 0 and in buffer 1
+        self.arrays = [tuple(self.pairs[name][b] for name in FIELDS) for b in (0, 1)]
         descs = (_lib.EtlTeDesc * len(self.states))()
         for k, (s, pair) in enumerate(zip(self.states, self.pair_sets)):
             descs[k] = _describe(s, pair)
@@ -89,6 +92,12 @@ class FusedPlan:
         self.parity = 0
         self.n_src = [0] * len(self.states)
         self._cb_marks = self._material_marks()
+        self._epoch = _backend.host_write_epoch()
+        # the step entry point and its parity output, looked up once (advance is
+        # called once per step block by interactive loops)
+        self._advance_rows = _lib.load().etl_plan_advance_rows
+        self._parity_out = C.c_int()
+        self._parity_ref = C.byref(self._parity_out)
 
     def _material_marks(self) -> tuple:
         return tuple((s.cb_ex.writes, s.cb_ey.writes) for s in self.states)
@@ -96,10 +105,14 @@ class FusedPlan:
     def _check_materials(self) -> None:
         """cb_ex or cb_ey written from the host since the plan laid out its tiles: lay
         them out again (uniform-material tiles read the cb values of the layout)."""
+        epoch = _backend.host_write_epoch()
+        if epoch == self._epoch:  # no host write into any buffer since the last check
+            return
         marks = self._material_marks()
         if marks != self._cb_marks:
             _lib.call("etl_plan_touch", self.handle)
             self._cb_marks = marks
+        self._epoch = epoch
 
     def close(self) -> None:
         if self.handle:
@@ -160,10 +173,10 @@ class FusedPlan:
     def advance(self, n0: int, count: int, probe_row0: int = 0) -> int:
         """Steps [n0, n0 + count); probes write rows probe_row0 .. probe_row0 + count - 1."""
         self._check_materials()
-        parity = C.c_int()
-        _lib.call("etl_plan_advance_rows", self.handle, int(n0), int(count), int(probe_row0),
-                  C.byref(parity))
-        self.parity = parity.value
+        rc = self._advance_rows(self.handle, n0, count, probe_row0, self._parity_ref)
+        if rc:
+            raise _lib.HipError("etl_plan_advance_rows", rc, _lib.last_error(_lib.load()))
+        self.parity = self._parity_out.value
         return self.parity
 
     def set_checkpoints(self, arrays: Sequence, n_seg: int, sim: int = 0) -> None:

@@ -17,6 +17,7 @@ SI; arrays are cast once to the run's precision.
 from __future__ import annotations
 
 import functools
+import operator
 import time
 from typing import Any, Optional
 
@@ -27,10 +28,13 @@ from etalon_amd.backends import get_backend
 from etalon_amd.capabilities import check_supported
 from etalon_amd.cpml import cpml_axis
 from etalon_amd.grid import build_yee_grid, permittivity_on_nodes
+from etalon_amd.hip import backend as _hip_backend
 from etalon_amd.result import SimulationResult, monitor_data
 from etalon_amd.sampling import dft_phasors, monitor_samplings
 from etalon_amd.sources import point_injection
 from etalon_amd.state import FIELDS, build_te_state
+
+_state_fields = operator.attrgetter(*FIELDS)  # a state's seven arrays, FIELDS order
 
 # Steps per fused-plan table refill (source deltas and phasors): the first refill of an
 # advance covers _FIRST_TABLE_ROWS steps and each next one 4x more, up to
@@ -101,6 +105,7 @@ class FDTDSolver:
         self._plan_ids: list[int] = []
         self._plan_marks: tuple = ()
         self._plan_tables = (0, 0)
+        self._plan_epoch = -1  # host-write epoch at which the plan's buffers were checked
 
     # ---------------------------------------------------------------- one step
     def _inject(self, magnetic: bool) -> None:
@@ -239,6 +244,20 @@ class FDTDSolver:
         """
         if count <= 0:
             return
+        plan = self._plan
+        if plan is not None:
+            # The common case of a step loop: the same plan, no host write into any
+            # device buffer since it was checked, its tables covering the steps, the
+            # state in its buffers — one library call.
+            n, parity = self.n, plan.parity
+            lo, hi = self._plan_tables
+            if (self._plan_epoch == _hip_backend.host_write_epoch() and lo <= n
+                    and n + count <= hi
+                    and all(map(operator.is_, _state_fields(self.state), plan.arrays[parity]))):
+                if plan.advance(n, count) != parity:
+                    plan.bind()
+                self.n = n + count
+                return
         if not self.fused_supported():
             for _ in range(count):
                 self.step()
@@ -253,6 +272,7 @@ class FDTDSolver:
             for _ in range(count):
                 self.step()
             return
+        self._plan_epoch = _hip_backend.host_write_epoch()
         start, done, refills = self.n, 0, 0
         grow = _FIRST_TABLE_ROWS
         while done < count:

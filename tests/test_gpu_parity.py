@@ -245,6 +245,33 @@ TUNES_F32 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16), (1, 4, 16)
              (2, 1, 8), (2, 2, 8), (2, 4, 8)]
 TUNES_F64 = [(1, 1, 8), (1, 2, 8), (1, 4, 8), (1, 1, 16), (1, 2, 16), (1, 4, 16)]
 # launch 3: one kernel per tile kind on three streams; 1: one kernel per step block
+@pytest.mark.parametrize(("precision", "dtype"), PRECISIONS)
+def test_step_loop_of_short_advances_bitwise(hip, precision, dtype):
+    """An interactive loop of short advances (1 to 20 steps, the one-call path of a
+    checked plan, whose step counter each advance finds where the previous one left
+    it), broken by a timed block and a reset; every run ends bit for bit at the
+    oracle's state."""
+    from etalon_amd.solver import FDTDSolver
+
+    sim = full_sim(precision)
+    gpu = FDTDSolver(sim)
+    cpu = _oracle_full(precision, 600)
+    counts = [1, 2, 3, 5, 20, 7, 1, 11, 13, 17] * 4 + [18]  # 378 steps
+    for count in counts:
+        gpu.advance(count)
+    lo, hi = gpu._plan_tables
+    assert lo <= gpu.n and gpu.n + 24 <= hi
+    gpu._plan.time(gpu.n, 24)  # a timed block of direct launches on the same tables
+    gpu._plan.bind()
+    gpu.n += 24
+    gpu.advance(600 - gpu.n)
+    _compare_solver(gpu, cpu)
+    gpu.reset()
+    for count in counts + [600 - sum(counts)]:
+        gpu.advance(count)
+    _compare_solver(gpu, cpu)
+
+
 SHAPES = [("single", np.float32, *s, 3) for s in TUNES_F32] + \
     [("double", np.float64, *s, 3) for s in TUNES_F64] + \
     [(p, d, 1, t, 8, 1) for p, d in (("single", np.float32), ("double", np.float64))
