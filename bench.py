@@ -94,6 +94,15 @@ PMC_KIB = {("single", 1, 6, 16, 1): (39156.1, 49862.2),
 FETCH_SCALE, WRITE_SCALE = 1.99985, 1.0
 PMC_KIB_8024 = {("single", 8, 16): (439709.6, 795824.4),
                 ("double", 6, 16): (902790.5, 1563610.3)}
+# VALU wave-instructions one launch of the same build issues (rocprofv3 SQ_INSTS_VALU,
+# profiles/joint_kernel_sq_counters.md), keyed by (precision, temporal, band_rows, grid
+# rows).  A CU issues at most one wave64 float VALU instruction per cycle (four SIMDs of
+# 16 lanes, four cycles each, float32 and float64 alike), at up to 2.4 GHz: the count
+# over CUs x clock is the launch's VALU issue time, the time it could not go below with
+# the instructions it has.
+SQ_VALU = {("single", 6, 16, 2024): 12986303, ("double", 6, 16, 2024): 15597430,
+           ("single", 8, 16, 8024): 280114133}
+VALU_CLOCK_HZ = 2.4e9
 # Infinity Cache (MALL) of the MI355X: a launch whose working set fits it has part of
 # its L2 misses served there, not by HBM.
 MALL_BYTES = 256 * 2**20
@@ -252,6 +261,9 @@ def roofline(precision: str, r: dict, traffic: "int | None") -> dict:
     }
     if traffic is not None:
         roof["traffic_over_compulsory"] = round(traffic / compulsory, 3)
+    valu = _valu_issue(precision, r, kernel_s)
+    if valu is not None:
+        roof["valu_issue"] = valu
     # both buffers of Ex, Ey and Hz, cb_ex and cb_ey (the CPML strips' psi arrays aside)
     working_set = tm["kernel_cells"] * BYTES_PER_CELL[precision]
     if working_set < MALL_BYTES:
@@ -262,6 +274,28 @@ def roofline(precision: str, r: dict, traffic: "int | None") -> dict:
                               "instead of HBM; hbm_resident is the grid whose launches stream "
                               "from HBM")
     return roof
+
+
+def _valu_issue(precision: str, r: dict, kernel_s: float) -> "dict | None":
+    """The launch's VALU issue time (SQ_INSTS_VALU of this build over CUs x clock)
+    against its duration, when the SQ table holds this shape."""
+    import re
+
+    tune = r["tune"]
+    insts = SQ_VALU.get((precision, tune["temporal"], tune["band_rows"], r["cells"][0]))
+    cus = re.search(r"CUs=(\d+)", r.get("device", ""))
+    if insts is None or cus is None or kernel_build_id() != PMC_BUILD:
+        return None
+    issue_s = insts / (int(cus.group(1)) * VALU_CLOCK_HZ)
+    return {
+        "insts_per_launch": insts,
+        "issue_us": round(issue_s * 1e6, 2),
+        "frac": round(issue_s / kernel_s, 4),
+        "note": "VALU wave-instructions of one launch (rocprofv3 SQ_INSTS_VALU of this build) "
+                "at one per CU per cycle and 2.4 GHz, over the launch's event-timed "
+                "duration: how close the launch runs to the issue limit of its own "
+                "instructions",
+    }
 
 
 def hbm_resident(precision: str, steps: int) -> dict:
