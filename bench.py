@@ -88,12 +88,12 @@ REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
 # The counts belong to one build of the kernels: PMC_BUILD is the fingerprint of the
 # sources they were measured with (kernel_build_id), and a build with another
 # fingerprint reports no measured traffic until the counters are taken again.
-PMC_BUILD = "996121ae896827e4"
-PMC_KIB = {("single", 1, 6, 16, 1): (39156.1, 49862.2),
-           ("double", 1, 6, 16, 1): (93238.7, 106642.7)}
+PMC_BUILD = "e93db92e7640a816"
+PMC_KIB = {("single", 1, 6, 16, 1): (38766.8, 49955.1),
+           ("double", 1, 6, 16, 1): (92682.6, 106558.8)}
 FETCH_SCALE, WRITE_SCALE = 1.99985, 1.0
-PMC_KIB_8024 = {("single", 8, 16): (439709.6, 795824.4),
-                ("double", 6, 16): (902790.5, 1563610.3)}
+PMC_KIB_8024 = {("single", 8, 16): (440319.7, 795706.3),
+                ("double", 6, 16): (901522.7, 1563338.8)}
 # VALU wave-instructions one launch of the same build issues (rocprofv3 SQ_INSTS_VALU,
 # profiles/joint_kernel_sq_counters.md), keyed by (precision, temporal, band_rows, grid
 # rows).  A CU issues at most one wave64 float VALU instruction per cycle (four SIMDs of

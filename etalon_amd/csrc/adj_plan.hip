@@ -559,24 +559,30 @@ int etl_adjplan_sources(etl_adjplan_t p, int sim, int n, const int* comp, const 
         ETL_REQUIRE(!(comp[order[k]] == comp[order[k - 1]] && flat[order[k]] == flat[order[k - 1]]),
                     "etl_adjplan_sources: node listed twice (sum its columns)");
     const size_t el = p->elem();
-    auto sorted_table = [&]() {
-        std::vector<unsigned char> sorted(size_t(rows) * n * el);
-        const unsigned char* tb = static_cast<const unsigned char*>(table);
+    bool in_order = true;  // the caller's columns already in node order (the usual case)
+    for (int k = 0; k < n; ++k) in_order = in_order && order[k] == k;
+    // The table's columns in node order: the caller's table itself when they are, else
+    // a copy with the columns gathered row by row.
+    std::vector<unsigned char> gathered;
+    auto sorted_table = [&]() -> const void* {
+        if (in_order) return table;
+        gathered.resize(size_t(rows) * n * el);
+        const size_t w = el / 4;  // 32-bit words per REAL
+        const uint32_t* tb = static_cast<const uint32_t*>(table);
+        uint32_t* out = reinterpret_cast<uint32_t*>(gathered.data());
         for (int r = 0; r < rows; ++r)
             for (int k = 0; k < n; ++k)
-                std::memcpy(&sorted[(size_t(r) * n + k) * el], tb + (size_t(r) * n + order[k]) * el,
-                            el);
-        return sorted;
+                for (size_t b = 0; b < w; ++b)
+                    out[(size_t(r) * n + k) * w + b] = tb[(size_t(r) * n + order[k]) * w + b];
+        return gathered.data();
     };
+    const size_t table_bytes = size_t(rows) * n * el;
     // The same nodes and rows as before (the next gradient of a design loop): only the
     // table changes, in place; the layout and the graphs stay.
     bool same = s.src != nullptr && n > 0 && rows == s.src_rows && size_t(n) == s.comp.size();
     for (int k = 0; same && k < n; ++k)
         same = s.comp[k] == comp[order[k]] && s.flat[k] == flat[order[k]];
-    if (same) {
-        const std::vector<unsigned char> sorted = sorted_table();
-        return etl_memcpy_h2d(s.src, sorted.data(), sorted.size());
-    }
+    if (same) return etl_memcpy_h2d(s.src, sorted_table(), table_bytes);
     if (int rc_ = sync_lanes()) return rc_;
     s.comp.resize(n);
     s.flat.resize(n);
@@ -589,8 +595,7 @@ int etl_adjplan_sources(etl_adjplan_t p, int sim, int n, const int* comp, const 
     s.src = nullptr;
     s.src_rows = rows;
     if (n && rows) {
-        const std::vector<unsigned char> sorted = sorted_table();
-        int rc = upload_bytes(&s.src, sorted.data(), sorted.size());
+        int rc = upload_bytes(&s.src, sorted_table(), table_bytes);
         if (rc) return rc;
     }
     p->dirty = true;
@@ -622,6 +627,9 @@ int etl_adjplan_run_rows(etl_adjplan_t p, int64_t n_first, int count, int hist_r
     if ((rc = adj_prepare_run(p, n_first, count, hist_row0))) return rc;
     hipStream_t s = lane_stream(p->lane);
     if (p->graph == nullptr || p->graph_count != count || p->graph_parity != p->parity) {
+        // the previous graph's launches may still be queued: it is destroyed only once
+        // its work is done
+        if (p->graph && (rc = sync_lanes())) return rc;
         free_adj_graph(p);
         ETL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         int launches = 0;

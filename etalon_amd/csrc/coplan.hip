@@ -146,7 +146,12 @@ int co_run(EtlCoPlan* c, int64_t n, int count_f, int probe_row0, int64_t n_first
             e.apar == ai.parity && e.fgen == fi.gen && e.agen == ai.gen)
             hit = &e;
     if (hit == nullptr) {
-        if (c->graphs.size() >= 8) free_co_graphs(c);
+        if (c->graphs.size() >= 8) {
+            // the cached graphs' launches may still be queued: a graph is destroyed
+            // only once its work is done
+            if ((rc = sync_lanes())) return rc;
+            free_co_graphs(c);
+        }
         EtlCoPlan::Entry e{count_f, count_a, fi.parity, ai.parity, fi.gen, ai.gen,
                            nullptr, nullptr, 0, 0};
         ETL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -1237,7 +1237,9 @@ int etl_plan_destroy(etl_plan_t p) {
 int etl_plan_touch(etl_plan_t p) {
     ETL_CHECK_INIT();
     ETL_REQUIRE(p != nullptr, "etl_plan_touch: null plan");
-    p->dirty = true;
+    // only the uniform-material classification depends on the cb values: a plan
+    // without it reads the cb rows everywhere and keeps its layout and graphs
+    if (p->tune.uniform_cb == 1) p->dirty = true;
     return ETL_OK;
 }
 

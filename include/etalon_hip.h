@@ -208,8 +208,8 @@ int etl_plan_create_batch(const EtlTeDesc* descs, int n_sims, const EtlTune* tun
                           etl_plan_t* out);
 int etl_plan_destroy(etl_plan_t plan);
 /* The plan's cb_ex or cb_ey arrays were written: lay out the tiles again (the uniform-
- * material tiles, EtlTune.uniform_cb) before the next step.  No reference counterpart
- * (the reference has no plan). */
+ * material tiles, EtlTune.uniform_cb) before the next step; nothing to do for a plan
+ * without them.  No reference counterpart (the reference has no plan). */
 int etl_plan_touch(etl_plan_t plan);
 /* Point sources: node (comp, i, j) receives field -= delta[n][k] in step n; a node
  * listed more than once is updated once per listing, in list order. */
