@@ -88,12 +88,12 @@ REFERENCE_GCELLS = {"single": 2.944929933, "double": 1.5}
 # The counts belong to one build of the kernels: PMC_BUILD is the fingerprint of the
 # sources they were measured with (kernel_build_id), and a build with another
 # fingerprint reports no measured traffic until the counters are taken again.
-PMC_BUILD = "e93db92e7640a816"
-PMC_KIB = {("single", 1, 6, 16, 1): (38766.8, 49955.1),
-           ("double", 1, 6, 16, 1): (92682.6, 106558.8)}
+PMC_BUILD = "8349b2dbe7a09e00"
+PMC_KIB = {("single", 1, 6, 16, 1): (39201.9, 49890.3),
+           ("double", 1, 6, 16, 1): (93115.3, 106755.8)}
 FETCH_SCALE, WRITE_SCALE = 1.99985, 1.0
-PMC_KIB_8024 = {("single", 8, 16): (440319.7, 795706.3),
-                ("double", 6, 16): (901522.7, 1563338.8)}
+PMC_KIB_8024 = {("single", 8, 16): (440256.2, 795735.5),
+                ("double", 6, 16): (902089.5, 1563518.7)}
 # VALU wave-instructions one launch of the same build issues (rocprofv3 SQ_INSTS_VALU,
 # profiles/joint_kernel_sq_counters.md), keyed by (precision, temporal, band_rows, grid
 # rows).  A CU issues at most one wave64 float VALU instruction per cycle (four SIMDs of

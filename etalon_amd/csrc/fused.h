@@ -180,9 +180,18 @@ template <typename R>
 int launch_trace_dft(const TraceSim<R>* sims, int rows, int* counter, const int4* pairs,
                      const int* pair_sim, int n_pairs, R* const* accs, hipStream_t stream);
 int launch_bump(int* counter, int by, hipStream_t stream);
-// out[(counter[0] - counter[2] + r) * n + k] = trace[r * n_slots + slot_base + k], r < rows.
+// A time-domain probe's copy: its n trace slots from slot_base of a simulation's trace
+// (n_slots REALs per row) to rows of `out` (n REALs per row).
+struct ProbeCopy {
+    const void* trace;
+    void* out;
+    int n_slots, slot_base, n, pad;
+};
+// For every probe d of tab[0 .. n_probes): out[(counter[0] - counter[2] + r) * n + k] =
+// trace[r * n_slots + slot_base + k], r < rows — one launch for all of them (max_n: the
+// largest n).
 template <typename R>
-int launch_probe_copy(const R* trace, int n_slots, int slot_base, int n, R* out, int rows,
-                      const int* counter, hipStream_t stream);
+int launch_probe_copies(const ProbeCopy* tab, int n_probes, int max_n, int rows,
+                        const int* counter, hipStream_t stream);
 
 }  // namespace etl

@@ -180,6 +180,8 @@ struct EtlPlan {
     int n_interior = 0, n_strip = 0, n_extra = 0, n_extra_strip = 0;
     int n_uniform = 0;               // the last interior tiles, of uniform material
     void* uvals = nullptr;           // their cb_ex, cb_ey: REAL pairs in tile order
+    ProbeCopy* probe_tab = nullptr;  // every probe's copy, all simulations
+    int n_probe_tab = 0, probe_max_n = 0;
     int n_strip_y = 0, n_strip_x = 0;  // the strip tiles: y only, then x only, then both
     int64_t interior_cells = 0;      // Hz cells the interior tiles own
     int4* pairs = nullptr;           // DFT pairs of every simulation
@@ -242,6 +244,9 @@ void free_layout(EtlPlan* p) {
     p->accs = nullptr;
     p->uvals = nullptr;
     p->n_uniform = 0;
+    etl_free(p->probe_tab);
+    p->probe_tab = nullptr;
+    p->n_probe_tab = p->probe_max_n = 0;
 }
 
 int upload(void** dst, const void* src, size_t bytes) {
@@ -470,6 +475,21 @@ int build_layout(EtlPlan* p) {
             (rc = etl_malloc(&sim.trace, size_t(p->tune.trace_chunk) * sim.n_slots * p->elem())))
             return rc;
     }
+    // the probes' copies (the trace buffers are this layout's)
+    {
+        std::vector<ProbeCopy> pc;
+        for (const auto& sim : p->sims)
+            for (const auto& pr : sim.probes) {
+                pc.push_back(ProbeCopy{sim.trace, pr.out, sim.n_slots, pr.slot_base,
+                                       int(pr.flat.size()), 0});
+                p->probe_max_n = std::max(p->probe_max_n, int(pr.flat.size()));
+            }
+        p->n_probe_tab = int(pc.size());
+        int rc_p;
+        if (!pc.empty() &&
+            (rc_p = upload((void**)&p->probe_tab, pc.data(), pc.size() * sizeof(ProbeCopy))))
+            return rc_p;
+    }
     // uniform-material interior tiles last, with their two cb values
     if (p->tune.uniform_cb > 0 && !interior.empty()) {
         int rc_u;
@@ -592,13 +612,9 @@ int upload_args(EtlPlan* p) {
 // rows (which advances the step counter) or a counter bump.
 template <typename R>
 int after_chunk(EtlPlan* p, int count, hipStream_t s) {
-    for (const auto& sim : p->sims)
-        for (const auto& pr : sim.probes) {
-            const int rc = launch_probe_copy<R>((const R*)sim.trace, sim.n_slots, pr.slot_base,
-                                                int(pr.flat.size()), (R*)pr.out, count,
-                                                p->counter, s);
-            if (rc) return rc;
-        }
+    if (int rc = launch_probe_copies<R>(p->probe_tab, p->n_probe_tab, p->probe_max_n, count,
+                                        p->counter, s))
+        return rc;
     if (p->n_pairs && count)
         return launch_trace_dft<R>(static_cast<const TraceSim<R>*>(p->trace_sims_dev), count,
                                    p->counter, p->pairs, p->pair_sim, p->n_pairs,

@@ -86,15 +86,18 @@ __global__ __launch_bounds__(256) void trace_dft_kernel(const TraceSim<R>* __res
     }
 }
 
+// One probe per blockIdx.y; its rows x n values spread over blockIdx.x.
 template <typename R>
-__global__ void probe_copy_kernel(const R* __restrict__ trace, int n_slots, int slot_base, int n,
-                                  R* __restrict__ out, int rows, const int* __restrict__ counter) {
+__global__ void probe_copy_kernel(const ProbeCopy* __restrict__ tab, int rows,
+                                  const int* __restrict__ counter) {
+    const ProbeCopy d = tab[blockIdx.y];
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= int64_t(rows) * n) return;
-    const int r = int(t / n);
-    const int k = int(t - int64_t(r) * n);
+    if (t >= int64_t(rows) * d.n) return;
+    const int r = int(t / d.n);
+    const int k = int(t - int64_t(r) * d.n);
     const int64_t base = int64_t(counter[0]) - counter[2];
-    out[(base + r) * n + k] = trace[int64_t(r) * n_slots + slot_base + k];
+    static_cast<R*>(d.out)[(base + r) * d.n + k] =
+        static_cast<const R*>(d.trace)[int64_t(r) * d.n_slots + d.slot_base + k];
 }
 
 __global__ void bump_counter_kernel(int* counter, int by) {
@@ -114,12 +117,13 @@ int launch_trace_dft(const TraceSim<R>* sims, int rows, int* counter, const int4
 }
 
 template <typename R>
-int launch_probe_copy(const R* trace, int n_slots, int slot_base, int n, R* out, int rows,
-                      const int* counter, hipStream_t stream) {
-    const int64_t total = int64_t(rows) * n;
-    if (total == 0) return ETL_OK;
-    hipLaunchKernelGGL(probe_copy_kernel<R>, dim3(unsigned((total + 255) / 256)), dim3(256), 0,
-                       stream, trace, n_slots, slot_base, n, out, rows, counter);
+int launch_probe_copies(const ProbeCopy* tab, int n_probes, int max_n, int rows,
+                        const int* counter, hipStream_t stream) {
+    const int64_t total = int64_t(rows) * max_n;
+    if (total == 0 || n_probes == 0) return ETL_OK;
+    ETL_REQUIRE(n_probes <= 65535, "probe copy: %d probes in one plan (at most 65535)", n_probes);
+    hipLaunchKernelGGL(probe_copy_kernel<R>, dim3(unsigned((total + 255) / 256), unsigned(n_probes)),
+                       dim3(256), 0, stream, tab, rows, counter);
     ETL_LAUNCHED();
     return ETL_OK;
 }
@@ -130,10 +134,8 @@ int launch_bump(int* counter, int by, hipStream_t stream) {
     return ETL_OK;
 }
 
-template int launch_probe_copy<float>(const float*, int, int, int, float*, int, const int*,
-                                      hipStream_t);
-template int launch_probe_copy<double>(const double*, int, int, int, double*, int, const int*,
-                                       hipStream_t);
+template int launch_probe_copies<float>(const ProbeCopy*, int, int, int, const int*, hipStream_t);
+template int launch_probe_copies<double>(const ProbeCopy*, int, int, int, const int*, hipStream_t);
 template int launch_trace_dft<float>(const TraceSim<float>*, int, int*, const int4*, const int*,
                                      int, float* const*, hipStream_t);
 template int launch_trace_dft<double>(const TraceSim<double>*, int, int*, const int4*,
