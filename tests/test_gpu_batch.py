@@ -46,8 +46,6 @@ def assert_same(a, b, what):
                                    (1, 4, 16, 1, 1), (1, 6, 16, 1, 1), (1, 6, 8, 1, 1)])
 def test_batch_equals_each_simulation_alone(hip, precision, shape):
     vec, depth, band, waves, launch = shape
-    if precision == "double" and band == 16 and launch == 1:
-        pytest.skip("16-row interior tiles in one launch are compiled for float only")
     sims = small_batch(precision)
     print(f"batch of {len(sims)} {precision}, shape vec={vec} T={depth} band={band} waves={waves} "
           f"launch={launch}", flush=True)
@@ -92,3 +90,48 @@ def test_run_batch_groups_by_step_count(hip):
     for name in solo.monitors:
         for comp, data in solo[name].items():
             assert bitwise_equal(results[3][name][comp].values, data.values)
+
+
+def _probe_run(solvers, probes, steps):
+    """Advance ``solvers`` (each its own simulation) in one fused plan for ``steps``
+    steps with the time-domain probe ``probes[k]`` = (component, flat nodes) on
+    simulation k; returns each probe's (steps, nodes) rows."""
+    from etalon_amd.hip.backend import zeros
+    from etalon_amd.hip.plan import FusedPlan
+
+    plan = FusedPlan([s.state for s in solvers])
+    outs = []
+    for k, (s, (comp, flat)) in enumerate(zip(solvers, probes)):
+        ids = s.register(plan, k)
+        delta, phasors = s._tables(0, steps)
+        if delta is not None:
+            plan.source_table(0, delta, sim=k)
+        for mon_id, table in zip(ids, phasors):
+            if table is not None:
+                plan.phasor_table(mon_id, 0, table, sim=k)
+        out = zeros((steps, flat.size), s.real)
+        plan.add_probe(comp, flat, out, sim=k)
+        outs.append(out)
+    plan.advance(0, steps)
+    rows = [o.get() for o in outs]
+    plan.close()
+    return rows
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_probes_of_every_simulation_in_one_launch(hip, precision):
+    """Time-domain probes of different sizes on three simulations of one plan — one
+    launch copies all of them after each trace chunk, graph-replayed chunks and the
+    directly launched tail alike — hold, row for row and bit for bit, what the same
+    probe records on each simulation's own plan."""
+    sims = small_batch(precision, n=3, steps=400)
+    probes = [("Hz", np.arange(37, dtype=np.int64) * 3 + 200),
+              ("Ex", np.arange(5, dtype=np.int64) * 11 + 300),
+              ("Ey", np.arange(61, dtype=np.int64) + 1000)]
+    steps = 400  # trace chunks of 64 (graphs) and a tail of 16
+    together = _probe_run([FDTDSolver(s) for s in sims], probes, steps)
+    for k, sim in enumerate(sims):
+        alone_rows = _probe_run([FDTDSolver(sim)], [probes[k]], steps)[0]
+        assert np.abs(alone_rows).max() > 0, f"probe {k} saw nothing"
+        assert bitwise_equal(together[k], alone_rows), f"probe {k} differs"
+
